@@ -1,0 +1,171 @@
+"""Benchmark: direct-sum gravity on one MI355X at N = 262,144 (two Plummer spheres).
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 2] [--no-cpu]
+
+Prints ONE JSON line.  A step is one fused force + kick-drift pass over all N bodies
+(N^2 pairwise interactions, 20 flops each by the usual convention).  Inputs are resident
+in HBM before the timed region; the region is bracketed by device synchronisation, and
+the force kernel's own duration is measured with HIP events on the library's stream.
+The CPU baseline (oracle restatement of the reference brute-force loop) is timed on a
+bounded subset of rows on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "G-interactions/s + % FP32 peak at N=262144; served steps/s, p50/p99 latency"
+FP32_PEAK_TFLOPS = 157.3          # MI355X published FP32 vector peak
+FLOPS_PER_INTERACTION = 20        # GPU Gems 3 ch. 31 convention
+N_C2 = 262144
+EPS_C2 = 0.005
+DT_C2 = 5e-4
+SEED = 2026
+CPU_THREADS_MAX = 16              # the GPU box's CPU share for one GPU
+CPU_TARGET_S = 10.0
+
+
+def cpu_baseline(bodies: np.ndarray, eps: float) -> dict:
+    """Oracle restatement of the reference brute-force loop (source/solvers/
+    cpu_brute_force.h:26-40) on a seeded subset of rows x all N columns."""
+    from oracle import oracle as O  # the CPU baseline leg is the only oracle user here
+    if not os.path.exists(O.LIB_PATH):
+        O.build(ref=False)
+    threads = max(1, min(CPU_THREADS_MAX, os.cpu_count() or 1))
+    n = bodies.shape[0]
+    rng = np.random.default_rng(SEED)
+    work = bodies.copy()
+    rows = 256
+    t = 0.0
+    while True:
+        sel = rng.choice(n, rows, replace=False).astype(np.uint32)
+        t0 = time.perf_counter()
+        O.brute_force_rows(work, sel, G=1.0, eps=eps, threads=threads)
+        t = time.perf_counter() - t0
+        if t >= 0.5 or rows >= n:
+            break
+        rows = min(n, rows * 4)
+    # scale the final sample to roughly CPU_TARGET_S
+    rows = int(min(n, max(rows, rows * CPU_TARGET_S / max(t, 1e-6))))
+    sel = rng.choice(n, rows, replace=False).astype(np.uint32)
+    t0 = time.perf_counter()
+    O.brute_force_rows(work, sel, G=1.0, eps=eps, threads=threads)
+    t = time.perf_counter() - t0
+    rate = rows * n / t
+    return {"value": rate / 1e9, "unit": "G-interactions/s", "cores": threads, "kind": "port",
+            "sample": f"{rows} seeded rows x all {n} bodies, 1 force pass, {t:.1f} s, "
+                      f"oracle/nbody_oracle.c brute force (-O2, {threads} threads)"}
+
+
+def load_traffic(name: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def run_direct(args) -> dict:
+    from nbody_amd import _lib, ics
+    n = N_C2
+    bodies = ics.two_plummer(n, seed=SEED)
+    p = _lib.params(dt=DT_C2, eps=EPS_C2, size=10000.0, wrap=1, G=1.0)
+    ctx = _lib.Context(0)
+    ctx.upload_bodies(bodies)
+    e0 = ctx.diagnostics(p)
+    ctx.step(p, args.warmup)
+    ctx.synchronize()
+
+    t0 = time.perf_counter()
+    ctx.timer_start()
+    ctx.step(p, args.steps)
+    ms_events, launches = ctx.timer_stop()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+
+    e1 = ctx.diagnostics(p)
+    # accuracy: forces at the final positions vs FP64 on 1024 seeded bodies
+    ctx.accelerate(p)
+    sample = np.random.default_rng(SEED).choice(n, 1024, replace=False).astype(np.uint32)
+    _, rel = ctx.force_error_f64(p, sample)
+    ctx.close()
+
+    inter = float(n) * n
+    per_launch_ms = ms_events / max(launches, 1)
+    value = inter * args.steps / (ms_events / 1e3) / 1e9
+    achieved_tflops = FLOPS_PER_INTERACTION * inter / (per_launch_ms / 1e3) / 1e12
+    E0 = e0[0] + e0[1]
+    E1 = e1[0] + e1[1]
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "G-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_events / args.steps,
+        "wall_ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "pct_fp32_peak": 100.0 * achieved_tflops / FP32_PEAK_TFLOPS,
+        "config": {
+            "workload": "direct-sum two-Plummer collision, N=262144, eps=0.005, dt=5e-4, "
+                        "fused force+kick-drift step, FP32 with TwoSum-compensated force sum",
+            "n_bodies": n, "eps": EPS_C2, "dt": DT_C2, "seed": SEED,
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved_tflops,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / FP32_PEAK_TFLOPS,
+            "traffic": load_traffic("traffic_direct_n262144.json"),
+            "kernel": "k_force (fused step)",
+            "kernel_ms": per_launch_ms,
+            "launches": launches,
+        },
+        "accuracy": {
+            "rel_err_vs_f64_max": float(rel.max()),
+            "rel_err_vs_f64_median": float(np.median(rel)),
+            "sample": "1024 seeded bodies, FP64 direct sum on the device",
+            "energy_drift": float((E1 - E0) / abs(E0)),
+            "energy_steps": args.warmup + args.steps,
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(bodies, EPS_C2)
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the multi-GPU "
+              "sharding of this solver is a design, not a build; see DESIGN.md",
+              file=sys.stderr)
+        return 2
+    out = run_direct(args)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

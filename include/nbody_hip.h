@@ -1,0 +1,184 @@
+/*
+ * nbody_hip.h -- C ABI of the MI355X (gfx950) gravity solver library libnbody_hip.so.
+ *
+ * This is the drop-in boundary between the stett/nbody simulation API (C++) and the
+ * hand-written HIP kernels.  It replaces the Vulkan device backend `nbody::GpuDevice`
+ * (reference source/gpu.h:135-353, source/gpu.cpp:76-1071) for one new solver that
+ * subclasses `nbody::Solver` (reference source/solver.h:17-75).  The C++ side of that
+ * solver lives in nbody_amd/host/ and only ever talks to the device through the
+ * functions declared here.
+ *
+ * Conventions
+ *  - Plain C types only: pointers, sizes, floats.  No HIP, torch or C++ types cross
+ *    this boundary, and no exception ever does.
+ *  - Every function that can fail returns an int status (NBHIP_OK == 0).  The text of
+ *    the last failure on a context is returned by nbhip_last_error(ctx).
+ *  - Body data crosses as the reference's split layout (reference source/gpu.h:32-48):
+ *    pos_mass   = float4 {x, y, z, mass}
+ *    vel_radius = float4 {vx, vy, vz, radius}
+ *    acc        = float4 {ax, ay, az, 0}
+ *    or as the reference's 48-byte AoS `nbody::Body` (reference include/nbody/body.h:6-19):
+ *    {pos[3], radius, vel[3], mass, acc[3], pad}.
+ *  - A context owns one HIP stream and all device buffers.  It is not thread safe: one
+ *    thread per context, like the reference API (reference source/sim.cpp:86-90).
+ *    Host-side calls that return data synchronise the context's stream; stepping calls
+ *    only enqueue work.
+ */
+#ifndef NBODY_HIP_H
+#define NBODY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------------- */
+#define NBHIP_OK              0
+#define NBHIP_ERR_INVALID     1   /* bad argument (null pointer, range, size)         */
+#define NBHIP_ERR_HIP         2   /* a HIP runtime call failed                        */
+#define NBHIP_ERR_NOMEM       3   /* device allocation failed                         */
+#define NBHIP_ERR_UNSUPPORTED 4   /* no usable gfx950 device / feature not available  */
+
+/* ---- modes (reference source/gpu.h:12 `enum class Mode { N2, NLogN }`) ------------ */
+#define NBHIP_MODE_N2     0       /* direct O(N^2) summation                           */
+#define NBHIP_MODE_NLOGN  1       /* Barnes-Hut octree                                 */
+
+/* ---- Barnes-Hut opening criterion ------------------------------------------------ */
+/* Reference (source/bhtree.cpp:125-128): a node is accepted when
+ *   |com - x|^2 > size^2 * theta^2         (larger theta = MORE accurate).          */
+#define NBHIP_OPEN_REFERENCE 0
+/* Textbook: accepted when size / |com - x| < theta (smaller theta = more accurate). */
+#define NBHIP_OPEN_TEXTBOOK  1
+
+/* ---- readback mask (reference source/gpu.h:52-59 `enum class Readback`) ---------- */
+#define NBHIP_RB_NONE          0u
+#define NBHIP_RB_POSITIONS     1u  /* pos_mass   */
+#define NBHIP_RB_VELOCITIES    2u  /* vel_radius */
+#define NBHIP_RB_ACCELERATIONS 4u  /* acc        */
+#define NBHIP_RB_ALL           7u
+
+typedef struct nbhip_ctx nbhip_ctx;
+
+/*
+ * Per-call simulation parameters.  The first fields are the reference push constants
+ * (source/gpu.h:17-27: dt, theta, G, size, wrap, mode); the body count is owned by the
+ * context.  Appended fields the reference has no counterpart for:
+ *   eps        Plummer softening length.  The law is
+ *                a_i += G m_j d / (|d|^2 + eps^2)^{3/2},  d = x_j - x_i,
+ *              and a pair contributes nothing when |d|^2 <= radius_i^2 (the reference
+ *              cutoff, source/detail/physics.h:28-41).  eps = 0 is the reference law.
+ *   opening    NBHIP_OPEN_REFERENCE or NBHIP_OPEN_TEXTBOOK (NLogN only).
+ *   leaf_size  maximum bodies per Barnes-Hut leaf; 1 reproduces the reference tree.
+ */
+typedef struct nbhip_params {
+    float dt;
+    float theta;
+    float G;
+    float size;
+    int   wrap;
+    int   mode;
+    float eps;
+    int   opening;
+    int   leaf_size;
+} nbhip_params;
+
+/* Fill `p` with the reference defaults (include/nbody/state.h:23-32: size 10000,
+ * theta 0.5, G 1, wrap on) and eps 0, N2 mode, reference opening, leaf size 1. */
+void nbhip_default_params(nbhip_params* p);
+
+/* ---- device discovery and lifetime ---------------------------------------------- */
+
+/* Cheap availability check, never fails loudly (reference GpuDevice::probe,
+ * source/gpu.h:142-148).  Returns NBHIP_OK when a gfx950 device is usable, else a
+ * non-zero status with a human-readable reason copied into `reason` (may be NULL). */
+int nbhip_probe(char* reason, size_t reason_cap);
+
+/* Bring up a context on HIP device `device` (reference GpuDevice::GpuDevice,
+ * source/gpu.cpp:76-113).  On failure *out is NULL and the reason is available from
+ * nbhip_last_error(NULL). */
+int nbhip_create(int device, nbhip_ctx** out);
+
+/* Release every device resource of the context.  NULL is a no-op. */
+void nbhip_destroy(nbhip_ctx* ctx);
+
+/* Text of the last failure on `ctx` (or of the last failed nbhip_create when ctx is
+ * NULL).  Empty string when there was none.  Valid until the next call on ctx. */
+const char* nbhip_last_error(const nbhip_ctx* ctx);
+
+/* ---- body storage (reference GpuDevice::reserve_bodies / map_bodies / download,
+ *      source/gpu.h:169-189) --------------------------------------------------------- */
+
+/* Set the body count to n, growing device storage if needed (it never shrinks).  A
+ * grow discards device contents; the caller must upload the whole range again. */
+int nbhip_reserve(nbhip_ctx* ctx, size_t n);
+
+/* Current body count. */
+size_t nbhip_count(const nbhip_ctx* ctx);
+
+/* Upload bodies [offset, offset+n) in the split layout.  Any of the three arrays may be
+ * NULL to leave that array untouched.  Each points at 4*n floats. */
+int nbhip_upload(nbhip_ctx* ctx, const float* pos_mass, const float* vel_radius,
+                 const float* acc, size_t offset, size_t n);
+
+/* Download the arrays named by `mask` for bodies [offset, offset+n).  Synchronises. */
+int nbhip_download(nbhip_ctx* ctx, unsigned mask, float* pos_mass, float* vel_radius,
+                   float* acc, size_t offset, size_t n);
+
+/* Upload n reference `Body` records (48 bytes each) and de-interleave them on the device
+ * (replaces the host-side de-interleave, source/solvers/gpu_solver_split.h:155-177).
+ * Sets the body count to n. */
+int nbhip_upload_bodies(nbhip_ctx* ctx, const void* bodies, size_t n);
+
+/* Re-interleave on the device and download min(n, count) reference `Body` records
+ * (replaces source/solvers/gpu_solver_split.h:188-221).  Synchronises. */
+int nbhip_download_bodies(nbhip_ctx* ctx, void* bodies, size_t n);
+
+/* ---- stepping (reference GpuDevice::accelerate / integrate / step,
+ *      source/gpu.h:191-197; Solver::accelerate/integrate/update, source/solver.h:57-59) */
+
+/* Accelerations of every body from the current positions (mode N2 or NLogN). */
+int nbhip_accelerate(nbhip_ctx* ctx, const nbhip_params* p);
+
+/* Semi-implicit Euler kick-drift with optional torus wrap, from the stored
+ * accelerations (reference detail::integrate_euler, source/detail/physics.h:60-70). */
+int nbhip_integrate(nbhip_ctx* ctx, const nbhip_params* p);
+
+/* `steps` full steps, each = accelerate from pre-step positions, then integrate.  In N2
+ * mode the kick-drift is fused into the force pass (ping-pong position buffers). */
+int nbhip_step(nbhip_ctx* ctx, const nbhip_params* p, int steps);
+
+/* Block until all work enqueued on the context has finished. */
+int nbhip_synchronize(nbhip_ctx* ctx);
+
+/* ---- diagnostics (FP64 reductions on the device) --------------------------------- */
+
+/* out[0] kinetic energy, out[1] potential energy (softened, with the radius cutoff),
+ * out[2..4] linear momentum, out[5..7] angular momentum about the origin, out[8] total
+ * mass.  Synchronises. */
+int nbhip_diagnostics(nbhip_ctx* ctx, const nbhip_params* p, double out[9]);
+
+/* Accuracy probe: for the bodies listed in `sample` (ns indices), recompute the
+ * acceleration by FP64 direct summation on the device from the current positions and
+ * write it to acc64 (3*ns doubles, may be NULL).  rel_err (ns doubles, may be NULL)
+ * receives |a_stored - a_f64| / |a_f64| against the stored FP32 accelerations.
+ * Synchronises. */
+int nbhip_force_error_f64(nbhip_ctx* ctx, const nbhip_params* p, const uint32_t* sample,
+                          size_t ns, double* acc64, double* rel_err);
+
+/* ---- timing: HIP events on the context's stream ----------------------------------- */
+
+/* Record a start event on the context's stream. */
+int nbhip_timer_start(nbhip_ctx* ctx);
+
+/* Record a stop event, wait for it, and return the elapsed milliseconds since the
+ * matching nbhip_timer_start, plus the number of force-kernel launches enqueued in
+ * between (launches may be NULL). */
+int nbhip_timer_stop(nbhip_ctx* ctx, float* ms, long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBODY_HIP_H */

@@ -1,0 +1,232 @@
+"""ctypes binding of libnbody_hip.so (the C ABI declared in include/nbody_hip.h).
+
+This module is plumbing for Python callers (tests, bench.py, the serving loop): it loads
+the in-tree library and fails loudly when it is missing.  There is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libnbody_hip.so")
+
+OK = 0
+ERR_INVALID = 1
+ERR_HIP = 2
+ERR_NOMEM = 3
+ERR_UNSUPPORTED = 4
+
+MODE_N2 = 0
+MODE_NLOGN = 1
+OPEN_REFERENCE = 0
+OPEN_TEXTBOOK = 1
+
+RB_NONE = 0
+RB_POSITIONS = 1
+RB_VELOCITIES = 2
+RB_ACCELERATIONS = 4
+RB_ALL = 7
+
+
+class Params(C.Structure):
+    """nbhip_params (include/nbody_hip.h)."""
+    _fields_ = [
+        ("dt", C.c_float), ("theta", C.c_float), ("G", C.c_float), ("size", C.c_float),
+        ("wrap", C.c_int), ("mode", C.c_int), ("eps", C.c_float), ("opening", C.c_int),
+        ("leaf_size", C.c_int),
+    ]
+
+
+# Every symbol the header declares, with (restype, argtypes).
+_vp = C.c_void_p
+_sz = C.c_size_t
+SIGNATURES = {
+    "nbhip_default_params": (None, [C.POINTER(Params)]),
+    "nbhip_probe": (C.c_int, [C.c_char_p, _sz]),
+    "nbhip_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "nbhip_destroy": (None, [_vp]),
+    "nbhip_last_error": (C.c_char_p, [_vp]),
+    "nbhip_reserve": (C.c_int, [_vp, _sz]),
+    "nbhip_count": (_sz, [_vp]),
+    "nbhip_upload": (C.c_int, [_vp, _vp, _vp, _vp, _sz, _sz]),
+    "nbhip_download": (C.c_int, [_vp, C.c_uint, _vp, _vp, _vp, _sz, _sz]),
+    "nbhip_upload_bodies": (C.c_int, [_vp, _vp, _sz]),
+    "nbhip_download_bodies": (C.c_int, [_vp, _vp, _sz]),
+    "nbhip_accelerate": (C.c_int, [_vp, C.POINTER(Params)]),
+    "nbhip_integrate": (C.c_int, [_vp, C.POINTER(Params)]),
+    "nbhip_step": (C.c_int, [_vp, C.POINTER(Params), C.c_int]),
+    "nbhip_synchronize": (C.c_int, [_vp]),
+    "nbhip_diagnostics": (C.c_int, [_vp, C.POINTER(Params), _vp]),
+    "nbhip_force_error_f64": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, _vp, _vp]),
+    "nbhip_timer_start": (C.c_int, [_vp]),
+    "nbhip_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_long)]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library (no GPU call is made).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it with `python -m nbody_amd.build` "
+            "(there is no CPU fallback)")
+    L = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+class HipError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"nbhip status {status}: {msg}")
+        self.status = status
+
+
+def params(**kw) -> Params:
+    p = Params()
+    load().nbhip_default_params(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def probe() -> tuple[bool, str]:
+    buf = C.create_string_buffer(256)
+    st = load().nbhip_probe(buf, 256)
+    return st == OK, buf.value.decode()
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+def _f4(a, n) -> np.ndarray | None:
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, np.float32)
+    if a.size != 4 * n:
+        raise ValueError(f"expected {4 * n} floats, got {a.size}")
+    return a
+
+
+class Context:
+    """One device context: body buffers + a HIP stream (include/nbody_hip.h)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        h = C.c_void_p()
+        st = self.L.nbhip_create(device, C.byref(h))
+        if st != OK:
+            raise HipError(st, self.L.nbhip_last_error(None).decode())
+        self.h = h
+
+    def _check(self, st: int) -> None:
+        if st != OK:
+            raise HipError(st, self.L.nbhip_last_error(self.h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.nbhip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- storage ----
+    @property
+    def count(self) -> int:
+        return self.L.nbhip_count(self.h)
+
+    def reserve(self, n: int) -> None:
+        self._check(self.L.nbhip_reserve(self.h, n))
+
+    def upload(self, pos_mass=None, vel_radius=None, acc=None, offset: int = 0, n: int | None = None):
+        if n is None:
+            for a in (pos_mass, vel_radius, acc):
+                if a is not None:
+                    n = np.asarray(a).size // 4
+                    break
+            else:
+                n = 0
+        pm, vr, ac = _f4(pos_mass, n), _f4(vel_radius, n), _f4(acc, n)
+        self._check(self.L.nbhip_upload(self.h, _ptr(pm), _ptr(vr), _ptr(ac), offset, n))
+
+    def download(self, mask: int = RB_ALL, offset: int = 0, n: int | None = None):
+        if n is None:
+            n = self.count - offset
+        pm = np.zeros((n, 4), np.float32) if mask & RB_POSITIONS else None
+        vr = np.zeros((n, 4), np.float32) if mask & RB_VELOCITIES else None
+        ac = np.zeros((n, 4), np.float32) if mask & RB_ACCELERATIONS else None
+        self._check(self.L.nbhip_download(self.h, mask, _ptr(pm), _ptr(vr), _ptr(ac), offset, n))
+        return pm, vr, ac
+
+    def upload_bodies(self, bodies: np.ndarray) -> None:
+        b = np.ascontiguousarray(bodies, np.float32)
+        if b.ndim != 2 or b.shape[1] != 12:
+            raise ValueError("bodies must have shape (n, 12)")
+        self._check(self.L.nbhip_upload_bodies(self.h, _ptr(b), b.shape[0]))
+
+    def download_bodies(self, n: int | None = None) -> np.ndarray:
+        if n is None:
+            n = self.count
+        b = np.zeros((n, 12), np.float32)
+        self._check(self.L.nbhip_download_bodies(self.h, _ptr(b), n))
+        return b
+
+    # ---- stepping ----
+    def accelerate(self, p: Params) -> None:
+        self._check(self.L.nbhip_accelerate(self.h, C.byref(p)))
+
+    def integrate(self, p: Params) -> None:
+        self._check(self.L.nbhip_integrate(self.h, C.byref(p)))
+
+    def step(self, p: Params, steps: int = 1) -> None:
+        self._check(self.L.nbhip_step(self.h, C.byref(p), steps))
+
+    def synchronize(self) -> None:
+        self._check(self.L.nbhip_synchronize(self.h))
+
+    # ---- diagnostics ----
+    def diagnostics(self, p: Params) -> np.ndarray:
+        out = np.zeros(9, np.float64)
+        self._check(self.L.nbhip_diagnostics(self.h, C.byref(p), _ptr(out)))
+        return out
+
+    def force_error_f64(self, p: Params, sample) -> tuple[np.ndarray, np.ndarray]:
+        s = np.ascontiguousarray(sample, np.uint32)
+        acc64 = np.zeros((s.size, 3), np.float64)
+        rel = np.zeros(s.size, np.float64)
+        self._check(self.L.nbhip_force_error_f64(self.h, C.byref(p), _ptr(s), s.size,
+                                                 _ptr(acc64), _ptr(rel)))
+        return acc64, rel
+
+    # ---- timing ----
+    def timer_start(self) -> None:
+        self._check(self.L.nbhip_timer_start(self.h))
+
+    def timer_stop(self) -> tuple[float, int]:
+        ms = C.c_float()
+        launches = C.c_long()
+        self._check(self.L.nbhip_timer_stop(self.h, C.byref(ms), C.byref(launches)))
+        return ms.value, launches.value

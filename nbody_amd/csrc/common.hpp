@@ -1,0 +1,117 @@
+// common.hpp -- internal declarations shared by the translation units of libnbody_hip.so.
+// Nothing here crosses the C ABI (include/nbody_hip.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "nbody_hip.h"
+
+namespace nbhip {
+
+// Error raised inside the library; converted to a status code + message at the ABI.
+struct Error {
+    int status;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int status, const char* fmt, ...);
+
+#define NBHIP_CHECK(expr)                                                                  \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            ::nbhip::fail(NBHIP_ERR_HIP, "%s failed: %s (%s:%d)", #expr,                   \
+                          hipGetErrorString(_e), __FILE__, __LINE__);                      \
+    } while (0)
+
+// A grow-only device allocation.
+template <typename T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t cap = 0;  // elements
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    // Grow to at least n elements; keeps the first `keep` elements when it moves.
+    void grow(size_t n, size_t keep, hipStream_t s) {
+        if (n <= cap) return;
+        size_t want = cap ? cap : 256;
+        while (want < n) want *= 2;
+        T* p = nullptr;
+        hipError_t e = hipMalloc(&p, want * sizeof(T));
+        if (e != hipSuccess)
+            fail(NBHIP_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", want * sizeof(T),
+                 hipGetErrorString(e));
+        if (ptr && keep) {
+            NBHIP_CHECK(hipMemcpyAsync(p, ptr, (keep < cap ? keep : cap) * sizeof(T),
+                                       hipMemcpyDeviceToDevice, s));
+            NBHIP_CHECK(hipStreamSynchronize(s));
+        }
+        if (ptr) (void)hipFree(ptr);
+        ptr = p;
+        cap = want;
+    }
+};
+
+// Per-launch tuning of the direct-sum force pass.
+struct DirectConfig {
+    int P = 4;       // i-bodies per lane
+    int W = 4;       // waves per block (all waves share the block's i-set, split j)
+    int jsplit = 1;  // blocks along j per i-block (>1 -> partial sums + reduce pass)
+};
+
+}  // namespace nbhip
+
+// The context behind the opaque handle of the C ABI.
+struct nbhip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t count = 0;
+
+    // Split body layout (reference source/gpu.h:32-48); positions ping-pong so a fused
+    // step can write next positions while other blocks still read the current ones.
+    nbhip::DevBuf<float4> pm[2];
+    int cur = 0;
+    nbhip::DevBuf<float4> vr;
+    nbhip::DevBuf<float4> acc;
+
+    // radius bookkeeping: does any body have radius > 0?  (selects the cutoff kernel)
+    bool radius_known = false;
+    bool any_radius = false;
+
+    // scratch: j-split partial sums, staging for AoS transfers, reductions
+    nbhip::DevBuf<float> partial;
+    nbhip::DevBuf<float> aos;
+    nbhip::DevBuf<double> dred;
+    nbhip::DevBuf<float> fscratch;
+    nbhip::DevBuf<unsigned> uscratch;
+
+    // timing
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    long launches = 0;
+    long launches_at_start = 0;
+
+    std::string err;
+};
+
+namespace nbhip {
+
+// direct.hip
+void direct_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse_integrate);
+void integrate(nbhip_ctx* c, const nbhip_params& p);
+void diagnostics(nbhip_ctx* c, const nbhip_params& p, double out[9]);
+void force_error_f64(nbhip_ctx* c, const nbhip_params& p, const uint32_t* sample, size_t ns,
+                     double* acc64, double* rel_err);
+void ensure_radius_flag(nbhip_ctx* c);
+void aos_to_split(nbhip_ctx* c, size_t n);
+void split_to_aos(nbhip_ctx* c, size_t n);
+DirectConfig choose_direct_config(size_t n);
+
+}  // namespace nbhip

@@ -1,0 +1,580 @@
+// direct.hip -- direct-summation force pass, kick-drift, diagnostics and layout kernels
+// for gfx950.
+//
+// Force law (reference source/detail/physics.h:21-42, GLSL shaders/include/common.glsl:41-56,
+// with a softening extension; eps = 0 is the reference law):
+//     a_i = G * sum_{j : |d|^2 > radius_i^2}  m_j d / (|d|^2 + eps^2)^{3/2},  d = x_j - x_i
+// The self pair has |d| = 0 <= radius_i^2 and drops out by the cutoff, exactly as in the
+// reference (the brute-force loop's j != i skip is only a shortcut there).
+//
+// Kernel shape (k_force):
+//   * bodies are SoA float4 {x, y, z, m} (reference source/gpu.h:32-36), read coalesced;
+//   * a block of W waves owns 64*P i-bodies: lane l holds bodies ibase + p*64 + l,
+//     p < P, in registers, so every j read from LDS feeds P interactions;
+//   * the block stages 64*W j-bodies per tile into LDS with one coalesced float4 load per
+//     thread; wave w consumes its own 64-body slice of the tile, reading each j with a
+//     wave-uniform address (an LDS broadcast), so the W waves split j between them;
+//   * per interaction: 3 sub, 3 FMA (|d|^2 + eps^2), one v_rsq_f32, 3 mul, 3 FMA -- all
+//     FP32 on the vector ALU (no MFMA: a pairwise sum is not a dense contraction);
+//   * each wave sums its 64-body slice in plain FP32, then folds the slice total into a
+//     running (sum, compensation) pair with Knuth's TwoSum -- once per 64 interactions;
+//   * wave partials meet in LDS and are combined in a fixed order; a j-split across
+//     blocks (small N) writes partials that k_reduce combines in a fixed order too, so
+//     results are bit-reproducible and independent of launch configuration timing;
+//   * OUT_STEP fuses the semi-implicit Euler kick-drift + wrap into the epilogue and
+//     writes next positions to the other ping-pong buffer.
+#include <algorithm>
+#include <cstdlib>
+#include <vector>
+
+#include "common.hpp"
+
+namespace nbhip {
+namespace {
+
+constexpr int WAVE = 64;
+
+enum : int { OUT_PARTIAL = 0, OUT_ACC = 1, OUT_STEP = 2 };
+
+struct ForceArgs {
+    const float4* pm;  // current {x, y, z, m}
+    const float4* vr;  // {vx, vy, vz, radius}
+    int n;
+    float G;
+    float eps2;
+    int jchunk;        // j bodies handled per blockIdx.y
+    int out_mode;
+    float* partial;    // OUT_PARTIAL: [gridDim.y][3][n] (unscaled by G)
+    float4* acc;       // OUT_ACC / OUT_STEP
+    float4* pm_next;   // OUT_STEP
+    float4* vr_out;    // OUT_STEP (may alias vr: each body is read and written by one lane)
+    float dt;
+    float size;
+    int wrap;
+};
+
+// Knuth's TwoSum: s + c += t with the rounding error of the addition kept in c.
+__device__ __forceinline__ void two_sum(float& s, float& c, float t)
+{
+    const float u = s + t;
+    const float bp = u - s;
+    const float e = (s - (u - bp)) + (t - bp);
+    s = u;
+    c += e;
+}
+
+// 3-torus wrap (reference detail::wrap, source/detail/physics.h:50-57).
+__device__ __forceinline__ float wrap_coord(float x, float size)
+{
+    if (size <= 0.f) return x;
+    const float half = size * .5f;
+    return fmodf(fmodf(x + half, size) + size, size) - half;
+}
+
+// Semi-implicit Euler + wrap (reference detail::integrate_euler, physics.h:60-70), with the
+// reference's operation order and no FMA contraction so it rounds like the CPU build.
+__device__ __forceinline__ void kick_drift(float4& x, float4& v, float ax, float ay, float az,
+                                           float dt, float size, int wrap)
+{
+#pragma clang fp contract(off)
+    v.x = v.x + ax * dt;
+    v.y = v.y + ay * dt;
+    v.z = v.z + az * dt;
+    x.x = x.x + v.x * dt;
+    x.y = x.y + v.y * dt;
+    x.z = x.z + v.z * dt;
+    if (wrap) {
+        x.x = wrap_coord(x.x, size);
+        x.y = wrap_coord(x.y, size);
+        x.z = wrap_coord(x.z, size);
+    }
+}
+
+// Write the final acceleration of body i and, for OUT_STEP, advance it.
+__device__ __forceinline__ void finish_body(const ForceArgs& a, int i, float4 xi, float ax,
+                                            float ay, float az)
+{
+    a.acc[i] = make_float4(ax, ay, az, 0.f);
+    if (a.out_mode == OUT_STEP) {
+        float4 v = a.vr[i];
+        kick_drift(xi, v, ax, ay, az, a.dt, a.size, a.wrap);
+        a.pm_next[i] = xi;
+        a.vr_out[i] = v;
+    }
+}
+
+template <int P, int W, bool CUT>
+__global__ __launch_bounds__(WAVE* W) void k_force(ForceArgs a)
+{
+    __shared__ float4 tile[WAVE * W];
+    __shared__ float red[W][P][3][WAVE];
+
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int n = a.n;
+    const int ibase = blockIdx.x * (WAVE * P);
+
+    float4 xi[P];
+    float r2[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int i = ibase + p * WAVE + lane;
+        xi[p] = i < n ? a.pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (CUT) {
+            const float r = i < n ? a.vr[i].w : 0.f;
+            r2[p] = r * r;
+        }
+    }
+
+    float S[P][3], C[P][3];
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) S[p][k] = C[p][k] = 0.f;
+
+    const float eps2 = a.eps2;
+    const int j0 = blockIdx.y * a.jchunk;
+    const int j1 = min(n, j0 + a.jchunk);
+    for (int t = j0; t < j1; t += WAVE * W) {
+        const int jt = t + threadIdx.x;
+        __syncthreads();  // the previous tile has been consumed
+        // Padding bodies have zero mass at the origin: they add exactly 0 (and the
+        // cutoff masks the 0/0 case when eps == 0).
+        tile[threadIdx.x] = jt < j1 ? a.pm[jt] : make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+
+        float T[P][3];
+#pragma unroll
+        for (int p = 0; p < P; ++p) T[p][0] = T[p][1] = T[p][2] = 0.f;
+
+        const float4* tw = tile + wave * WAVE;
+#pragma unroll 8
+        for (int jj = 0; jj < WAVE; ++jj) {
+            const float4 q = tw[jj];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const float dx = q.x - xi[p].x;
+                const float dy = q.y - xi[p].y;
+                const float dz = q.z - xi[p].z;
+                float w;
+                if (CUT) {
+                    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
+                    w = d2 > r2[p] ? q.w * (inv * inv * inv) : 0.f;
+                } else {
+                    const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+                    const float inv = __builtin_amdgcn_rsqf(d2);
+                    w = q.w * (inv * inv * inv);
+                }
+                T[p][0] = fmaf(w, dx, T[p][0]);
+                T[p][1] = fmaf(w, dy, T[p][1]);
+                T[p][2] = fmaf(w, dz, T[p][2]);
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) two_sum(S[p][k], C[p][k], T[p][k]);
+    }
+
+    // Combine the W wave partials in a fixed order.
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) red[wave][p][k][lane] = S[p][k] + C[p][k];
+    __syncthreads();
+
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        if ((p % W) != wave) continue;
+        const int i = ibase + p * WAVE + lane;
+        if (i >= n) continue;
+        float tot[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float s = 0.f, c = 0.f;
+#pragma unroll
+            for (int w = 0; w < W; ++w) two_sum(s, c, red[w][p][k][lane]);
+            tot[k] = s + c;
+        }
+        if (a.out_mode == OUT_PARTIAL) {
+            float* dst = a.partial + size_t(blockIdx.y) * 3 * n;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) dst[size_t(k) * n + i] = tot[k];
+        } else {
+            finish_body(a, i, xi[p], a.G * tot[0], a.G * tot[1], a.G * tot[2]);
+        }
+    }
+}
+
+// Combine j-split partials in a fixed order, then finish as OUT_ACC / OUT_STEP.
+__global__ __launch_bounds__(256) void k_reduce(ForceArgs a, int jsplit)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const size_t n = a.n;
+    float tot[3];
+    for (int k = 0; k < 3; ++k) {
+        float s = 0.f, c = 0.f;
+        for (int y = 0; y < jsplit; ++y) two_sum(s, c, a.partial[(size_t(y) * 3 + k) * n + i]);
+        tot[k] = s + c;
+    }
+    finish_body(a, i, a.pm[i], a.G * tot[0], a.G * tot[1], a.G * tot[2]);
+}
+
+__global__ __launch_bounds__(256) void k_integrate(float4* pm, float4* vr, const float4* acc, int n,
+                                                   float dt, float size, int wrap)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float4 x = pm[i], v = vr[i];
+    const float4 g = acc[i];
+    kick_drift(x, v, g.x, g.y, g.z, dt, size, wrap);
+    pm[i] = x;
+    vr[i] = v;
+}
+
+// Per-block maximum of |radius| -> out[blockIdx.x].
+__global__ __launch_bounds__(256) void k_max_radius(const float4* vr, int n, float* out)
+{
+    __shared__ float m[256];
+    float v = 0.f;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        v = fmaxf(v, fabsf(vr[i].w));
+    m[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) m[threadIdx.x] = fmaxf(m[threadIdx.x], m[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = m[0];
+}
+
+// ---- potential (diagnostics): phi_i = sum_j m_j / sqrt(|d|^2 + eps^2), cutoff as above --
+__global__ __launch_bounds__(256) void k_potential(const float4* pm, const float4* vr, int n,
+                                                   float eps2, float* phi)
+{
+    __shared__ float4 tile[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const float4 xi = i < n ? pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float r = i < n ? vr[i].w : 0.f;
+    const float r2 = r * r;
+    float S = 0.f, Cc = 0.f;
+    for (int t = 0; t < n; t += 256) {
+        __syncthreads();
+        tile[threadIdx.x] = t + threadIdx.x < n ? pm[t + threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+        float T = 0.f;
+        for (int jj = 0; jj < 256; ++jj) {
+            const float4 q = tile[jj];
+            const float dx = q.x - xi.x, dy = q.y - xi.y, dz = q.z - xi.z;
+            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
+            T += d2 > r2 ? q.w * inv : 0.f;
+        }
+        two_sum(S, Cc, T);
+    }
+    if (i < n) phi[i] = S + Cc;
+}
+
+// FP64 moments: per block partial sums of [KE, PE, px, py, pz, Lx, Ly, Lz, M] (9 doubles).
+__global__ __launch_bounds__(256) void k_moments(const float4* pm, const float4* vr, const float* phi,
+                                                 int n, double G, double* part)
+{
+    __shared__ double sh[9][256];
+    double v[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const float4 x = pm[i], u = vr[i];
+        const double m = x.w, X = x.x, Y = x.y, Z = x.z, vx = u.x, vy = u.y, vz = u.z;
+        v[0] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+        v[1] += -0.5 * G * m * (double)phi[i];
+        v[2] += m * vx;
+        v[3] += m * vy;
+        v[4] += m * vz;
+        v[5] += m * (Y * vz - Z * vy);
+        v[6] += m * (Z * vx - X * vz);
+        v[7] += m * (X * vy - Y * vx);
+        v[8] += m;
+    }
+    for (int k = 0; k < 9; ++k) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 9; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 9) part[blockIdx.x * 9 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// Sum nb rows of 9 doubles in a fixed order -> out[0..8].
+__global__ __launch_bounds__(256) void k_sum_rows9(const double* part, int nb, double* out)
+{
+    __shared__ double sh[9][256];
+    double v[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+    for (int b = threadIdx.x; b < nb; b += 256)
+        for (int k = 0; k < 9; ++k) v[k] += part[b * 9 + k];
+    for (int k = 0; k < 9; ++k) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 9; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 9) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// FP64 direct sum for sampled bodies: one block per sample; out[s*4 + 0..2] = a64,
+// out[s*4 + 3] = |a_stored - a64| / |a64|.
+__global__ __launch_bounds__(256) void k_force_f64(const float4* pm, const float4* vr,
+                                                   const float4* acc, int n, double G, double eps2,
+                                                   const unsigned* sample, double* out)
+{
+    __shared__ double sh[3][256];
+    const unsigned i = sample[blockIdx.x];
+    const float4 xi = pm[i];
+    const double r = vr[i].w;
+    double ax = 0, ay = 0, az = 0;
+    for (int j = threadIdx.x; j < n; j += 256) {
+        const float4 q = pm[j];
+        const double dx = (double)q.x - xi.x, dy = (double)q.y - xi.y, dz = (double)q.z - xi.z;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 <= r * r) continue;
+        const double inv = 1.0 / sqrt(d2 + eps2);
+        const double w = (double)q.w * inv * inv * inv;
+        ax += w * dx;
+        ay += w * dy;
+        az += w * dz;
+    }
+    sh[0][threadIdx.x] = ax;
+    sh[1][threadIdx.x] = ay;
+    sh[2][threadIdx.x] = az;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double gx = G * sh[0][0], gy = G * sh[1][0], gz = G * sh[2][0];
+        const float4 a = acc[i];
+        const double ex = a.x - gx, ey = a.y - gy, ez = a.z - gz;
+        const double mag = sqrt(gx * gx + gy * gy + gz * gz);
+        const double err = sqrt(ex * ex + ey * ey + ez * ez);
+        double* o = out + size_t(blockIdx.x) * 4;
+        o[0] = gx;
+        o[1] = gy;
+        o[2] = gz;
+        o[3] = mag > 0 ? err / mag : err;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_aos_to_split(const float* aos, int n, float4* pm, float4* vr,
+                                                      float4* acc)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float* b = aos + size_t(i) * 12;
+    // Body: pos[3], radius, vel[3], mass, acc[3], pad (reference include/nbody/body.h:6-19)
+    pm[i] = make_float4(b[0], b[1], b[2], b[7]);
+    vr[i] = make_float4(b[4], b[5], b[6], b[3]);
+    acc[i] = make_float4(b[8], b[9], b[10], 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_split_to_aos(float* aos, int n, const float4* pm,
+                                                      const float4* vr, const float4* acc)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float* b = aos + size_t(i) * 12;
+    const float4 x = pm[i], v = vr[i], g = acc[i];
+    b[0] = x.x; b[1] = x.y; b[2] = x.z; b[3] = v.w;
+    b[4] = v.x; b[5] = v.y; b[6] = v.z; b[7] = x.w;
+    b[8] = g.x; b[9] = g.y; b[10] = g.z; b[11] = 0.f;
+}
+
+// ---- dispatch over the compiled (P, W, CUT) variants ----------------------------------
+
+using ForceKernel = void (*)(ForceArgs);
+
+template <int P, int W>
+ForceKernel pick_cut(bool cut)
+{
+    return cut ? (ForceKernel)k_force<P, W, true> : (ForceKernel)k_force<P, W, false>;
+}
+
+ForceKernel pick_force(int P, int W, bool cut)
+{
+#define NBHIP_CASE(p, w) \
+    if (P == p && W == w) return pick_cut<p, w>(cut);
+    NBHIP_CASE(1, 1) NBHIP_CASE(1, 2) NBHIP_CASE(1, 4) NBHIP_CASE(1, 8)
+    NBHIP_CASE(2, 1) NBHIP_CASE(2, 2) NBHIP_CASE(2, 4) NBHIP_CASE(2, 8)
+    NBHIP_CASE(4, 1) NBHIP_CASE(4, 2) NBHIP_CASE(4, 4) NBHIP_CASE(4, 8)
+    NBHIP_CASE(8, 1) NBHIP_CASE(8, 2) NBHIP_CASE(8, 4)
+#undef NBHIP_CASE
+    return nullptr;
+}
+
+int env_int(const char* name, int dflt)
+{
+    const char* v = std::getenv(name);
+    return v && *v ? std::atoi(v) : dflt;
+}
+
+unsigned grid1(size_t n) { return unsigned((n + 255) / 256); }
+
+}  // namespace
+
+DirectConfig choose_direct_config(size_t n)
+{
+    DirectConfig c;
+    c.P = env_int("NBHIP_DIRECT_P", 4);
+    c.W = env_int("NBHIP_DIRECT_W", 4);
+    const size_t per_block = size_t(WAVE) * c.P;
+    const size_t iblocks = (n + per_block - 1) / per_block;
+    const int forced = env_int("NBHIP_DIRECT_JSPLIT", 0);
+    if (forced > 0) {
+        c.jsplit = forced;
+    } else {
+        // Aim for >= 2048 blocks (8 per CU over 256 CUs) so small systems still fill the
+        // chip; never split below one tile of j per block.
+        const size_t target = 2048;
+        size_t js = iblocks >= target ? 1 : (target + iblocks - 1) / iblocks;
+        const size_t tile = size_t(WAVE) * c.W;
+        const size_t max_js = std::max<size_t>(1, (n + tile - 1) / tile);
+        c.jsplit = int(std::min(js, max_js));
+    }
+    return c;
+}
+
+void ensure_radius_flag(nbhip_ctx* c)
+{
+    if (c->radius_known) return;
+    const unsigned nb = std::min<unsigned>(1024, grid1(c->count));
+    c->fscratch.grow(nb, 0, c->stream);
+    hipLaunchKernelGGL(k_max_radius, dim3(nb), dim3(256), 0, c->stream, c->vr.ptr, int(c->count),
+                       c->fscratch.ptr);
+    NBHIP_CHECK(hipGetLastError());
+    std::vector<float> h(nb);
+    NBHIP_CHECK(hipMemcpyAsync(h.data(), c->fscratch.ptr, nb * sizeof(float), hipMemcpyDeviceToHost,
+                               c->stream));
+    NBHIP_CHECK(hipStreamSynchronize(c->stream));
+    float m = 0.f;
+    for (float v : h) m = std::max(m, v);
+    c->any_radius = m > 0.f;
+    c->radius_known = true;
+}
+
+void direct_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse)
+{
+    const size_t n = c->count;
+    ensure_radius_flag(c);
+    const bool cut = c->any_radius || p.eps == 0.f;
+    DirectConfig cfg = choose_direct_config(n);
+    ForceKernel k = pick_force(cfg.P, cfg.W, cut);
+    if (!k) fail(NBHIP_ERR_INVALID, "no force kernel for P=%d W=%d", cfg.P, cfg.W);
+
+    const size_t tile = size_t(WAVE) * cfg.W;
+    size_t jchunk = (n + cfg.jsplit - 1) / cfg.jsplit;
+    jchunk = (jchunk + tile - 1) / tile * tile;
+    const int jsplit = int((n + jchunk - 1) / jchunk);
+
+    ForceArgs a{};
+    a.pm = c->pm[c->cur].ptr;
+    a.vr = c->vr.ptr;
+    a.n = int(n);
+    a.G = p.G;
+    a.eps2 = p.eps * p.eps;
+    a.jchunk = int(jchunk);
+    a.acc = c->acc.ptr;
+    a.pm_next = c->pm[1 - c->cur].ptr;
+    a.vr_out = c->vr.ptr;
+    a.dt = p.dt;
+    a.size = p.size;
+    a.wrap = p.wrap;
+    const int final_mode = fuse ? OUT_STEP : OUT_ACC;
+
+    const size_t per_block = size_t(WAVE) * cfg.P;
+    const dim3 grid(unsigned((n + per_block - 1) / per_block), unsigned(jsplit));
+    if (jsplit == 1) {
+        a.out_mode = final_mode;
+        hipLaunchKernelGGL(k, grid, dim3(WAVE * cfg.W), 0, c->stream, a);
+        NBHIP_CHECK(hipGetLastError());
+    } else {
+        c->partial.grow(size_t(jsplit) * 3 * n, 0, c->stream);
+        a.partial = c->partial.ptr;
+        a.out_mode = OUT_PARTIAL;
+        hipLaunchKernelGGL(k, grid, dim3(WAVE * cfg.W), 0, c->stream, a);
+        NBHIP_CHECK(hipGetLastError());
+        a.out_mode = final_mode;
+        hipLaunchKernelGGL(k_reduce, dim3(grid1(n)), dim3(256), 0, c->stream, a, jsplit);
+        NBHIP_CHECK(hipGetLastError());
+    }
+    c->launches += 1;
+    if (fuse) c->cur = 1 - c->cur;
+}
+
+void integrate(nbhip_ctx* c, const nbhip_params& p)
+{
+    const size_t n = c->count;
+    hipLaunchKernelGGL(k_integrate, dim3(grid1(n)), dim3(256), 0, c->stream, c->pm[c->cur].ptr,
+                       c->vr.ptr, c->acc.ptr, int(n), p.dt, p.size, p.wrap);
+    NBHIP_CHECK(hipGetLastError());
+}
+
+void diagnostics(nbhip_ctx* c, const nbhip_params& p, double out[9])
+{
+    const size_t n = c->count;
+    c->fscratch.grow(n, 0, c->stream);
+    hipLaunchKernelGGL(k_potential, dim3(grid1(n)), dim3(256), 0, c->stream, c->pm[c->cur].ptr,
+                       c->vr.ptr, int(n), p.eps * p.eps, c->fscratch.ptr);
+    NBHIP_CHECK(hipGetLastError());
+    const unsigned nb = std::min<unsigned>(1024, grid1(n));
+    c->dred.grow(size_t(nb) * 9 + 9, 0, c->stream);
+    hipLaunchKernelGGL(k_moments, dim3(nb), dim3(256), 0, c->stream, c->pm[c->cur].ptr, c->vr.ptr,
+                       c->fscratch.ptr, int(n), double(p.G), c->dred.ptr);
+    NBHIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_rows9, dim3(1), dim3(256), 0, c->stream, c->dred.ptr, int(nb),
+                       c->dred.ptr + size_t(nb) * 9);
+    NBHIP_CHECK(hipGetLastError());
+    NBHIP_CHECK(hipMemcpyAsync(out, c->dred.ptr + size_t(nb) * 9, 9 * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream));
+    NBHIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+void force_error_f64(nbhip_ctx* c, const nbhip_params& p, const uint32_t* sample, size_t ns,
+                     double* acc64, double* rel_err)
+{
+    c->uscratch.grow(ns, 0, c->stream);
+    c->dred.grow(ns * 4, 0, c->stream);
+    NBHIP_CHECK(hipMemcpyAsync(c->uscratch.ptr, sample, ns * sizeof(unsigned), hipMemcpyHostToDevice,
+                               c->stream));
+    hipLaunchKernelGGL(k_force_f64, dim3(unsigned(ns)), dim3(256), 0, c->stream, c->pm[c->cur].ptr,
+                       c->vr.ptr, c->acc.ptr, int(c->count), double(p.G),
+                       double(p.eps) * double(p.eps), c->uscratch.ptr, c->dred.ptr);
+    NBHIP_CHECK(hipGetLastError());
+    std::vector<double> h(ns * 4);
+    NBHIP_CHECK(hipMemcpyAsync(h.data(), c->dred.ptr, ns * 4 * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+    NBHIP_CHECK(hipStreamSynchronize(c->stream));
+    for (size_t s = 0; s < ns; ++s) {
+        if (acc64)
+            for (int k = 0; k < 3; ++k) acc64[3 * s + k] = h[4 * s + k];
+        if (rel_err) rel_err[s] = h[4 * s + 3];
+    }
+}
+
+void aos_to_split(nbhip_ctx* c, size_t n)
+{
+    hipLaunchKernelGGL(k_aos_to_split, dim3(grid1(n)), dim3(256), 0, c->stream, c->aos.ptr, int(n),
+                       c->pm[c->cur].ptr, c->vr.ptr, c->acc.ptr);
+    NBHIP_CHECK(hipGetLastError());
+}
+
+void split_to_aos(nbhip_ctx* c, size_t n)
+{
+    hipLaunchKernelGGL(k_split_to_aos, dim3(grid1(n)), dim3(256), 0, c->stream, c->aos.ptr, int(n),
+                       c->pm[c->cur].ptr, c->vr.ptr, c->acc.ptr);
+    NBHIP_CHECK(hipGetLastError());
+}
+
+}  // namespace nbhip

@@ -1,0 +1,376 @@
+// runtime.hip -- the C ABI of libnbody_hip.so: device bring-up, body storage, transfers,
+// stepping entry points, timers.  Counterpart of the reference's Vulkan backend
+// `nbody::GpuDevice` (reference source/gpu.h:135-353, source/gpu.cpp:76-1071); the
+// kernels live in direct.hip.
+#include <cstring>
+#include <new>
+
+#include "common.hpp"
+
+namespace nbhip {
+
+void fail(int status, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    throw Error{ status, buf };
+}
+
+}  // namespace nbhip
+
+using nbhip::Error;
+using nbhip::fail;
+
+namespace {
+
+thread_local std::string g_create_error;  // nbhip_last_error(NULL)
+
+// Run `f`, turning any library error into a status code and the context's message.
+template <typename F>
+int guarded(nbhip_ctx* c, F&& f)
+{
+    try {
+        if (c) c->err.clear();
+        f();
+        return NBHIP_OK;
+    } catch (const Error& e) {
+        if (c) c->err = e.msg;
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        if (c) c->err = "host allocation failed";
+        return NBHIP_ERR_NOMEM;
+    } catch (...) {
+        if (c) c->err = "unknown error";
+        return NBHIP_ERR_HIP;
+    }
+}
+
+void check_ctx(const nbhip_ctx* c)
+{
+    if (!c) fail(NBHIP_ERR_INVALID, "null context");
+}
+
+void check_params(const nbhip_params* p)
+{
+    if (!p) fail(NBHIP_ERR_INVALID, "null params");
+    if (p->mode != NBHIP_MODE_N2 && p->mode != NBHIP_MODE_NLOGN)
+        fail(NBHIP_ERR_INVALID, "unknown mode %d", p->mode);
+    if (!(p->eps >= 0.f)) fail(NBHIP_ERR_INVALID, "eps must be >= 0");
+}
+
+// Is `name` a gfx950 architecture string ("gfx950", "gfx950:sramecc+:xnack-", ...)?
+bool is_gfx950(const char* name) { return std::strncmp(name, "gfx950", 6) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+void nbhip_default_params(nbhip_params* p)
+{
+    if (!p) return;
+    p->dt = 0.f;
+    p->theta = .5f;
+    p->G = 1.f;
+    p->size = 10000.f;
+    p->wrap = 1;
+    p->mode = NBHIP_MODE_N2;
+    p->eps = 0.f;
+    p->opening = NBHIP_OPEN_REFERENCE;
+    p->leaf_size = 1;
+}
+
+int nbhip_probe(char* reason, size_t cap)
+{
+    auto say = [&](const char* msg, int status) {
+        if (reason && cap) {
+            std::strncpy(reason, msg, cap - 1);
+            reason[cap - 1] = 0;
+        }
+        return status;
+    };
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return say("no HIP device is visible", NBHIP_ERR_UNSUPPORTED);
+    for (int d = 0; d < n; ++d) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d) == hipSuccess && is_gfx950(prop.gcnArchName))
+            return say("", NBHIP_OK);
+    }
+    return say("no gfx950 (MI355X) device among the visible HIP devices", NBHIP_ERR_UNSUPPORTED);
+}
+
+int nbhip_create(int device, nbhip_ctx** out)
+{
+    g_create_error.clear();
+    if (!out) {
+        g_create_error = "null output pointer";
+        return NBHIP_ERR_INVALID;
+    }
+    *out = nullptr;
+    nbhip_ctx* c = nullptr;
+    try {
+        int n = 0;
+        NBHIP_CHECK(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n)
+            fail(NBHIP_ERR_INVALID, "device %d out of range (%d visible)", device, n);
+        hipDeviceProp_t prop;
+        NBHIP_CHECK(hipGetDeviceProperties(&prop, device));
+        if (!is_gfx950(prop.gcnArchName))
+            fail(NBHIP_ERR_UNSUPPORTED, "device %d is %s, this library is built for gfx950",
+                 device, prop.gcnArchName);
+        c = new nbhip_ctx();
+        c->device = device;
+        NBHIP_CHECK(hipSetDevice(device));
+        NBHIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        NBHIP_CHECK(hipEventCreate(&c->ev_start));
+        NBHIP_CHECK(hipEventCreate(&c->ev_stop));
+        *out = c;
+        return NBHIP_OK;
+    } catch (const Error& e) {
+        g_create_error = e.msg;
+        nbhip_destroy(c);
+        return e.status;
+    } catch (...) {
+        g_create_error = "context creation failed";
+        nbhip_destroy(c);
+        return NBHIP_ERR_HIP;
+    }
+}
+
+void nbhip_destroy(nbhip_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->pm[0].release();
+    c->pm[1].release();
+    c->vr.release();
+    c->acc.release();
+    c->partial.release();
+    c->aos.release();
+    c->dred.release();
+    c->fscratch.release();
+    c->uscratch.release();
+    if (c->ev_start) (void)hipEventDestroy(c->ev_start);
+    if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* nbhip_last_error(const nbhip_ctx* c)
+{
+    return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+int nbhip_reserve(nbhip_ctx* c, size_t n)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        if (n > (size_t(1) << 31)) fail(NBHIP_ERR_INVALID, "body count %zu too large", n);
+        NBHIP_CHECK(hipSetDevice(c->device));
+        const size_t keep = c->count;
+        c->pm[0].grow(n, keep, c->stream);
+        c->pm[1].grow(n, keep, c->stream);
+        c->vr.grow(n, keep, c->stream);
+        c->acc.grow(n, keep, c->stream);
+        c->count = n;
+        c->radius_known = false;
+    });
+}
+
+size_t nbhip_count(const nbhip_ctx* c) { return c ? c->count : 0; }
+
+int nbhip_upload(nbhip_ctx* c, const float* pos_mass, const float* vel_radius, const float* acc,
+                 size_t offset, size_t n)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        if (offset > c->count || n > c->count - offset)
+            fail(NBHIP_ERR_INVALID, "upload range [%zu, %zu) beyond body count %zu", offset,
+                 offset + n, c->count);
+        if (n == 0) return;
+        NBHIP_CHECK(hipSetDevice(c->device));
+        const size_t bytes = n * sizeof(float4);
+        if (pos_mass)
+            NBHIP_CHECK(hipMemcpyAsync(c->pm[c->cur].ptr + offset, pos_mass, bytes,
+                                       hipMemcpyHostToDevice, c->stream));
+        if (vel_radius) {
+            NBHIP_CHECK(hipMemcpyAsync(c->vr.ptr + offset, vel_radius, bytes,
+                                       hipMemcpyHostToDevice, c->stream));
+            c->radius_known = false;
+        }
+        if (acc)
+            NBHIP_CHECK(hipMemcpyAsync(c->acc.ptr + offset, acc, bytes, hipMemcpyHostToDevice,
+                                       c->stream));
+        // The host buffers belong to the caller once we return.
+        NBHIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int nbhip_download(nbhip_ctx* c, unsigned mask, float* pos_mass, float* vel_radius, float* acc,
+                   size_t offset, size_t n)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        if (offset > c->count || n > c->count - offset)
+            fail(NBHIP_ERR_INVALID, "download range [%zu, %zu) beyond body count %zu", offset,
+                 offset + n, c->count);
+        NBHIP_CHECK(hipSetDevice(c->device));
+        const size_t bytes = n * sizeof(float4);
+        if (n && (mask & NBHIP_RB_POSITIONS)) {
+            if (!pos_mass) fail(NBHIP_ERR_INVALID, "null pos_mass with POSITIONS requested");
+            NBHIP_CHECK(hipMemcpyAsync(pos_mass, c->pm[c->cur].ptr + offset, bytes,
+                                       hipMemcpyDeviceToHost, c->stream));
+        }
+        if (n && (mask & NBHIP_RB_VELOCITIES)) {
+            if (!vel_radius) fail(NBHIP_ERR_INVALID, "null vel_radius with VELOCITIES requested");
+            NBHIP_CHECK(hipMemcpyAsync(vel_radius, c->vr.ptr + offset, bytes,
+                                       hipMemcpyDeviceToHost, c->stream));
+        }
+        if (n && (mask & NBHIP_RB_ACCELERATIONS)) {
+            if (!acc) fail(NBHIP_ERR_INVALID, "null acc with ACCELERATIONS requested");
+            NBHIP_CHECK(hipMemcpyAsync(acc, c->acc.ptr + offset, bytes, hipMemcpyDeviceToHost,
+                                       c->stream));
+        }
+        NBHIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int nbhip_upload_bodies(nbhip_ctx* c, const void* bodies, size_t n)
+{
+    int st = nbhip_reserve(c, n);
+    if (st != NBHIP_OK) return st;
+    return guarded(c, [&] {
+        if (n == 0) return;
+        if (!bodies) fail(NBHIP_ERR_INVALID, "null bodies");
+        c->aos.grow(12 * n, 0, c->stream);
+        NBHIP_CHECK(hipMemcpyAsync(c->aos.ptr, bodies, 48 * n, hipMemcpyHostToDevice, c->stream));
+        nbhip::aos_to_split(c, n);
+        NBHIP_CHECK(hipStreamSynchronize(c->stream));
+        c->radius_known = false;
+    });
+}
+
+int nbhip_download_bodies(nbhip_ctx* c, void* bodies, size_t n)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        const size_t k = n < c->count ? n : c->count;
+        if (k == 0) return;
+        if (!bodies) fail(NBHIP_ERR_INVALID, "null bodies");
+        NBHIP_CHECK(hipSetDevice(c->device));
+        c->aos.grow(12 * c->count, 0, c->stream);
+        nbhip::split_to_aos(c, k);
+        NBHIP_CHECK(hipMemcpyAsync(bodies, c->aos.ptr, 48 * k, hipMemcpyDeviceToHost, c->stream));
+        NBHIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int nbhip_accelerate(nbhip_ctx* c, const nbhip_params* p)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        check_params(p);
+        if (c->count == 0) return;  // nothing to do (source/solvers/gpu_solver_split.h:81-82)
+        NBHIP_CHECK(hipSetDevice(c->device));
+        if (p->mode == NBHIP_MODE_N2)
+            nbhip::direct_accelerate(c, *p, false);
+        else
+            fail(NBHIP_ERR_UNSUPPORTED, "NLogN mode is not available in this build");
+    });
+}
+
+int nbhip_integrate(nbhip_ctx* c, const nbhip_params* p)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        check_params(p);
+        if (c->count == 0) return;
+        NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::integrate(c, *p);
+    });
+}
+
+int nbhip_step(nbhip_ctx* c, const nbhip_params* p, int steps)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        check_params(p);
+        if (steps < 0) fail(NBHIP_ERR_INVALID, "negative step count");
+        if (c->count == 0) return;
+        NBHIP_CHECK(hipSetDevice(c->device));
+        for (int s = 0; s < steps; ++s) {
+            if (p->mode == NBHIP_MODE_N2)
+                nbhip::direct_accelerate(c, *p, true);
+            else
+                fail(NBHIP_ERR_UNSUPPORTED, "NLogN mode is not available in this build");
+        }
+    });
+}
+
+int nbhip_synchronize(nbhip_ctx* c)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        NBHIP_CHECK(hipSetDevice(c->device));
+        NBHIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int nbhip_diagnostics(nbhip_ctx* c, const nbhip_params* p, double out[9])
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        check_params(p);
+        if (!out) fail(NBHIP_ERR_INVALID, "null output");
+        for (int k = 0; k < 9; ++k) out[k] = 0;
+        if (c->count == 0) return;
+        NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::diagnostics(c, *p, out);
+    });
+}
+
+int nbhip_force_error_f64(nbhip_ctx* c, const nbhip_params* p, const uint32_t* sample, size_t ns,
+                          double* acc64, double* rel_err)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        check_params(p);
+        if (ns == 0) return;
+        if (!sample) fail(NBHIP_ERR_INVALID, "null sample");
+        for (size_t k = 0; k < ns; ++k)
+            if (sample[k] >= c->count)
+                fail(NBHIP_ERR_INVALID, "sample index %u beyond body count %zu", sample[k],
+                     c->count);
+        NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::force_error_f64(c, *p, sample, ns, acc64, rel_err);
+    });
+}
+
+int nbhip_timer_start(nbhip_ctx* c)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        NBHIP_CHECK(hipSetDevice(c->device));
+        NBHIP_CHECK(hipEventRecord(c->ev_start, c->stream));
+        c->launches_at_start = c->launches;
+    });
+}
+
+int nbhip_timer_stop(nbhip_ctx* c, float* ms, long* launches)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        if (!ms) fail(NBHIP_ERR_INVALID, "null output");
+        NBHIP_CHECK(hipSetDevice(c->device));
+        NBHIP_CHECK(hipEventRecord(c->ev_stop, c->stream));
+        NBHIP_CHECK(hipEventSynchronize(c->ev_stop));
+        NBHIP_CHECK(hipEventElapsedTime(ms, c->ev_start, c->ev_stop));
+        if (launches) *launches = c->launches - c->launches_at_start;
+    });
+}
+
+}  // extern "C"

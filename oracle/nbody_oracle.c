@@ -1,0 +1,547 @@
+/*
+ * nbody_oracle.c -- CPU oracle for the MI355X gravity solver.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (nbody_amd/, include/) links, loads
+ * or calls this file; only tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py do, and only as the checker / the timed CPU baseline.
+ *
+ * A plain-C restatement of the stett/nbody CPU path, plus FP64 references:
+ *   orc_gravity          source/detail/physics.h:21-42   (force law, cutoff on radius_i)
+ *   orc_wrap             source/detail/physics.h:50-57   (3-torus wrap, double fmod)
+ *   orc_integrate        source/detail/physics.h:60-70 + source/solvers/cpu_solver.h:29-42
+ *   orc_brute_force      source/solvers/cpu_brute_force.h:17-43
+ *   orc_bh_build         source/detail/tree.h:14-29 + source/bhtree.cpp:14-95
+ *                        (+ include/nbody/bounds.h:31-52 octant code / child bounds)
+ *   orc_bh_accelerate    source/solvers/cpu_barnes_hut.h:22-44 + source/bhtree.cpp:97-138
+ *   orc_morton_keys      the Bounds::quadrant descent of source/bhtree.cpp:19-24 written
+ *                        as a key: 3 bits per level, digit = reference octant code
+ *   orc_stable_sort      stable ascending sort of keys, ties by index (std::stable_sort)
+ *   orc_direct_f64       FP64 direct sum (no reference counterpart: the accuracy yardstick)
+ *   orc_energy_f64       FP64 kinetic/potential energy, momentum, angular momentum
+ *
+ * Bodies are the reference's 48-byte AoS `nbody::Body` (include/nbody/body.h:6-19):
+ *   float pos[3], radius, vel[3], mass, acc[3], pad   -> 12 floats per body.
+ *
+ * The softening extension: eps = 0 is the reference law bit for bit; eps > 0 replaces
+ * |d|^2 by |d|^2 + eps^2 in the denominator only (the cutoff still tests raw |d|^2).
+ *
+ * Build with -ffp-contract=off: the reference's usual x86-64 build has no FMA, and the
+ * evaluation order below follows its operator sequence exactly (source/vector.cpp:10-13,
+ * include/nbody/vector.h:48-53), so results are bit-identical to the reference's.
+ */
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#define BF 12 /* floats per Body */
+
+/* ------------------------------------------------------------------------------- */
+/* thread partition: contiguous blocks, remainder spread over the first blocks, like  */
+/* BS::thread_pool::submit_blocks (source/detail/parallel.h:17-22).  Results never    */
+/* depend on it: threads only split the i range.                                      */
+/* ------------------------------------------------------------------------------- */
+typedef void (*orc_block_fn)(void* arg, size_t begin, size_t end);
+typedef struct { orc_block_fn fn; void* arg; size_t begin, end; } orc_job;
+
+static void* orc_thread_main(void* p)
+{
+    orc_job* j = (orc_job*)p;
+    j->fn(j->arg, j->begin, j->end);
+    return NULL;
+}
+
+int orc_hardware_threads(void)
+{
+    long n = sysconf(_SC_NPROCESSORS_ONLN);
+    return n > 0 ? (int)n : 1;
+}
+
+static void orc_parallel_blocks(size_t n, int nthreads, orc_block_fn fn, void* arg)
+{
+    if (n == 0)
+        return;
+    if (nthreads <= 0)
+        nthreads = orc_hardware_threads();
+    if ((size_t)nthreads > n)
+        nthreads = (int)n;
+    if (nthreads == 1) {
+        fn(arg, 0, n);
+        return;
+    }
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    orc_job* jobs = (orc_job*)malloc(sizeof(orc_job) * (size_t)nthreads);
+    const size_t base = n / (size_t)nthreads, rem = n % (size_t)nthreads;
+    size_t at = 0;
+    for (int t = 0; t < nthreads; ++t) {
+        const size_t len = base + ((size_t)t < rem ? 1 : 0);
+        jobs[t].fn = fn;
+        jobs[t].arg = arg;
+        jobs[t].begin = at;
+        jobs[t].end = at + len;
+        at += len;
+        pthread_create(&th[t], NULL, orc_thread_main, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; ++t)
+        pthread_join(th[t], NULL);
+    free(th);
+    free(jobs);
+}
+
+/* ------------------------------------------------------------------------------- */
+/* force law (source/detail/physics.h:21-42)                                           */
+/* ------------------------------------------------------------------------------- */
+static inline void orc_gravity(const float* pos, float radius, const float* src, float src_mass,
+                               float G, float eps2, float* out)
+{
+    const float dx = src[0] - pos[0];
+    const float dy = src[1] - pos[1];
+    const float dz = src[2] - pos[2];
+    const float delta_sq = ((dx * dx) + (dy * dy)) + (dz * dz);
+    const float radii_sq = radius * radius;
+    if (delta_sq <= radii_sq) {
+        out[0] = out[1] = out[2] = 0.f;
+        return;
+    }
+    const float soft_sq = delta_sq + eps2; /* == delta_sq when eps == 0 */
+    const float s = G * src_mass;
+    const float den = sqrtf(soft_sq) * soft_sq;
+    out[0] = (s * dx) / den;
+    out[1] = (s * dy) / den;
+    out[2] = (s * dz) / den;
+}
+
+/* exported for tests of single pairs */
+void orc_gravity_pair(const float* pos, float radius, const float* src, float src_mass, float G,
+                      float eps, float* out)
+{
+    orc_gravity(pos, radius, src, src_mass, G, eps * eps, out);
+}
+
+/* ------------------------------------------------------------------------------- */
+/* wrap + semi-implicit Euler (source/detail/physics.h:50-70)                          */
+/* ------------------------------------------------------------------------------- */
+float orc_wrap(float x, float size)
+{
+    if (size <= 0.f)
+        return x;
+    const float half = size * .5f;
+    return fmodf(fmodf(x + half, size) + size, size) - half;
+}
+
+typedef struct { float* b; float dt, size; int wrap; } orc_integrate_arg;
+
+static void orc_integrate_block(void* p, size_t begin, size_t end)
+{
+    const orc_integrate_arg* a = (const orc_integrate_arg*)p;
+    for (size_t i = begin; i < end; ++i) {
+        float* body = a->b + i * BF;
+        for (int k = 0; k < 3; ++k)
+            body[4 + k] = body[4 + k] + (body[8 + k] * a->dt);
+        for (int k = 0; k < 3; ++k)
+            body[k] = body[k] + (body[4 + k] * a->dt);
+        if (!a->wrap)
+            continue;
+        for (int k = 0; k < 3; ++k)
+            body[k] = orc_wrap(body[k], a->size);
+    }
+}
+
+void orc_integrate(float* bodies, size_t n, float dt, float size, int wrap, int nthreads)
+{
+    orc_integrate_arg a = { bodies, dt, size, wrap };
+    orc_parallel_blocks(n, nthreads, orc_integrate_block, &a);
+}
+
+/* ------------------------------------------------------------------------------- */
+/* brute force (source/solvers/cpu_brute_force.h:17-43)                                */
+/* ------------------------------------------------------------------------------- */
+typedef struct {
+    float* b; size_t n; float G, eps2;
+    const uint32_t* rows; /* optional subset of i (NULL = all) */
+} orc_bf_arg;
+
+static void orc_bf_block(void* p, size_t begin, size_t end)
+{
+    const orc_bf_arg* a = (const orc_bf_arg*)p;
+    float* const b = a->b;
+    for (size_t r = begin; r < end; ++r) {
+        const size_t i = a->rows ? a->rows[r] : r;
+        const float pos[3] = { b[i * BF + 0], b[i * BF + 1], b[i * BF + 2] };
+        const float radius = b[i * BF + 3];
+        float acc[3] = { 0.f, 0.f, 0.f };
+        for (size_t j = 0; j < a->n; ++j) {
+            if (j == i)
+                continue;
+            float g[3];
+            orc_gravity(pos, radius, b + j * BF, b[j * BF + 7], a->G, a->eps2, g);
+            acc[0] += g[0];
+            acc[1] += g[1];
+            acc[2] += g[2];
+        }
+        b[i * BF + 8] = acc[0];
+        b[i * BF + 9] = acc[1];
+        b[i * BF + 10] = acc[2];
+    }
+}
+
+void orc_brute_force(float* bodies, size_t n, float G, float eps, int nthreads)
+{
+    orc_bf_arg a = { bodies, n, G, eps * eps, NULL };
+    orc_parallel_blocks(n, nthreads, orc_bf_block, &a);
+}
+
+/* Same loop over a subset of rows only (the CPU baseline at large N). */
+void orc_brute_force_rows(float* bodies, size_t n, const uint32_t* rows, size_t nrows, float G,
+                          float eps, int nthreads)
+{
+    orc_bf_arg a = { bodies, n, G, eps * eps, rows };
+    orc_parallel_blocks(nrows, nthreads, orc_bf_block, &a);
+}
+
+/* ------------------------------------------------------------------------------- */
+/* FP64 direct sum: the accuracy yardstick (no reference counterpart)                  */
+/* ------------------------------------------------------------------------------- */
+typedef struct {
+    const float* b; size_t n; const uint32_t* sample; double G, eps2; double* out;
+} orc_f64_arg;
+
+static void orc_f64_block(void* p, size_t begin, size_t end)
+{
+    const orc_f64_arg* a = (const orc_f64_arg*)p;
+    const float* b = a->b;
+    for (size_t s = begin; s < end; ++s) {
+        const size_t i = a->sample ? a->sample[s] : s;
+        const double xi = b[i * BF], yi = b[i * BF + 1], zi = b[i * BF + 2];
+        const double r = b[i * BF + 3];
+        const double r2 = r * r;
+        double ax = 0, ay = 0, az = 0;
+        for (size_t j = 0; j < a->n; ++j) {
+            const double dx = (double)b[j * BF] - xi;
+            const double dy = (double)b[j * BF + 1] - yi;
+            const double dz = (double)b[j * BF + 2] - zi;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 <= r2)
+                continue;
+            const double e2 = d2 + a->eps2;
+            const double inv = 1.0 / sqrt(e2);
+            const double w = (double)b[j * BF + 7] * inv * inv * inv;
+            ax += w * dx;
+            ay += w * dy;
+            az += w * dz;
+        }
+        a->out[3 * s + 0] = a->G * ax;
+        a->out[3 * s + 1] = a->G * ay;
+        a->out[3 * s + 2] = a->G * az;
+    }
+}
+
+void orc_direct_f64(const float* bodies, size_t n, const uint32_t* sample, size_t ns, double G,
+                    double eps, double* acc_out, int nthreads)
+{
+    orc_f64_arg a = { bodies, n, sample, G, eps * eps, acc_out };
+    orc_parallel_blocks(sample ? ns : n, nthreads, orc_f64_block, &a);
+}
+
+/* per-body potential sum phi_i = sum_j m_j / sqrt(|d|^2 + eps^2) (cutoff as above) */
+typedef struct { const float* b; size_t n; double eps2; double* phi; } orc_phi_arg;
+
+static void orc_phi_block(void* p, size_t begin, size_t end)
+{
+    const orc_phi_arg* a = (const orc_phi_arg*)p;
+    const float* b = a->b;
+    for (size_t i = begin; i < end; ++i) {
+        const double xi = b[i * BF], yi = b[i * BF + 1], zi = b[i * BF + 2];
+        const double r = b[i * BF + 3];
+        double acc = 0;
+        for (size_t j = 0; j < a->n; ++j) {
+            const double dx = (double)b[j * BF] - xi;
+            const double dy = (double)b[j * BF + 1] - yi;
+            const double dz = (double)b[j * BF + 2] - zi;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 <= r * r)
+                continue;
+            acc += (double)b[j * BF + 7] / sqrt(d2 + a->eps2);
+        }
+        a->phi[i] = acc;
+    }
+}
+
+/* out: [0] kinetic, [1] potential, [2..4] momentum, [5..7] angular momentum, [8] mass */
+void orc_energy_f64(const float* bodies, size_t n, double G, double eps, double* out, int nthreads)
+{
+    double* phi = (double*)calloc(n ? n : 1, sizeof(double));
+    orc_phi_arg a = { bodies, n, eps * eps, phi };
+    orc_parallel_blocks(n, nthreads, orc_phi_block, &a);
+    for (int k = 0; k < 9; ++k)
+        out[k] = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const float* b = bodies + i * BF;
+        const double m = b[7];
+        const double x = b[0], y = b[1], z = b[2], vx = b[4], vy = b[5], vz = b[6];
+        out[0] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+        out[1] += -0.5 * G * m * phi[i];
+        out[2] += m * vx;
+        out[3] += m * vy;
+        out[4] += m * vz;
+        out[5] += m * (y * vz - z * vy);
+        out[6] += m * (z * vx - x * vz);
+        out[7] += m * (x * vy - y * vx);
+        out[8] += m;
+    }
+    free(phi);
+}
+
+/* ------------------------------------------------------------------------------- */
+/* Morton keys by the reference's own octant descent                                  */
+/* Bounds::quadrant: q = (x<cx) | (y<cy)<<1 | (z<cz)<<2  (include/nbody/bounds.h:31-37) */
+/* Bounds::quadrant_bounds: child centre c -/+ size*0.25, size*0.5 (bounds.h:40-52)    */
+/* root: centre (0,0,0), edge `size` (source/detail/tree.h:20)                         */
+/* key = q_1 q_2 ... q_levels, most significant level first, 3 bits per level          */
+/* ------------------------------------------------------------------------------- */
+void orc_morton_keys(const float* bodies, size_t n, float size, int levels, uint64_t* keys)
+{
+    for (size_t i = 0; i < n; ++i) {
+        const float x = bodies[i * BF], y = bodies[i * BF + 1], z = bodies[i * BF + 2];
+        float cx = 0.f, cy = 0.f, cz = 0.f, s = size;
+        uint64_t key = 0;
+        for (int l = 0; l < levels; ++l) {
+            const unsigned q = (unsigned)(x < cx) | ((unsigned)(y < cy) << 1) | ((unsigned)(z < cz) << 2);
+            key = (key << 3) | q;
+            const float quart = s * .25f;
+            cx = (q & 1u) ? (cx - quart) : (cx + quart);
+            cy = (q & 2u) ? (cy - quart) : (cy + quart);
+            cz = (q & 4u) ? (cz - quart) : (cz + quart);
+            s = s * .5f;
+        }
+        keys[i] = key;
+    }
+}
+
+/* Stable ascending sort of keys: perm[k] = index of the k-th smallest key, equal keys in
+ * ascending index order (what std::stable_sort over (key, index) gives).  Bottom-up
+ * merge sort. */
+void orc_stable_sort(const uint64_t* keys, size_t n, uint32_t* perm)
+{
+    uint32_t* tmp = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    for (size_t i = 0; i < n; ++i)
+        perm[i] = (uint32_t)i;
+    uint32_t* src = perm;
+    uint32_t* dst = tmp;
+    for (size_t w = 1; w < n; w *= 2) {
+        for (size_t lo = 0; lo < n; lo += 2 * w) {
+            size_t mid = lo + w < n ? lo + w : n;
+            size_t hi = lo + 2 * w < n ? lo + 2 * w : n;
+            size_t a = lo, b = mid, o = lo;
+            while (a < mid && b < hi)
+                dst[o++] = (keys[src[b]] < keys[src[a]]) ? src[b++] : src[a++];
+            while (a < mid)
+                dst[o++] = src[a++];
+            while (b < hi)
+                dst[o++] = src[b++];
+        }
+        uint32_t* t = src;
+        src = dst;
+        dst = t;
+    }
+    if (src != perm)
+        memcpy(perm, src, sizeof(uint32_t) * n);
+    free(tmp);
+}
+
+/* ------------------------------------------------------------------------------- */
+/* Reference octree (include/nbody/bhtree.h:13-62, source/bhtree.cpp:14-138)           */
+/* 48-byte node, same field order as bh::Node                                          */
+/* ------------------------------------------------------------------------------- */
+typedef struct {
+    float cx, cy, cz, size;   /* Bounds */
+    float mx, my, mz, mass;   /* com, mass */
+    uint32_t next, children, pad0, pad1;
+} orc_node;
+
+typedef struct { orc_node* v; size_t n, cap; int overflow; } orc_tree;
+
+static size_t orc_push(orc_tree* t, orc_node nd)
+{
+    if (t->n >= t->cap) {
+        t->overflow = 1;
+        return t->n;
+    }
+    t->v[t->n] = nd;
+    return t->n++;
+}
+
+static unsigned orc_quadrant(const orc_node* nd, const float* p)
+{
+    return (unsigned)(p[0] < nd->cx) | ((unsigned)(p[1] < nd->cy) << 1) | ((unsigned)(p[2] < nd->cz) << 2);
+}
+
+static orc_node orc_child_bounds(const orc_node* nd, unsigned q)
+{
+    orc_node c;
+    memset(&c, 0, sizeof c);
+    const float half = nd->size * .5f;
+    const float quart = nd->size * .25f;
+    c.cx = (q & 1u) ? (nd->cx - quart) : (nd->cx + quart);
+    c.cy = (q & 2u) ? (nd->cy - quart) : (nd->cy + quart);
+    c.cz = (q & 4u) ? (nd->cz - quart) : (nd->cz + quart);
+    c.size = half;
+    return c;
+}
+
+/* Tree::accumulate (source/bhtree.cpp:87-95) */
+static void orc_accumulate(orc_node* nd, const float* p, float m)
+{
+    const float total = m + nd->mass;
+    nd->mx = ((p[0] * m) + (nd->mx * nd->mass)) / total;
+    nd->my = ((p[1] * m) + (nd->my * nd->mass)) / total;
+    nd->mz = ((p[2] * m) + (nd->mz * nd->mass)) / total;
+    nd->mass = total;
+}
+
+/* Tree::insert (source/bhtree.cpp:14-85) */
+static void orc_insert(orc_tree* t, const float* p, float m)
+{
+    uint32_t idx = 0;
+    while (t->v[idx].children != 0) {
+        orc_accumulate(&t->v[idx], p, m);
+        idx = t->v[idx].children + orc_quadrant(&t->v[idx], p);
+    }
+    if (t->v[idx].mass == 0) {
+        t->v[idx].mass = m;
+        t->v[idx].mx = p[0];
+        t->v[idx].my = p[1];
+        t->v[idx].mz = p[2];
+        return;
+    }
+    for (;;) {
+        if (t->v[idx].size < FLT_EPSILON) {
+            t->v[idx].mass += m;
+            return;
+        }
+        const orc_node nb = t->v[idx];
+        if (t->n + 8 > t->cap) {
+            t->overflow = 1;
+            return;
+        }
+        const uint32_t child = (uint32_t)t->n;
+        t->v[idx].children = child;
+        for (unsigned q = 0; q < 7; ++q) {
+            orc_node c = orc_child_bounds(&nb, q);
+            c.next = child + q + 1;
+            orc_push(t, c);
+        }
+        {
+            orc_node c = orc_child_bounds(&nb, 7);
+            c.next = t->v[idx].next;
+            orc_push(t, c);
+        }
+        const unsigned new_q = orc_quadrant(&nb, p);
+        const float old_com[3] = { t->v[idx].mx, t->v[idx].my, t->v[idx].mz };
+        const unsigned old_q = orc_quadrant(&nb, old_com);
+        if (new_q == old_q) {
+            const uint32_t c = t->v[idx].children + new_q;
+            t->v[c].mass = t->v[idx].mass;
+            t->v[c].mx = t->v[idx].mx;
+            t->v[c].my = t->v[idx].my;
+            t->v[c].mz = t->v[idx].mz;
+            orc_accumulate(&t->v[idx], p, m);
+            idx = c;
+        } else {
+            const uint32_t ni = t->v[idx].children + new_q;
+            const uint32_t oi = t->v[idx].children + old_q;
+            t->v[ni].mx = p[0];
+            t->v[ni].my = p[1];
+            t->v[ni].mz = p[2];
+            t->v[ni].mass = m;
+            t->v[oi].mx = t->v[idx].mx;
+            t->v[oi].my = t->v[idx].my;
+            t->v[oi].mz = t->v[idx].mz;
+            t->v[oi].mass = t->v[idx].mass;
+            orc_accumulate(&t->v[idx], p, m);
+            return;
+        }
+    }
+}
+
+/* detail::build_tree (source/detail/tree.h:14-29): clear to a root of edge `size`
+ * centred at the origin, insert every body in index order.  Returns the node count, or
+ * 0 if `cap` was too small. */
+size_t orc_bh_build(const float* bodies, size_t n, float size, orc_node* nodes, size_t cap)
+{
+    if (cap < 1)
+        return 0;
+    orc_tree t = { nodes, 0, cap, 0 };
+    orc_node root;
+    memset(&root, 0, sizeof root);
+    root.size = size;
+    orc_push(&t, root);
+    for (size_t i = 0; i < n && !t.overflow; ++i)
+        orc_insert(&t, bodies + i * BF, bodies[i * BF + 7]);
+    return t.overflow ? 0 : t.n;
+}
+
+/* Tree::apply with the gravity callback (source/bhtree.cpp:97-138,
+ * source/solvers/cpu_barnes_hut.h:36-41) */
+typedef struct {
+    float* b; const orc_node* nodes; size_t nn; float theta, G, eps2;
+    uint64_t* visits; /* optional per-body accepted-node counts */
+} orc_bh_arg;
+
+static void orc_bh_block(void* p, size_t begin, size_t end)
+{
+    const orc_bh_arg* a = (const orc_bh_arg*)p;
+    const float theta_sq = a->theta * a->theta;
+    for (size_t i = begin; i < end; ++i) {
+        float* body = a->b + i * BF;
+        const float pos[3] = { body[0], body[1], body[2] };
+        float acc[3] = { 0.f, 0.f, 0.f };
+        uint64_t visits = 0;
+        uint32_t idx = 0;
+        do {
+            const orc_node* nd = &a->nodes[idx];
+            if (nd->mass == 0) {
+                idx = nd->next;
+                continue;
+            }
+            int apply = 0;
+            if (nd->children == 0) {
+                apply = 1;
+            } else {
+                const float node_size_sq = nd->size * nd->size;
+                const float dx = nd->mx - pos[0], dy = nd->my - pos[1], dz = nd->mz - pos[2];
+                const float dist_sq = ((dx * dx) + (dy * dy)) + (dz * dz);
+                apply = dist_sq > node_size_sq * theta_sq;
+            }
+            if (apply) {
+                const float com[3] = { nd->mx, nd->my, nd->mz };
+                float g[3];
+                orc_gravity(pos, body[3], com, nd->mass, a->G, a->eps2, g);
+                acc[0] += g[0];
+                acc[1] += g[1];
+                acc[2] += g[2];
+                ++visits;
+                idx = nd->next;
+                continue;
+            }
+            idx = nd->children;
+        } while (0 < idx && idx < a->nn);
+        body[8] = acc[0];
+        body[9] = acc[1];
+        body[10] = acc[2];
+        if (a->visits)
+            a->visits[i] = visits;
+    }
+}
+
+void orc_bh_accelerate(float* bodies, size_t n, const orc_node* nodes, size_t nn, float theta,
+                       float G, float eps, uint64_t* visits, int nthreads)
+{
+    orc_bh_arg a = { bodies, nodes, nn, theta, G, eps * eps, visits };
+    orc_parallel_blocks(n, nthreads, orc_bh_block, &a);
+}
+
+size_t orc_node_size(void) { return sizeof(orc_node); }

@@ -1,0 +1,199 @@
+"""Pins the CPU oracle (oracle/nbody_oracle.c) before anything is checked against it.
+
+Each oracle function is checked against the golden vectors written from the reference's
+own sources (tests/golden/make_golden.py), and, where those leave something open, against
+what the mathematics fixes: closed forms, invariants, special cases and brute force.
+"""
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+# ---- golden vectors (bit-exact) --------------------------------------------------------
+
+def test_gravity_pairs_match_reference(orc):
+    g = gold("gravity_pairs.npz")
+    for i in range(g["pos"].shape[0]):
+        out = orc.gravity_pair(g["pos"][i], g["radius"][i], g["src"][i], g["mass"][i], g["G"][i], 0.0)
+        assert np.array_equal(bits(out), bits(g["acc"][i])), i
+
+
+def test_wrap_matches_reference(orc):
+    g = gold("wrap_cases.npz")
+    for x, s, w in zip(g["x"], g["size"], g["wrapped"]):
+        assert np.float32(orc.wrap(float(x), float(s))).view(np.uint32) == np.float32(w).view(np.uint32)
+
+
+@pytest.mark.parametrize("case,G", [("disk", 1.0), ("cube", 4.0)])
+def test_brute_force_matches_reference(orc, case, G):
+    g = gold("brute_force.npz")
+    b = g[case].copy()
+    orc.brute_force(b, G=G, eps=0.0, threads=3)
+    assert np.array_equal(bits(b), bits(g[case + "_out"]))
+
+
+def test_integrate_matches_reference(orc):
+    g = gold("integrate.npz")
+    for out, dt, size, wrap in [("out_a", 1 / 60, 10000.0, True), ("out_b", 0.5, 50.0, True),
+                                ("out_c", 0.5, 50.0, False)]:
+        b = g["bodies"].copy()
+        orc.integrate(b, np.float32(dt), size, wrap)
+        assert np.array_equal(bits(b), bits(g[out])), out
+
+
+@pytest.mark.parametrize("case,size", [("disk64", 10000.0), ("cube100", 1024.0)])
+def test_tree_matches_reference(orc, case, size):
+    g = gold("tree.npz")
+    nodes = orc.bh_build(g[case], size)
+    assert np.array_equal(nodes.view(np.uint8), g[case + "_nodes"])
+
+
+@pytest.mark.parametrize("theta", [0.5, 8.0])
+def test_barnes_hut_matches_reference(orc, theta):
+    g = gold("barnes_hut.npz")
+    tag = "05" if theta == 0.5 else "8"
+    b = g["bodies"].copy()
+    _, visits = orc.bh_accelerate(b, 10000.0, theta=theta, visits=True, threads=2)
+    assert np.array_equal(bits(b), bits(g["out_theta" + tag]))
+    assert np.array_equal(visits, g["visits_theta" + tag])
+
+
+@pytest.mark.parametrize("size", [256, 10000, 100])
+def test_morton_keys_match_reference_descent(orc, size):
+    g = gold("cell_keys.npz")
+    keys = orc.morton_keys(g["bodies"], float(size), 21)
+    assert np.array_equal(keys, g["keys_%d" % size])
+
+
+# ---- mathematics (what the golden vectors leave open) ---------------------------------
+
+def test_two_body_closed_form(orc):
+    # |a| = G m / r^2 along the separation; softened: G m r / (r^2 + eps^2)^{3/2}
+    a = orc.gravity_pair([0, 0, 0], 0.0, [3, 0, 4], 2.0, G=1.5, eps=0.0)
+    assert np.allclose(a, 1.5 * 2.0 / 25.0 * np.array([0.6, 0, 0.8]), rtol=1e-6)
+    a = orc.gravity_pair([0, 0, 0], 0.0, [3, 0, 4], 2.0, G=1.0, eps=0.5)
+    assert np.allclose(a, 2.0 * np.array([3, 0, 4]) / (25.25 ** 1.5), rtol=1e-6)
+    # cutoff on the receiving body's radius (not strict: equal distance is cut)
+    assert np.all(orc.gravity_pair([0, 0, 0], 5.0, [3, 0, 4], 2.0) == 0)
+    assert np.any(orc.gravity_pair([0, 0, 0], 4.99, [3, 0, 4], 2.0) != 0)
+
+
+def test_brute_force_momentum_and_f64(orc):
+    from nbody_amd import ics
+    b = ics.plummer(512, seed=7)
+    orc.brute_force(b, eps=0.01)
+    m = b[:, 7:8].astype(np.float64)
+    a = b[:, 8:11].astype(np.float64)
+    # Newton's third law: sum m a = 0 up to FP32 rounding
+    assert np.abs((m * a).sum(0)).max() < 1e-5 * np.abs(m * a).sum(0).max()
+    a64 = orc.direct_f64(b, eps=0.01)
+    rel = np.linalg.norm(a - a64, axis=1) / np.linalg.norm(a64, axis=1)
+    assert rel.max() < 1e-4 and np.median(rel) < 1e-6
+    # FP64 sample path equals the full path
+    s = np.array([0, 17, 511], np.uint32)
+    assert np.array_equal(orc.direct_f64(b, s, eps=0.01), a64[s])
+
+
+def test_direct_f64_plummer_field(orc):
+    # Outside a uniform-density ball the field is G M / r^2 (shell theorem): a body far
+    # away sees the whole cloud as a point mass at its centre.
+    rng = np.random.default_rng(3)
+    n = 4000
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    r = rng.uniform(0, 1, n) ** (1 / 3)
+    b = np.zeros((n + 1, 12), np.float32)
+    b[:n, 0:3] = u * r[:, None]
+    b[:n, 7] = 1.0 / n
+    b[n, 0:3] = [50.0, 0, 0]
+    b[n, 7] = 1e-9
+    a = orc.direct_f64(b, np.array([n], np.uint32))[0]
+    assert abs(a[0] + 1.0 / 2500.0) < 1e-3 / 2500.0
+
+
+def test_integrate_closed_form(orc):
+    # G = 0 drift: x + v dt exactly; wrap 499 + 10 -> -491 in a box of 1000
+    # (tests/source/test_wrap.cpp:15-26,48)
+    b = np.zeros((1, 12), np.float32)
+    b[0, 0:3] = [499, 0, 0]
+    b[0, 4:7] = [10, 0, 0]
+    orc.integrate(b, 1.0, 1000.0, True)
+    assert b[0, 0] == -491.0
+    b[0, 0:3] = [499, 0, 0]
+    orc.integrate(b, 1.0, 1000.0, False)
+    assert b[0, 0] == 509.0
+
+
+def test_energy_f64_two_bodies(orc):
+    b = np.zeros((2, 12), np.float32)
+    b[0, 0:3] = [1, 0, 0]
+    b[1, 0:3] = [-1, 0, 0]
+    b[0, 4:7] = [0, 0.5, 0]
+    b[1, 4:7] = [0, -0.5, 0]
+    b[:, 7] = [1.0, 1.0]
+    e = orc.energy_f64(b, G=1.0, eps=0.0)
+    assert np.isclose(e[0], 0.25)          # 2 * 1/2 * 1 * 0.25
+    assert np.isclose(e[1], -0.5)          # -G m m / r
+    assert np.allclose(e[2:5], 0)
+    assert np.isclose(e[7], 1.0)           # L_z = 2 * (1 * 0.5)
+    assert e[8] == 2.0
+
+
+def test_stable_sort_is_std_stable_sort(orc):
+    rng = np.random.default_rng(11)
+    keys = rng.integers(0, 50, 5000).astype(np.uint64)  # many ties
+    perm = orc.stable_sort(keys)
+    assert np.array_equal(perm, np.argsort(keys, kind="stable").astype(np.uint32))
+
+
+def test_morton_descent_is_floor_quantisation(orc):
+    # With a power-of-two box the comparison descent equals floor quantisation, digit =
+    # complement of the usual Morton bit (a set bit means the lower half).
+    rng = np.random.default_rng(5)
+    b = np.zeros((3000, 12), np.float32)
+    b[:, 0:3] = rng.uniform(-64, 64, (3000, 3)).astype(np.float32)
+    keys = orc.morton_keys(b, 128.0, 10)
+    cell = np.floor((b[:, 0:3].astype(np.float64) + 64.0) / (128.0 / 1024)).astype(np.int64)
+    expect = np.zeros(3000, np.uint64)
+    for lvl in range(10):
+        bit = 9 - lvl
+        q = ((1 - ((cell[:, 0] >> bit) & 1)) | ((1 - ((cell[:, 1] >> bit) & 1)) << 1)
+             | ((1 - ((cell[:, 2] >> bit) & 1)) << 2))
+        expect = (expect << np.uint64(3)) | q.astype(np.uint64)
+    assert np.array_equal(keys, expect)
+
+
+def test_bh_special_cases(orc):
+    from nbody_amd import ics
+    b = ics.plummer(300, seed=2)
+    # theta -> 0 (reference convention): only leaves can be applied... and the root is
+    # accepted only if dist^2 > 0; theta huge: every internal node is opened, so the sum
+    # visits every body's leaf = brute force up to summation order.
+    bf = b.copy()
+    orc.brute_force(bf, eps=0.0)
+    bh = b.copy()
+    orc.bh_accelerate(bh, 64.0, theta=1e6, eps=0.0)
+    rel = np.linalg.norm(bh[:, 8:11] - bf[:, 8:11], axis=1) / np.linalg.norm(bf[:, 8:11], axis=1)
+    assert rel.max() < 1e-4
+    # theta = 0 accepts any internal node with a distance > 0: for a body outside the
+    # root's com every accepted node is the root itself -> a = G M (com - x) / |.|^3
+    nodes = orc.bh_build(b, 64.0)
+    one = b.copy()
+    orc.bh_accelerate(one, 64.0, theta=0.0, nodes=nodes)
+    com = nodes[0]["com"].astype(np.float64)
+    d = com - one[:, 0:3].astype(np.float64)
+    expect = nodes[0]["mass"] * d / np.linalg.norm(d, axis=1, keepdims=True) ** 3
+    assert np.allclose(one[:, 8:11], expect, rtol=1e-4)
+    # mass conservation in the tree (tests/source/test_bhtree.cpp:169-212)
+    assert np.isclose(nodes[0]["mass"], b[:, 7].sum(), rtol=1e-5)
