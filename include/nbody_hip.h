@@ -167,6 +167,38 @@ int nbhip_diagnostics(nbhip_ctx* ctx, const nbhip_params* p, double out[9]);
 int nbhip_force_error_f64(nbhip_ctx* ctx, const nbhip_params* p, const uint32_t* sample,
                           size_t ns, double* acc64, double* rel_err);
 
+/* ---- Barnes-Hut internals (NLogN mode), exposed for bit-exact and structural checks --- */
+
+/* One node of the device octree (32 bytes), in DFS pre-order: the children of an
+ * internal node follow it, and `next` is the first node after its subtree (the
+ * reference's `next` threading, source/bhtree.cpp:44-54).  A run of cells whose bodies
+ * all lie in one child is stored once, with the deepest cell's edge. */
+typedef struct nbhip_bh_node {
+    float    com[3];   /* centre of mass (accumulated in FP64)                        */
+    float    mass;
+    uint32_t next;     /* pre-order index of the first node after this subtree        */
+    float    open_sq;  /* applied as one mass when |com - x|^2 > open_sq              */
+    uint32_t first;    /* leaf / bucket: first body, as a position in key order       */
+    uint32_t count;    /* 0 = internal node; >= 1 = leaf holding `count` bodies       */
+} nbhip_bh_node;
+
+/* Keys and sort permutation of the current positions: keys[i] is body i's 63-bit key
+ * (21 levels x 3 bits of the reference's octant descent, include/nbody/bounds.h:31-52,
+ * in a root cube of edge p->size centred at the origin, most significant level first);
+ * perm[k] is the body with the k-th smallest key, ties in index order (a stable sort).
+ * Either output may be NULL.  Synchronises. */
+int nbhip_bh_keys(nbhip_ctx* ctx, const nbhip_params* p, uint64_t* keys, uint32_t* perm);
+
+/* Build the octree of the current positions (as an NLogN accelerate does) and copy out
+ * at most `cap` nodes; *count receives the node count.  Synchronises. */
+int nbhip_bh_nodes(nbhip_ctx* ctx, const nbhip_params* p, nbhip_bh_node* nodes, size_t cap,
+                   size_t* count);
+
+/* NLogN accelerate that also counts body-node interactions: per_body[i] (may be NULL)
+ * and their sum in *total (may be NULL).  Synchronises. */
+int nbhip_bh_interactions(nbhip_ctx* ctx, const nbhip_params* p, uint32_t* per_body,
+                          uint64_t* total);
+
 /* ---- timing: HIP events on the context's stream ----------------------------------- */
 
 /* Record a start event on the context's stream. */

@@ -31,6 +31,13 @@ RB_ACCELERATIONS = 4
 RB_ALL = 7
 
 
+BH_NODE_DTYPE = np.dtype([
+    ("com", np.float32, 3), ("mass", np.float32), ("next", np.uint32),
+    ("open_sq", np.float32), ("first", np.uint32), ("count", np.uint32),
+])
+assert BH_NODE_DTYPE.itemsize == 32
+
+
 class Params(C.Structure):
     """nbhip_params (include/nbody_hip.h)."""
     _fields_ = [
@@ -61,6 +68,9 @@ SIGNATURES = {
     "nbhip_synchronize": (C.c_int, [_vp]),
     "nbhip_diagnostics": (C.c_int, [_vp, C.POINTER(Params), _vp]),
     "nbhip_force_error_f64": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, _vp, _vp]),
+    "nbhip_bh_keys": (C.c_int, [_vp, C.POINTER(Params), _vp, _vp]),
+    "nbhip_bh_nodes": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, C.POINTER(_sz)]),
+    "nbhip_bh_interactions": (C.c_int, [_vp, C.POINTER(Params), _vp, C.POINTER(C.c_uint64)]),
     "nbhip_timer_start": (C.c_int, [_vp]),
     "nbhip_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_long)]),
 }
@@ -220,6 +230,27 @@ class Context:
         self._check(self.L.nbhip_force_error_f64(self.h, C.byref(p), _ptr(s), s.size,
                                                  _ptr(acc64), _ptr(rel)))
         return acc64, rel
+
+    # ---- Barnes-Hut internals ----
+    def bh_keys(self, p: Params) -> tuple[np.ndarray, np.ndarray]:
+        n = self.count
+        keys = np.zeros(n, np.uint64)
+        perm = np.zeros(n, np.uint32)
+        self._check(self.L.nbhip_bh_keys(self.h, C.byref(p), _ptr(keys), _ptr(perm)))
+        return keys, perm
+
+    def bh_nodes(self, p: Params) -> np.ndarray:
+        cnt = C.c_size_t()
+        self._check(self.L.nbhip_bh_nodes(self.h, C.byref(p), None, 0, C.byref(cnt)))
+        nodes = np.zeros(cnt.value, BH_NODE_DTYPE)
+        self._check(self.L.nbhip_bh_nodes(self.h, C.byref(p), _ptr(nodes), nodes.size, C.byref(cnt)))
+        return nodes
+
+    def bh_interactions(self, p: Params) -> tuple[np.ndarray, int]:
+        per = np.zeros(self.count, np.uint32)
+        tot = C.c_uint64()
+        self._check(self.L.nbhip_bh_interactions(self.h, C.byref(p), _ptr(per), C.byref(tot)))
+        return per, tot.value
 
     # ---- timing ----
     def timer_start(self) -> None:

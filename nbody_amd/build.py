@@ -30,7 +30,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-I", INCLUDE, "-I", CSRC,
 ]
 CXX = os.environ.get("CXX", "g++")
-CXX_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-Wall", "-I", INCLUDE, "-I", HOST]
+CXX_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-Wall", "-I", INCLUDE, "-I", HOST,
+             "-I", os.path.join(HOST, "api")]
 
 
 def _run(cmd: list[str]) -> None:
@@ -70,9 +71,27 @@ def build_host() -> str | None:
     return HOSTLIB
 
 
+TESTS_CPP = os.path.join(ROOT, "tests", "cpp")
+TEST_BIN = os.path.join(TESTS_CPP, "_build", "test_hip_solver")
+
+
+def build_cpp_tests() -> str | None:
+    """The C++ conformance tests of the Solver layer (tests/cpp)."""
+    src = os.path.join(TESTS_CPP, "test_hip_solver.cpp")
+    if not os.path.exists(src) or not os.path.exists(HOSTLIB):
+        return None
+    os.makedirs(os.path.dirname(TEST_BIN), exist_ok=True)
+    tmp = TEST_BIN + ".tmp"
+    _run([CXX, *CXX_FLAGS, "-o", tmp, src, "-L", PKG, "-lnbody_host", "-lnbody_hip",
+          "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/../../../nbody_amd"])
+    os.replace(tmp, TEST_BIN)
+    return TEST_BIN
+
+
 def build_all() -> None:
     build_hip()
     build_host()
+    build_cpp_tests()
 
 
 if __name__ == "__main__":

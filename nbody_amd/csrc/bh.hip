@@ -1,0 +1,702 @@
+// bh.hip -- Barnes-Hut on gfx950: Morton keys, LSD radix sort, compressed octree in DFS
+// pre-order, FP64 centres of mass, and a wavefront-coherent stackless traversal.
+//
+// Semantics follow the reference octree (reference source/bhtree.cpp:14-138,
+// include/nbody/bounds.h:31-52, source/detail/tree.h:14-29):
+//   * root cube of edge `size` centred at the origin; octant code
+//     q = (x<cx) | (y<cy)<<1 | (z<cz)<<2, so a set bit selects the LOWER half;
+//   * children visited in q order 0..7 -- exactly ascending order of the key whose
+//     digits are the q codes, most significant level first;
+//   * a leaf holds one body; a node is applied when it is a leaf or when
+//     |com - x|^2 > size^2 * theta^2 (the reference's inverted theta; the textbook
+//     criterion size/|com - x| < theta is the same test with threshold size^2/theta^2);
+//   * nodes of zero mass are skipped; a body inside an accepted cell receives the cell's
+//     aggregate, its own mass included.
+// A cell whose bodies all sit in one child (a chain) is stored once, with the size of
+// the deepest cell of the chain: its mass and centre are those of every cell in the
+// chain and the deepest cell's test is the first one of the chain to pass, so the
+// traversal visits and applies exactly what the reference's does.
+//
+// Build (all on the device, no host synchronisation):
+//   1. k_keys      63-bit key per body: 21 levels of the reference's own float descent.
+//   2. radix sort  8 stable LSD passes of 8 bits over (key, body index) pairs.
+//   3. k_gather    bodies in key order + per-body FP64 moments; exclusive scan -> prefix
+//                  sums, so a node's mass and centre over bodies [a, b] are two lookups.
+//   4. k_count /   with d(k) = common prefix (in octal digits) of keys k and k+1, the
+//      k_emit      nodes that start at sorted position a are found by walking the prefix
+//                  minima of d to the right of a (each found with a galloping search);
+//                  pre-order is "for a = 0..n-1: nodes starting at a, shallow to deep,
+//                  then leaf a", so a scan of per-position counts gives every node's
+//                  index, and a node covering bodies [a, b] has next = base[b + 1].
+// Cells of 21 levels that still hold several bodies, and (leaf_size > 1) cells of at
+// most leaf_size bodies, become buckets: tested like an internal node, applied body by
+// body when opened.
+//
+// Traversal (k_walk): one lane per body in key order, so a wave holds 64 neighbouring
+// bodies.  Each lane keeps its own monotone pre-order pointer; the wave takes the
+// minimum pointer over its lanes, loads that node once with a wave-uniform address, and
+// only the lanes sitting on it advance.  Every lane therefore sees exactly its own
+// reference traversal while the node fetches stay uniform.
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+
+extern "C" __device__ unsigned int __ockl_wfred_min_u32(unsigned int);
+
+namespace nbhip {
+namespace {
+
+constexpr int LEVELS = 21;
+
+__device__ __forceinline__ Dbl4 operator+(const Dbl4& a, const Dbl4& b)
+{
+    return Dbl4{ a.m + b.m, a.x + b.x, a.y + b.y, a.z + b.z };
+}
+
+// ---- 1. keys ------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_keys(const float4* pm, int n, float size,
+                                              unsigned long long* keys, unsigned* vals)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pm[i];
+    float cx = 0.f, cy = 0.f, cz = 0.f, s = size;
+    unsigned long long key = 0;
+    for (int l = 0; l < LEVELS; ++l) {
+        const unsigned q = unsigned(p.x < cx) | (unsigned(p.y < cy) << 1) | (unsigned(p.z < cz) << 2);
+        key = (key << 3) | q;
+        const float quart = s * .25f;
+        cx = (q & 1u) ? (cx - quart) : (cx + quart);
+        cy = (q & 2u) ? (cy - quart) : (cy + quart);
+        cz = (q & 4u) ? (cz - quart) : (cz + quart);
+        s = s * .5f;
+    }
+    keys[i] = key;
+    vals[i] = unsigned(i);
+}
+
+// ---- scans (3-phase: tile totals, scan of totals, tile scans) -------------------------------
+constexpr int SC_BLOCK = 256;
+constexpr int SC_IPT = 8;
+constexpr int SC_TILE = SC_BLOCK * SC_IPT;
+
+template <typename T>
+__device__ T block_exclusive_scan(T v, T* sh, T& total)
+{
+    // Hillis-Steele inclusive scan over the block's 256 values, then shift.
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < SC_BLOCK; off <<= 1) {
+        const T add = t >= off ? sh[t - off] : T{};
+        __syncthreads();
+        if (t >= off) sh[t] = sh[t] + add;
+        __syncthreads();
+    }
+    total = sh[SC_BLOCK - 1];
+    const T excl = t > 0 ? sh[t - 1] : T{};
+    __syncthreads();
+    return excl;
+}
+
+template <typename T>
+__global__ __launch_bounds__(SC_BLOCK) void k_scan_reduce(const T* in, int n, T* part)
+{
+    __shared__ T sh[SC_BLOCK];
+    const int base = blockIdx.x * SC_TILE + threadIdx.x * SC_IPT;
+    T v{};
+    for (int k = 0; k < SC_IPT; ++k)
+        if (base + k < n) v = v + in[base + k];
+    T total;
+    (void)block_exclusive_scan(v, sh, total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+// Exclusive scan of the nb tile totals, in place, by one block.
+template <typename T>
+__global__ __launch_bounds__(SC_BLOCK) void k_scan_part(T* part, int nb)
+{
+    __shared__ T sh[SC_BLOCK];
+    T carry{};
+    for (int base = 0; base < nb; base += SC_BLOCK) {
+        const int i = base + threadIdx.x;
+        const T v = i < nb ? part[i] : T{};
+        T total;
+        const T ex = block_exclusive_scan(v, sh, total);
+        if (i < nb) part[i] = carry + ex;
+        carry = carry + total;
+    }
+}
+
+// out[i] = sum of in[0..i-1]; out[n] = total.  `out` may alias `in`.
+template <typename T>
+__global__ __launch_bounds__(SC_BLOCK) void k_scan_apply(const T* in, int n, const T* part, T* out)
+{
+    __shared__ T sh[SC_BLOCK];
+    const int base = blockIdx.x * SC_TILE + threadIdx.x * SC_IPT;
+    T v[SC_IPT];
+    T sum{};
+    for (int k = 0; k < SC_IPT; ++k) {
+        v[k] = base + k < n ? in[base + k] : T{};
+        sum = sum + v[k];
+    }
+    T total;
+    T run = part[blockIdx.x] + block_exclusive_scan(sum, sh, total);
+    for (int k = 0; k < SC_IPT; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run = run + v[k];
+    }
+    if (base <= n - 1 && n - 1 < base + SC_IPT) out[n] = run;  // the thread holding the last item
+}
+
+template <typename T>
+void exclusive_scan(nbhip_ctx* c, const T* in, int n, T* out, DevBuf<T>& part)
+{
+    const int nb = std::max(1, (n + SC_TILE - 1) / SC_TILE);
+    part.grow(nb, 0, c->stream);
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(SC_BLOCK), 0, c->stream, in, n, part.ptr);
+    NBHIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_part<T>, dim3(1), dim3(SC_BLOCK), 0, c->stream, part.ptr, nb);
+    NBHIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3(nb), dim3(SC_BLOCK), 0, c->stream, in, n, part.ptr, out);
+    NBHIP_CHECK(hipGetLastError());
+}
+
+// ---- 2. LSD radix sort, 8-bit digits, stable ----------------------------------------------
+constexpr int RS_BLOCK = 256;
+constexpr int RS_WAVES = RS_BLOCK / 64;
+constexpr int RS_IPT = 16;
+constexpr int RS_TILE = RS_BLOCK * RS_IPT;
+
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const unsigned long long* keys, int n, int shift,
+                                                      unsigned* hist)
+{
+    __shared__ unsigned h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_IPT; ++r) {
+        const int k = base + r * RS_BLOCK + threadIdx.x;
+        if (k < n) atomicAdd(&h[unsigned(keys[k] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];  // digit-major
+}
+
+// Scatter with stable ranks: items are taken in index order, 256 per round; within a
+// wave, lanes holding the same digit are found with 8 ballots; wave counts are combined
+// in wave order through LDS.
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const unsigned long long* kin, const unsigned* vin,
+                                                         unsigned long long* kout, unsigned* vout, int n,
+                                                         int shift, const unsigned* offs)
+{
+    __shared__ unsigned running[256];
+    __shared__ unsigned wcnt[RS_WAVES][256];
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = t >> 6;
+    running[t] = offs[t * gridDim.x + blockIdx.x];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int base = blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_IPT; ++r) {
+        const int k = base + r * RS_BLOCK + t;
+        const bool valid = k < n;
+        const unsigned long long key = valid ? kin[k] : 0ull;
+        const unsigned digit = unsigned(key >> shift) & 255u;
+        for (int w = 0; w < RS_WAVES; ++w) wcnt[w][t] = 0;
+        __syncthreads();
+        unsigned long long peers = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (digit >> b) & 1u;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const unsigned rank = unsigned(__popcll(peers & below));
+        if (valid && rank == 0) wcnt[wave][digit] = unsigned(__popcll(peers));
+        __syncthreads();
+        if (valid) {
+            unsigned pos = running[digit] + rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][digit];
+            if (pos < unsigned(n)) {
+                kout[pos] = key;
+                vout[pos] = vin[k];
+            }
+        }
+        __syncthreads();
+        unsigned add = 0;
+        for (int w = 0; w < RS_WAVES; ++w) add += wcnt[w][t];
+        running[t] += add;
+        __syncthreads();
+    }
+}
+
+// Sorts keys[src]/vals[src] over 63 key bits; returns the buffer index of the result.
+int radix_sort(nbhip_ctx* c, int n, int src)
+{
+    const int nb = std::max(1, (n + RS_TILE - 1) / RS_TILE);
+    c->bh_hist.grow(size_t(256) * nb + 1, 0, c->stream);
+    for (int shift = 0; shift < 64; shift += 8) {
+        const int dst = 1 - src;
+        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_BLOCK), 0, c->stream, c->bh_keys[src].ptr, n,
+                           shift, c->bh_hist.ptr);
+        NBHIP_CHECK(hipGetLastError());
+        exclusive_scan<unsigned>(c, c->bh_hist.ptr, 256 * nb, c->bh_hist.ptr, c->bh_uscan);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_BLOCK), 0, c->stream, c->bh_keys[src].ptr,
+                           c->bh_vals[src].ptr, c->bh_keys[dst].ptr, c->bh_vals[dst].ptr, n, shift,
+                           c->bh_hist.ptr);
+        NBHIP_CHECK(hipGetLastError());
+        src = dst;
+    }
+    return src;
+}
+
+// ---- 3. bodies in key order + FP64 moments -------------------------------------------------
+__global__ __launch_bounds__(256) void k_gather(const unsigned* perm, const float4* pm, int n, float4* spm,
+                                                Dbl4* mom)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const unsigned j = perm[k];
+    const float4 p = j < unsigned(n) ? pm[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    spm[k] = p;
+    const double m = p.w;
+    mom[k] = Dbl4{ m, m * double(p.x), m * double(p.y), m * double(p.z) };
+}
+
+// ---- 4. node structure from adjacent-key prefixes ---------------------------------------------
+struct KeyView {
+    const unsigned long long* K;
+    int n;
+
+    // common prefix of keys i and j in octal digits (21 when equal)
+    __device__ int cpl(int i, int j) const
+    {
+        const unsigned long long x = K[i] ^ K[j];
+        return x == 0ull ? LEVELS : (__clzll((long long)x) - 1) / 3;
+    }
+    // d(k) = cpl(k, k+1) for 0 <= k < n-1, else -1
+    __device__ int d(int k) const { return (k < 0 || k >= n - 1) ? -1 : cpl(k, k + 1); }
+    // largest b >= a with cpl(a, b) >= m (keys sorted: cpl(a, .) is non-increasing)
+    __device__ int last(int a, int m) const
+    {
+        int lo = a, step = 1;
+        for (;;) {
+            const int t = lo + step;
+            if (t >= n || cpl(a, t) < m) break;
+            lo = t;
+            step <<= 1;
+        }
+        while (step > 1) {
+            step >>= 1;
+            const int t = lo + step;
+            if (t < n && cpl(a, t) >= m) lo = t;
+        }
+        return lo;
+    }
+    // smallest a' <= a with cpl(a', a) >= m
+    __device__ int first(int a, int m) const
+    {
+        int hi = a, step = 1;
+        for (;;) {
+            const int t = hi - step;
+            if (t < 0 || cpl(t, a) < m) break;
+            hi = t;
+            step <<= 1;
+        }
+        while (step > 1) {
+            step >>= 1;
+            const int t = hi - step;
+            if (t >= 0 && cpl(t, a) >= m) hi = t;
+        }
+        return hi;
+    }
+};
+
+// The entries position a contributes to the pre-order, shallow to deep: internal nodes
+// and at most one bucket starting at a, then (if it is not inside a bucket) leaf a.
+struct Walk {
+    int cnt;          // internal + bucket entries
+    bool leaf;        // leaf a follows them
+    int lvl[LEVELS + 1];
+    int end[LEVELS + 1];
+    bool bucket[LEVELS + 1];
+};
+
+__device__ void walk_position(const KeyView& kv, int a, int leaf_size, Walk& w)
+{
+    w.cnt = 0;
+    w.leaf = false;
+    if (kv.n == 1) {
+        w.leaf = true;
+        return;
+    }
+    const int dl = kv.d(a - 1);
+    int tl[LEVELS + 1], te[LEVELS + 1];
+    int t = 0;
+    for (int m = kv.d(a); m > dl && t <= LEVELS;) {
+        const int b = kv.last(a, m);
+        tl[t] = m;
+        te[t] = b;
+        ++t;
+        m = kv.d(b);
+    }
+    for (int s = t - 1; s >= 0; --s) {
+        const int size = te[s] - a + 1;
+        if (size <= leaf_size || tl[s] == LEVELS) {
+            bool emit = true;
+            if (s == t - 1) {  // its parent starts before a (or it is the root)
+                const int pl = max(dl, kv.d(te[s]));
+                if (pl >= 0) emit = kv.last(a, pl) - kv.first(a, pl) + 1 > leaf_size;
+            }
+            if (emit) {
+                w.lvl[w.cnt] = tl[s];
+                w.end[w.cnt] = te[s];
+                w.bucket[w.cnt] = true;
+                ++w.cnt;
+            }
+            return;  // everything deeper at a, and body a itself, is inside the bucket
+        }
+        w.lvl[w.cnt] = tl[s];
+        w.end[w.cnt] = te[s];
+        w.bucket[w.cnt] = false;
+        ++w.cnt;
+    }
+    if (t > 0) {
+        w.leaf = true;  // the deepest node holding a starts at a and is internal
+    } else {
+        // the deepest node holding a starts before a, at level d(a-1)
+        const int size = kv.last(a, dl) - kv.first(a, dl) + 1;
+        w.leaf = size > leaf_size && dl < LEVELS;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_count(const unsigned long long* keys, int n, int leaf_size,
+                                               unsigned* cnt)
+{
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    if (a >= n) return;
+    Walk w;
+    walk_position(KeyView{ keys, n }, a, leaf_size, w);
+    cnt[a] = unsigned(w.cnt + (w.leaf ? 1 : 0));
+}
+
+struct EmitArgs {
+    const unsigned long long* keys;
+    int n;
+    int leaf_size;
+    const unsigned* base;  // n + 1 entries
+    const Dbl4* pref;      // n + 1 entries
+    const float4* spm;
+    float size;
+    float theta;
+    int opening;
+    BhNode* nodes;
+    unsigned capacity;     // nodes allocated; entries past it are never written
+};
+
+__global__ __launch_bounds__(256) void k_emit(EmitArgs e)
+{
+#pragma clang fp contract(off)
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    if (a >= e.n) return;
+    Walk w;
+    walk_position(KeyView{ e.keys, e.n }, a, e.leaf_size, w);
+    unsigned slot = e.base[a];
+    const float theta_sq = e.theta * e.theta;
+    for (int j = 0; j < w.cnt; ++j, ++slot) {
+        if (slot >= e.capacity) return;
+        const int b = w.end[j];
+        const Dbl4 hi = e.pref[b + 1], lo = e.pref[a];
+        const double M = hi.m - lo.m;
+        BhNode nd;
+        if (M != 0.0)
+            nd.cm = make_float4(float((hi.x - lo.x) / M), float((hi.y - lo.y) / M), float((hi.z - lo.z) / M),
+                                float(M));
+        else
+            nd.cm = make_float4(0.f, 0.f, 0.f, 0.f);
+        float s = e.size;  // the cell edge at this level, halved as the reference does
+        for (int l = 0; l < w.lvl[j]; ++l) s = s * .5f;
+        const float s2 = s * s;
+        nd.thr = e.opening == NBHIP_OPEN_TEXTBOOK ? s2 / theta_sq : s2 * theta_sq;
+        nd.next = e.base[b + 1];
+        nd.first = unsigned(a);
+        nd.count = w.bucket[j] ? unsigned(b - a + 1) : 0u;
+        e.nodes[slot] = nd;
+    }
+    if (w.leaf && slot < e.capacity) {
+        BhNode nd;
+        nd.cm = e.spm[a];
+        nd.thr = 0.f;
+        nd.next = e.base[a + 1];
+        nd.first = unsigned(a);
+        nd.count = 1;
+        e.nodes[slot] = nd;
+    }
+}
+
+// ---- 5. traversal ------------------------------------------------------------------------------
+struct WalkArgs {
+    const BhNode* nodes;
+    const unsigned* total;  // node count (= base[n])
+    unsigned capacity;      // nodes allocated
+    const float4* spm;      // bodies in key order
+    const unsigned* perm;   // key order -> body index
+    float4* pm;             // current positions (written in place when stepping)
+    float4* vr;
+    float4* acc;
+    unsigned* visits;       // per-body applied-node counts (COUNT only)
+    int n;
+    float G;
+    float eps2;
+    float dt;
+    float size;
+    int wrap;
+};
+
+template <bool CUT>
+__device__ __forceinline__ void interact(const float4 x, const float r2, const float4 q, const float eps2,
+                                         float& ax, float& ay, float& az)
+{
+    const float dx = q.x - x.x, dy = q.y - x.y, dz = q.z - x.z;
+    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
+    float w = q.w * (inv * inv * inv);
+    if (CUT) w = d2 > r2 ? w : 0.f;
+    ax = fmaf(w, dx, ax);
+    ay = fmaf(w, dy, ay);
+    az = fmaf(w, dz, az);
+}
+
+// Knuth's TwoSum: s + c += t with the rounding error of the addition kept in c.
+__device__ __forceinline__ void two_sum(float& s, float& c, float t)
+{
+    const float u = s + t;
+    const float bp = u - s;
+    const float e = (s - (u - bp)) + (t - bp);
+    s = u;
+    c += e;
+}
+
+// Per-body force sum: interactions accumulate in plain FP32 and the partial is folded
+// into a compensated running sum every FOLD interactions.
+struct Accum {
+    static constexpr unsigned FOLD = 32;
+    float tx = 0.f, ty = 0.f, tz = 0.f;      // current partial
+    float sx = 0.f, sy = 0.f, sz = 0.f;      // running sums
+    float cx = 0.f, cy = 0.f, cz = 0.f;      // their compensation
+    unsigned k = 0;
+
+    __device__ void fold()
+    {
+        two_sum(sx, cx, tx);
+        two_sum(sy, cy, ty);
+        two_sum(sz, cz, tz);
+        tx = ty = tz = 0.f;
+        k = 0;
+    }
+    __device__ void tick(unsigned m = 1)
+    {
+        k += m;
+        if (k >= FOLD) fold();
+    }
+    __device__ float3 total()
+    {
+        fold();
+        return make_float3(sx + cx, sy + cy, sz + cz);
+    }
+};
+
+template <bool CUT, bool STEP, bool COUNT>
+__global__ __launch_bounds__(256) void k_walk(WalkArgs a)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = k < a.n;
+    const unsigned total = min(*a.total, a.capacity);
+    const float4 x = valid ? a.spm[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const unsigned body = valid ? min(a.perm[k], unsigned(a.n - 1)) : 0u;
+    float r2 = 0.f;
+    if (CUT && valid) {
+        const float r = a.vr[body].w;
+        r2 = r * r;
+    }
+    unsigned idx = valid ? 0u : total;
+    Accum acc;
+    unsigned applied = 0;
+    for (;;) {
+        const unsigned cur = __ockl_wfred_min_u32(idx);  // wave-uniform
+        if (cur >= total) break;
+        const BhNode nd = a.nodes[cur];
+        if (idx != cur) continue;
+        unsigned nxt = nd.next;
+        if (nd.cm.w != 0.f) {  // empty nodes are skipped (reference bhtree.cpp:108-113)
+            bool agg = nd.count == 1;
+            if (!agg) {
+#pragma clang fp contract(off)
+                const float dx = nd.cm.x - x.x, dy = nd.cm.y - x.y, dz = nd.cm.z - x.z;
+                agg = ((dx * dx) + (dy * dy)) + (dz * dz) > nd.thr;
+            }
+            if (agg) {
+                interact<CUT>(x, r2, nd.cm, a.eps2, acc.tx, acc.ty, acc.tz);
+                acc.tick();
+                if (COUNT) ++applied;
+            } else if (nd.count == 0) {
+                nxt = cur + 1;  // open: the first child follows in pre-order
+            } else {
+                const unsigned stop = min(nd.first + nd.count, unsigned(a.n));
+                for (unsigned b = nd.first; b < stop; ++b)
+                    interact<CUT>(x, r2, a.spm[b], a.eps2, acc.tx, acc.ty, acc.tz);
+                acc.tick(nd.count);
+                if (COUNT) applied += nd.count;
+            }
+        }
+        idx = max(nxt, cur + 1);  // pre-order pointers only move forward
+    }
+    if (!valid) return;
+    const float3 t = acc.total();
+    const float ax = a.G * t.x, ay = a.G * t.y, az = a.G * t.z;
+    a.acc[body] = make_float4(ax, ay, az, 0.f);
+    if (COUNT) a.visits[body] = applied;
+    if (STEP) {
+#pragma clang fp contract(off)
+        float4 v = a.vr[body];
+        float4 p = x;
+        v.x = v.x + ax * a.dt;
+        v.y = v.y + ay * a.dt;
+        v.z = v.z + az * a.dt;
+        p.x = p.x + v.x * a.dt;
+        p.y = p.y + v.y * a.dt;
+        p.z = p.z + v.z * a.dt;
+        if (a.wrap && a.size > 0.f) {
+            const float half = a.size * .5f;
+            p.x = fmodf(fmodf(p.x + half, a.size) + a.size, a.size) - half;
+            p.y = fmodf(fmodf(p.y + half, a.size) + a.size, a.size) - half;
+            p.z = fmodf(fmodf(p.z + half, a.size) + a.size, a.size) - half;
+        }
+        a.pm[body] = p;
+        a.vr[body] = v;
+    }
+}
+
+unsigned grid(size_t n) { return unsigned((n + 255) / 256); }
+
+}  // namespace
+
+void bh_build(nbhip_ctx* c, const nbhip_params& p)
+{
+    const int n = int(c->count);
+    hipStream_t s = c->stream;
+    c->bh_keys[0].grow(n, 0, s);
+    c->bh_keys[1].grow(n, 0, s);
+    c->bh_vals[0].grow(n, 0, s);
+    c->bh_vals[1].grow(n, 0, s);
+    c->bh_spm.grow(n, 0, s);
+    c->bh_moments.grow(n, 0, s);
+    c->bh_pref.grow(size_t(n) + 1, 0, s);
+    c->bh_base.grow(size_t(n) + 1, 0, s);
+    c->bh_nodes.grow(2 * size_t(n) + 1, 0, s);
+
+    hipLaunchKernelGGL(k_keys, dim3(grid(n)), dim3(256), 0, s, c->pm[c->cur].ptr, n, p.size,
+                       c->bh_keys[0].ptr, c->bh_vals[0].ptr);
+    NBHIP_CHECK(hipGetLastError());
+    const int sorted = radix_sort(c, n, 0);
+    c->bh_sorted = sorted;
+    const unsigned long long* keys = c->bh_keys[sorted].ptr;
+    const unsigned* perm = c->bh_vals[sorted].ptr;
+
+    hipLaunchKernelGGL(k_gather, dim3(grid(n)), dim3(256), 0, s, perm, c->pm[c->cur].ptr, n, c->bh_spm.ptr,
+                       c->bh_moments.ptr);
+    NBHIP_CHECK(hipGetLastError());
+    exclusive_scan<Dbl4>(c, c->bh_moments.ptr, n, c->bh_pref.ptr, c->bh_dscan);
+
+    const int leaf = std::max(1, p.leaf_size);
+    hipLaunchKernelGGL(k_count, dim3(grid(n)), dim3(256), 0, s, keys, n, leaf, c->bh_base.ptr);
+    NBHIP_CHECK(hipGetLastError());
+    exclusive_scan<unsigned>(c, c->bh_base.ptr, n, c->bh_base.ptr, c->bh_uscan);
+
+    EmitArgs e{ keys, n, leaf, c->bh_base.ptr, c->bh_pref.ptr, c->bh_spm.ptr, p.size, p.theta,
+                p.opening, c->bh_nodes.ptr, unsigned(c->bh_nodes.cap) };
+    hipLaunchKernelGGL(k_emit, dim3(grid(n)), dim3(256), 0, s, e);
+    NBHIP_CHECK(hipGetLastError());
+    c->bh_built = true;
+}
+
+void bh_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse, bool count)
+{
+    const size_t n = c->count;
+    bh_build(c, p);
+    ensure_radius_flag(c);
+    const bool cut = c->any_radius || p.eps == 0.f;
+    if (count) c->bh_visits.grow(n, 0, c->stream);
+    WalkArgs a{};
+    a.nodes = c->bh_nodes.ptr;
+    a.total = c->bh_base.ptr + n;
+    a.capacity = unsigned(c->bh_nodes.cap);
+    a.spm = c->bh_spm.ptr;
+    a.perm = c->bh_vals[c->bh_sorted].ptr;
+    a.pm = c->pm[c->cur].ptr;
+    a.vr = c->vr.ptr;
+    a.acc = c->acc.ptr;
+    a.visits = c->bh_visits.ptr;
+    a.n = int(n);
+    a.G = p.G;
+    a.eps2 = p.eps * p.eps;
+    a.dt = p.dt;
+    a.size = p.size;
+    a.wrap = p.wrap;
+    using K = void (*)(WalkArgs);
+    K k;
+    if (count)
+        k = cut ? (fuse ? (K)k_walk<true, true, true> : (K)k_walk<true, false, true>)
+                : (fuse ? (K)k_walk<false, true, true> : (K)k_walk<false, false, true>);
+    else
+        k = cut ? (fuse ? (K)k_walk<true, true, false> : (K)k_walk<true, false, false>)
+                : (fuse ? (K)k_walk<false, true, false> : (K)k_walk<false, false, false>);
+    hipLaunchKernelGGL(k, dim3(grid(n)), dim3(256), 0, c->stream, a);
+    NBHIP_CHECK(hipGetLastError());
+    c->launches += 1;
+}
+
+void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm)
+{
+    const int n = int(c->count);
+    hipStream_t s = c->stream;
+    for (int i = 0; i < 2; ++i) {
+        c->bh_keys[i].grow(n, 0, s);
+        c->bh_vals[i].grow(n, 0, s);
+    }
+    hipLaunchKernelGGL(k_keys, dim3(grid(n)), dim3(256), 0, s, c->pm[c->cur].ptr, n, p.size,
+                       c->bh_keys[0].ptr, c->bh_vals[0].ptr);
+    NBHIP_CHECK(hipGetLastError());
+    if (keys) {
+        NBHIP_CHECK(hipMemcpyAsync(keys, c->bh_keys[0].ptr, size_t(n) * sizeof(uint64_t),
+                                   hipMemcpyDeviceToHost, s));
+        NBHIP_CHECK(hipStreamSynchronize(s));
+    }
+    const int sorted = radix_sort(c, n, 0);
+    if (perm) {
+        NBHIP_CHECK(hipMemcpyAsync(perm, c->bh_vals[sorted].ptr, size_t(n) * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, s));
+    }
+    NBHIP_CHECK(hipStreamSynchronize(s));
+}
+
+void bh_release(nbhip_ctx* c)
+{
+    for (int i = 0; i < 2; ++i) {
+        c->bh_keys[i].release();
+        c->bh_vals[i].release();
+    }
+    c->bh_spm.release();
+    c->bh_moments.release();
+    c->bh_pref.release();
+    c->bh_base.release();
+    c->bh_nodes.release();
+    c->bh_hist.release();
+    c->bh_uscan.release();
+    c->bh_dscan.release();
+    c->bh_visits.release();
+}
+
+}  // namespace nbhip

@@ -65,6 +65,22 @@ struct DirectConfig {
     int P = 4;       // i-bodies per lane
     int W = 4;       // waves per block (all waves share the block's i-set, split j)
     int jsplit = 1;  // blocks along j per i-block (>1 -> partial sums + reduce pass)
+    bool rcp = false;  // |d|^-3 as rsq * rcp instead of rsq^3
+};
+
+// One node of the compressed octree, in DFS pre-order (32 bytes, see bh.hip).
+struct BhNode {
+    float4 cm;         // centre of mass xyz, mass
+    unsigned next;     // pre-order index of the first node after this subtree
+    float thr;         // accepted when |com - x|^2 > thr
+    unsigned first;    // first body (key order) of a leaf / bucket
+    unsigned count;    // 0: internal node; >= 1: leaf holding `count` bodies
+};
+static_assert(sizeof(BhNode) == sizeof(nbhip_bh_node), "BhNode must match nbhip_bh_node");
+
+// FP64 running sums of m, m x, m y, m z over bodies in key order.
+struct Dbl4 {
+    double m, x, y, z;
 };
 
 }  // namespace nbhip
@@ -93,6 +109,22 @@ struct nbhip_ctx {
     nbhip::DevBuf<float> fscratch;
     nbhip::DevBuf<unsigned> uscratch;
 
+    // Barnes-Hut: keys + permutation (double-buffered for the radix sort), bodies in key
+    // order, FP64 prefix sums, per-position node counts / pre-order bases, the nodes
+    nbhip::DevBuf<unsigned long long> bh_keys[2];
+    nbhip::DevBuf<unsigned> bh_vals[2];
+    nbhip::DevBuf<float4> bh_spm;
+    nbhip::DevBuf<nbhip::Dbl4> bh_moments;
+    nbhip::DevBuf<nbhip::Dbl4> bh_pref;
+    nbhip::DevBuf<unsigned> bh_base;
+    nbhip::DevBuf<nbhip::BhNode> bh_nodes;
+    nbhip::DevBuf<unsigned> bh_hist;
+    nbhip::DevBuf<unsigned> bh_uscan;
+    nbhip::DevBuf<nbhip::Dbl4> bh_dscan;
+    nbhip::DevBuf<unsigned> bh_visits;
+    int bh_sorted = 0;          // which keys/vals buffer holds the sorted result
+    bool bh_built = false;      // a tree of the current count exists on the device
+
     // timing
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     long launches = 0;
@@ -113,5 +145,11 @@ void ensure_radius_flag(nbhip_ctx* c);
 void aos_to_split(nbhip_ctx* c, size_t n);
 void split_to_aos(nbhip_ctx* c, size_t n);
 DirectConfig choose_direct_config(size_t n);
+
+// bh.hip
+void bh_build(nbhip_ctx* c, const nbhip_params& p);
+void bh_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse_integrate, bool count);
+void bh_release(nbhip_ctx* c);
+void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm);
 
 }  // namespace nbhip

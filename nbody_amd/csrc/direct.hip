@@ -103,7 +103,10 @@ __device__ __forceinline__ void finish_body(const ForceArgs& a, int i, float4 xi
     }
 }
 
-template <int P, int W, bool CUT>
+// RCP selects how |d|^-3 is formed: false -> inv * inv * inv with inv = rsq(d2) (three
+// VALU multiplies with the mass); true -> inv * rcp(d2) (the reciprocal goes to the
+// transcendental unit, saving one VALU multiply per interaction).
+template <int P, int W, bool CUT, bool RCP>
 __global__ __launch_bounds__(WAVE* W) void k_force(ForceArgs a)
 {
     __shared__ float4 tile[WAVE * W];
@@ -159,12 +162,14 @@ __global__ __launch_bounds__(WAVE* W) void k_force(ForceArgs a)
                 float w;
                 if (CUT) {
                     const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
-                    w = d2 > r2[p] ? q.w * (inv * inv * inv) : 0.f;
+                    const float e2 = d2 + eps2;
+                    const float inv = __builtin_amdgcn_rsqf(e2);
+                    const float inv3 = RCP ? inv * __builtin_amdgcn_rcpf(e2) : inv * inv * inv;
+                    w = d2 > r2[p] ? q.w * inv3 : 0.f;
                 } else {
-                    const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-                    const float inv = __builtin_amdgcn_rsqf(d2);
-                    w = q.w * (inv * inv * inv);
+                    const float e2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+                    const float inv = __builtin_amdgcn_rsqf(e2);
+                    w = RCP ? (q.w * inv) * __builtin_amdgcn_rcpf(e2) : q.w * (inv * inv * inv);
                 }
                 T[p][0] = fmaf(w, dx, T[p][0]);
                 T[p][1] = fmaf(w, dy, T[p][1]);
@@ -397,15 +402,16 @@ __global__ __launch_bounds__(256) void k_split_to_aos(float* aos, int n, const f
 using ForceKernel = void (*)(ForceArgs);
 
 template <int P, int W>
-ForceKernel pick_cut(bool cut)
+ForceKernel pick_cut(bool cut, bool rcp)
 {
-    return cut ? (ForceKernel)k_force<P, W, true> : (ForceKernel)k_force<P, W, false>;
+    if (rcp) return cut ? (ForceKernel)k_force<P, W, true, true> : (ForceKernel)k_force<P, W, false, true>;
+    return cut ? (ForceKernel)k_force<P, W, true, false> : (ForceKernel)k_force<P, W, false, false>;
 }
 
-ForceKernel pick_force(int P, int W, bool cut)
+ForceKernel pick_force(int P, int W, bool cut, bool rcp)
 {
 #define NBHIP_CASE(p, w) \
-    if (P == p && W == w) return pick_cut<p, w>(cut);
+    if (P == p && W == w) return pick_cut<p, w>(cut, rcp);
     NBHIP_CASE(1, 1) NBHIP_CASE(1, 2) NBHIP_CASE(1, 4) NBHIP_CASE(1, 8)
     NBHIP_CASE(2, 1) NBHIP_CASE(2, 2) NBHIP_CASE(2, 4) NBHIP_CASE(2, 8)
     NBHIP_CASE(4, 1) NBHIP_CASE(4, 2) NBHIP_CASE(4, 4) NBHIP_CASE(4, 8)
@@ -429,6 +435,7 @@ DirectConfig choose_direct_config(size_t n)
     DirectConfig c;
     c.P = env_int("NBHIP_DIRECT_P", 4);
     c.W = env_int("NBHIP_DIRECT_W", 4);
+    c.rcp = env_int("NBHIP_DIRECT_RCP", 0) != 0;
     const size_t per_block = size_t(WAVE) * c.P;
     const size_t iblocks = (n + per_block - 1) / per_block;
     const int forced = env_int("NBHIP_DIRECT_JSPLIT", 0);
@@ -470,7 +477,7 @@ void direct_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse)
     ensure_radius_flag(c);
     const bool cut = c->any_radius || p.eps == 0.f;
     DirectConfig cfg = choose_direct_config(n);
-    ForceKernel k = pick_force(cfg.P, cfg.W, cut);
+    ForceKernel k = pick_force(cfg.P, cfg.W, cut, cfg.rcp);
     if (!k) fail(NBHIP_ERR_INVALID, "no force kernel for P=%d W=%d", cfg.P, cfg.W);
 
     const size_t tile = size_t(WAVE) * cfg.W;

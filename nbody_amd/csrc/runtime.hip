@@ -4,6 +4,7 @@
 // kernels live in direct.hip.
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "common.hpp"
 
@@ -59,6 +60,13 @@ void check_params(const nbhip_params* p)
     if (p->mode != NBHIP_MODE_N2 && p->mode != NBHIP_MODE_NLOGN)
         fail(NBHIP_ERR_INVALID, "unknown mode %d", p->mode);
     if (!(p->eps >= 0.f)) fail(NBHIP_ERR_INVALID, "eps must be >= 0");
+    if (p->mode == NBHIP_MODE_NLOGN) {
+        if (p->opening != NBHIP_OPEN_REFERENCE && p->opening != NBHIP_OPEN_TEXTBOOK)
+            fail(NBHIP_ERR_INVALID, "unknown opening criterion %d", p->opening);
+        if (p->leaf_size < 1) fail(NBHIP_ERR_INVALID, "leaf_size must be >= 1");
+        if (!(p->size > 0.f)) fail(NBHIP_ERR_INVALID, "NLogN needs a positive root size");
+        if (!(p->theta >= 0.f)) fail(NBHIP_ERR_INVALID, "theta must be >= 0");
+    }
 }
 
 // Is `name` a gfx950 architecture string ("gfx950", "gfx950:sramecc+:xnack-", ...)?
@@ -154,6 +162,7 @@ void nbhip_destroy(nbhip_ctx* c)
     c->dred.release();
     c->fscratch.release();
     c->uscratch.release();
+    nbhip::bh_release(c);
     if (c->ev_start) (void)hipEventDestroy(c->ev_start);
     if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -279,7 +288,7 @@ int nbhip_accelerate(nbhip_ctx* c, const nbhip_params* p)
         if (p->mode == NBHIP_MODE_N2)
             nbhip::direct_accelerate(c, *p, false);
         else
-            fail(NBHIP_ERR_UNSUPPORTED, "NLogN mode is not available in this build");
+            nbhip::bh_accelerate(c, *p, false, false);
     });
 }
 
@@ -306,7 +315,7 @@ int nbhip_step(nbhip_ctx* c, const nbhip_params* p, int steps)
             if (p->mode == NBHIP_MODE_N2)
                 nbhip::direct_accelerate(c, *p, true);
             else
-                fail(NBHIP_ERR_UNSUPPORTED, "NLogN mode is not available in this build");
+                nbhip::bh_accelerate(c, *p, true, false);
         }
     });
 }
@@ -347,6 +356,63 @@ int nbhip_force_error_f64(nbhip_ctx* c, const nbhip_params* p, const uint32_t* s
                      c->count);
         NBHIP_CHECK(hipSetDevice(c->device));
         nbhip::force_error_f64(c, *p, sample, ns, acc64, rel_err);
+    });
+}
+
+int nbhip_bh_keys(nbhip_ctx* c, const nbhip_params* p, uint64_t* keys, uint32_t* perm)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        check_params(p);
+        if (c->count == 0) return;
+        NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::bh_keys(c, *p, keys, perm);
+    });
+}
+
+int nbhip_bh_nodes(nbhip_ctx* c, const nbhip_params* p, nbhip_bh_node* nodes, size_t cap,
+                   size_t* count)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        check_params(p);
+        if (count) *count = 0;
+        if (c->count == 0) return;
+        NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::bh_build(c, *p);
+        unsigned total = 0;
+        NBHIP_CHECK(hipMemcpyAsync(&total, c->bh_base.ptr + c->count, sizeof total,
+                                   hipMemcpyDeviceToHost, c->stream));
+        NBHIP_CHECK(hipStreamSynchronize(c->stream));
+        if (count) *count = total;
+        const size_t k = total < cap ? total : cap;
+        if (k) {
+            if (!nodes) fail(NBHIP_ERR_INVALID, "null nodes");
+            NBHIP_CHECK(hipMemcpyAsync(nodes, c->bh_nodes.ptr, k * sizeof(nbhip_bh_node),
+                                       hipMemcpyDeviceToHost, c->stream));
+            NBHIP_CHECK(hipStreamSynchronize(c->stream));
+        }
+    });
+}
+
+int nbhip_bh_interactions(nbhip_ctx* c, const nbhip_params* p, uint32_t* per_body,
+                          uint64_t* total)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        check_params(p);
+        if (total) *total = 0;
+        if (c->count == 0) return;
+        NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::bh_accelerate(c, *p, false, true);
+        std::vector<uint32_t> v(c->count);
+        NBHIP_CHECK(hipMemcpyAsync(v.data(), c->bh_visits.ptr, v.size() * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, c->stream));
+        NBHIP_CHECK(hipStreamSynchronize(c->stream));
+        uint64_t sum = 0;
+        for (uint32_t x : v) sum += x;
+        if (total) *total = sum;
+        if (per_body) std::memcpy(per_body, v.data(), v.size() * sizeof(uint32_t));
     });
 }
 

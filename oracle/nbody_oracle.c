@@ -13,6 +13,9 @@
  *   orc_bh_build         source/detail/tree.h:14-29 + source/bhtree.cpp:14-95
  *                        (+ include/nbody/bounds.h:31-52 octant code / child bounds)
  *   orc_bh_accelerate    source/solvers/cpu_barnes_hut.h:22-44 + source/bhtree.cpp:97-138
+ *   orc_bh_accelerate_f64com  the same tree and traversal with FP64 centres of mass and
+ *                        FP64 sums (separates the reference's FP32 rounding from the
+ *                        Barnes-Hut approximation itself)
  *   orc_morton_keys      the Bounds::quadrant descent of source/bhtree.cpp:19-24 written
  *                        as a key: 3 bits per level, digit = reference octant code
  *   orc_stable_sort      stable ascending sort of keys, ties by index (std::stable_sort)
@@ -542,6 +545,90 @@ void orc_bh_accelerate(float* bodies, size_t n, const orc_node* nodes, size_t nn
 {
     orc_bh_arg a = { bodies, nodes, nn, theta, G, eps * eps, visits };
     orc_parallel_blocks(n, nthreads, orc_bh_block, &a);
+}
+
+/* The same tree and traversal with exact centres of mass: every node's mass and centre
+ * are recomputed in FP64 from its subtree (children always follow their parent in the
+ * reference's node array, so one reverse sweep does it), the acceptance test uses that
+ * centre rounded to FP32 with the reference's FP32 operation order, and the applied
+ * forces are summed in FP64.  This is the Barnes-Hut approximation without the rounding
+ * of the reference's incremental FP32 means (source/bhtree.cpp:87-95). */
+typedef struct {
+    float* b; const orc_node* nodes; size_t nn; float theta; double G, eps2;
+    const double* nm; /* per node: m, m x, m y, m z (FP64) */
+    uint64_t* visits;
+} orc_bh64_arg;
+
+static void orc_bh64_block(void* p, size_t begin, size_t end)
+{
+    const orc_bh64_arg* a = (const orc_bh64_arg*)p;
+    const float theta_sq = a->theta * a->theta;
+    for (size_t i = begin; i < end; ++i) {
+        float* body = a->b + i * BF;
+        const double x = body[0], y = body[1], z = body[2];
+        const double r2 = (double)body[3] * (double)body[3];
+        double acc[3] = { 0, 0, 0 };
+        uint64_t visits = 0;
+        uint32_t idx = 0;
+        do {
+            const orc_node* nd = &a->nodes[idx];
+            const double M = a->nm[4 * idx];
+            if (nd->mass == 0 || M == 0) {
+                idx = nd->next;
+                continue;
+            }
+            const double cx = a->nm[4 * idx + 1] / M, cy = a->nm[4 * idx + 2] / M, cz = a->nm[4 * idx + 3] / M;
+            int apply = nd->children == 0;
+            if (!apply) {
+                const float fx = (float)cx - body[0], fy = (float)cy - body[1], fz = (float)cz - body[2];
+                const float dist_sq = ((fx * fx) + (fy * fy)) + (fz * fz);
+                apply = dist_sq > (nd->size * nd->size) * theta_sq;
+            }
+            if (apply) {
+                const double dx = cx - x, dy = cy - y, dz = cz - z;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 > r2) {
+                    const double e2 = d2 + a->eps2;
+                    const double w = a->G * M / (e2 * sqrt(e2));
+                    acc[0] += w * dx;
+                    acc[1] += w * dy;
+                    acc[2] += w * dz;
+                }
+                ++visits;
+                idx = nd->next;
+                continue;
+            }
+            idx = nd->children;
+        } while (0 < idx && idx < a->nn);
+        body[8] = (float)acc[0];
+        body[9] = (float)acc[1];
+        body[10] = (float)acc[2];
+        if (a->visits)
+            a->visits[i] = visits;
+    }
+}
+
+void orc_bh_accelerate_f64com(float* bodies, size_t n, const orc_node* nodes, size_t nn,
+                              float theta, double G, double eps, uint64_t* visits, int nthreads)
+{
+    double* nm = (double*)calloc(4 * (nn ? nn : 1), sizeof(double));
+    for (size_t i = nn; i-- > 0;) {
+        const orc_node* nd = &nodes[i];
+        if (nd->children == 0) {
+            /* a leaf: its body, or bodies merged below FLT_EPSILON at the first one's place */
+            nm[4 * i] = nd->mass;
+            nm[4 * i + 1] = (double)nd->mass * nd->mx;
+            nm[4 * i + 2] = (double)nd->mass * nd->my;
+            nm[4 * i + 3] = (double)nd->mass * nd->mz;
+        } else {
+            for (uint32_t c = nd->children; c < nd->children + 8; ++c)
+                for (int k = 0; k < 4; ++k)
+                    nm[4 * i + k] += nm[4 * c + k];
+        }
+    }
+    orc_bh64_arg a = { bodies, nodes, nn, theta, G, eps * eps, nm, visits };
+    orc_parallel_blocks(n, nthreads, orc_bh64_block, &a);
+    free(nm);
 }
 
 size_t orc_node_size(void) { return sizeof(orc_node); }

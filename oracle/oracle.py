@@ -36,6 +36,14 @@ _nodep = np.ctypeslib.ndpointer(NODE_DTYPE, flags="C_CONTIGUOUS")
 _lib = None
 _ref = None
 
+# Worker threads when a caller passes threads=0: the host's cores, at most 16 (the CPU
+# share of one GPU on the benchmark hosts).
+DEFAULT_THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def _t(threads: int) -> int:
+    return threads if threads > 0 else DEFAULT_THREADS
+
 
 def build(ref: bool = True) -> None:
     """Compile liboracle.so (and _ref/libnbref.so when the reference tree is present)."""
@@ -67,6 +75,8 @@ def lib():
     L.orc_bh_build.argtypes = [_f32p, sz, C.c_float, _nodep, sz]
     L.orc_bh_accelerate.argtypes = [_f32p, sz, _nodep, sz, C.c_float, C.c_float, C.c_float,
                                     C.c_void_p, C.c_int]
+    L.orc_bh_accelerate_f64com.argtypes = [_f32p, sz, _nodep, sz, C.c_float, C.c_double,
+                                           C.c_double, C.c_void_p, C.c_int]
     L.orc_node_size.restype = sz
     assert L.orc_node_size() == 48
     _lib = L
@@ -127,20 +137,20 @@ def wrap(x: float, size: float) -> float:
 def brute_force(bodies: np.ndarray, G=1.0, eps=0.0, threads=0) -> np.ndarray:
     """Accelerations in place (bodies[:, 8:11]); returns bodies."""
     b = _bodies(bodies)
-    lib().orc_brute_force(b, b.shape[0], G, eps, threads)
+    lib().orc_brute_force(b, b.shape[0], G, eps, _t(threads))
     return b
 
 
 def brute_force_rows(bodies: np.ndarray, rows: np.ndarray, G=1.0, eps=0.0, threads=0) -> np.ndarray:
     b = _bodies(bodies)
     r = np.ascontiguousarray(rows, np.uint32)
-    lib().orc_brute_force_rows(b, b.shape[0], r, r.size, G, eps, threads)
+    lib().orc_brute_force_rows(b, b.shape[0], r, r.size, G, eps, _t(threads))
     return b
 
 
 def integrate(bodies: np.ndarray, dt: float, size=10000.0, wrap=True, threads=0) -> np.ndarray:
     b = _bodies(bodies)
-    lib().orc_integrate(b, b.shape[0], dt, size, int(bool(wrap)), threads)
+    lib().orc_integrate(b, b.shape[0], dt, size, int(bool(wrap)), _t(threads))
     return b
 
 
@@ -149,11 +159,11 @@ def direct_f64(bodies: np.ndarray, sample=None, G=1.0, eps=0.0, threads=0) -> np
     b = _bodies(bodies)
     if sample is None:
         out = np.zeros((b.shape[0], 3), np.float64)
-        lib().orc_direct_f64(b, b.shape[0], None, 0, G, eps, out, threads)
+        lib().orc_direct_f64(b, b.shape[0], None, 0, G, eps, out, _t(threads))
     else:
         s = np.ascontiguousarray(sample, np.uint32)
         out = np.zeros((s.size, 3), np.float64)
-        lib().orc_direct_f64(b, b.shape[0], s.ctypes.data, s.size, G, eps, out, threads)
+        lib().orc_direct_f64(b, b.shape[0], s.ctypes.data, s.size, G, eps, out, _t(threads))
     return out
 
 
@@ -161,7 +171,7 @@ def energy_f64(bodies: np.ndarray, G=1.0, eps=0.0, threads=0) -> np.ndarray:
     """[kinetic, potential, px, py, pz, Lx, Ly, Lz, mass] in FP64."""
     b = _bodies(bodies)
     out = np.zeros(9, np.float64)
-    lib().orc_energy_f64(b, b.shape[0], G, eps, out, threads)
+    lib().orc_energy_f64(b, b.shape[0], G, eps, out, _t(threads))
     return out
 
 
@@ -198,7 +208,20 @@ def bh_accelerate(bodies: np.ndarray, size: float, theta=0.5, G=1.0, eps=0.0, th
         nodes = bh_build(b, size)
     v = np.zeros(b.shape[0], np.uint64) if visits else None
     lib().orc_bh_accelerate(b, b.shape[0], nodes, nodes.size, theta, G, eps,
-                            v.ctypes.data if v is not None else None, threads)
+                            v.ctypes.data if v is not None else None, _t(threads))
+    return b, v
+
+
+def bh_accelerate_f64com(bodies: np.ndarray, size: float, theta=0.5, G=1.0, eps=0.0, threads=0,
+                         nodes=None, visits=False):
+    """The reference tree and traversal with FP64 centres of mass and FP64 force sums
+    (the acceptance test still in FP32).  Returns (bodies, visits or None)."""
+    b = _bodies(bodies)
+    if nodes is None:
+        nodes = bh_build(b, size)
+    v = np.zeros(b.shape[0], np.uint64) if visits else None
+    lib().orc_bh_accelerate_f64com(b, b.shape[0], nodes, nodes.size, theta, G, eps,
+                                   v.ctypes.data if v is not None else None, _t(threads))
     return b, v
 
 

@@ -197,3 +197,22 @@ def test_bh_special_cases(orc):
     assert np.allclose(one[:, 8:11], expect, rtol=1e-4)
     # mass conservation in the tree (tests/source/test_bhtree.cpp:169-212)
     assert np.isclose(nodes[0]["mass"], b[:, 7].sum(), rtol=1e-5)
+
+
+def test_bh_f64com_oracle(orc):
+    # Pins the FP64-centre variant: same traversal as the reference on the golden disk,
+    # forces within the reference's own FP32 rounding, and with every cell opened it is
+    # the FP64 direct sum.
+    g = np.load(os.path.join(GOLD, "barnes_hut.npz"))
+    b = g["bodies"].copy()
+    _, v = orc.bh_accelerate_f64com(b, 10000.0, theta=0.5, visits=True)
+    assert np.array_equal(v, g["visits_theta05"])
+    ref = g["out_theta05"][:, 8:11].astype(np.float64)
+    rel = np.linalg.norm(b[:, 8:11] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert rel.max() < 1e-4
+    from nbody_amd import ics
+    p = ics.plummer(1500, seed=4)
+    x = p.copy()
+    orc.bh_accelerate_f64com(x, 64.0, theta=1e6, eps=0.01)
+    a64 = orc.direct_f64(p, eps=0.01)
+    assert (np.linalg.norm(x[:, 8:11] - a64, axis=1) / np.linalg.norm(a64, axis=1)).max() < 1e-6
