@@ -199,6 +199,54 @@ int nbhip_bh_nodes(nbhip_ctx* ctx, const nbhip_params* p, nbhip_bh_node* nodes, 
 int nbhip_bh_interactions(nbhip_ctx* ctx, const nbhip_params* p, uint32_t* per_body,
                           uint64_t* total);
 
+/* ---- batches: many independent direct-sum systems stepped together ------------------
+ * No reference counterpart: the reference runs one simulation per Sim.  A batch lives on
+ * a context (its stream, its device) and holds systems of any size, each with its own
+ * parameters (dt, G, eps, size, wrap; mode must be N2).  One step advances every system
+ * with one persistent launch over a work table of (system, block of bodies) items,
+ * largest systems first; a body's force is summed in the same order whatever else is in
+ * the batch, so each system's result is the same alone and in any mix. */
+typedef struct nbhip_batch nbhip_batch;
+
+#define NBHIP_BATCH_GRAPH      1u  /* capture the step sequence in a hipGraph and replay it */
+#define NBHIP_BATCH_PER_SYSTEM 2u  /* one launch per system per step: the unbatched path   */
+#define NBHIP_BATCH_ACCELERATE 4u  /* accelerations only, no kick-drift                    */
+
+typedef struct nbhip_batch_info {
+    size_t systems;
+    size_t bodies;
+    size_t items;                  /* work-table entries */
+    double interactions_per_step;  /* sum of N^2 */
+    size_t device_bytes;           /* body storage held on the device */
+    int    P, W, persistent;       /* launch configuration in use */
+} nbhip_batch_info;
+
+int  nbhip_batch_create(nbhip_ctx* ctx, nbhip_batch** out);
+void nbhip_batch_destroy(nbhip_batch* b);
+
+/* Add a system of n reference `Body` records; *id receives its handle.  Uploads. */
+int nbhip_batch_add(nbhip_batch* b, const void* bodies, size_t n, const nbhip_params* p, int* id);
+
+/* Remove a system and free its device storage. */
+int nbhip_batch_remove(nbhip_batch* b, int id);
+
+/* Advance every system `steps` steps (flags: NBHIP_BATCH_*).  Enqueues only. */
+int nbhip_batch_step(nbhip_batch* b, int steps, unsigned flags);
+
+/* Download min(n, N) bodies of system `id`.  Synchronises. */
+int nbhip_batch_download(nbhip_batch* b, int id, void* bodies, size_t n);
+
+int nbhip_batch_query(nbhip_batch* b, nbhip_batch_info* out);
+
+/* Choose the launch configuration by hand: P bodies per lane, W waves per block,
+ * persistent (1) or one block per item (0). */
+int nbhip_batch_configure(nbhip_batch* b, int P, int W, int persistent);
+
+/* Autotuner: time the compiled (P, W, persistent) candidates on the batch as it is
+ * (accelerations only, the state does not move), keep the fastest; *best_ms (may be
+ * NULL) receives its time per step.  Synchronises. */
+int nbhip_batch_tune(nbhip_batch* b, double* best_ms);
+
 /* ---- timing: HIP events on the context's stream ----------------------------------- */
 
 /* Record a start event on the context's stream. */

@@ -47,6 +47,20 @@ class Params(C.Structure):
     ]
 
 
+BATCH_GRAPH = 1
+BATCH_PER_SYSTEM = 2
+BATCH_ACCELERATE = 4
+
+
+class BatchInfo(C.Structure):
+    """nbhip_batch_info (include/nbody_hip.h)."""
+    _fields_ = [
+        ("systems", C.c_size_t), ("bodies", C.c_size_t), ("items", C.c_size_t),
+        ("interactions_per_step", C.c_double), ("device_bytes", C.c_size_t),
+        ("P", C.c_int), ("W", C.c_int), ("persistent", C.c_int),
+    ]
+
+
 # Every symbol the header declares, with (restype, argtypes).
 _vp = C.c_void_p
 _sz = C.c_size_t
@@ -71,6 +85,15 @@ SIGNATURES = {
     "nbhip_bh_keys": (C.c_int, [_vp, C.POINTER(Params), _vp, _vp]),
     "nbhip_bh_nodes": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, C.POINTER(_sz)]),
     "nbhip_bh_interactions": (C.c_int, [_vp, C.POINTER(Params), _vp, C.POINTER(C.c_uint64)]),
+    "nbhip_batch_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "nbhip_batch_destroy": (None, [_vp]),
+    "nbhip_batch_add": (C.c_int, [_vp, _vp, _sz, C.POINTER(Params), C.POINTER(C.c_int)]),
+    "nbhip_batch_remove": (C.c_int, [_vp, C.c_int]),
+    "nbhip_batch_step": (C.c_int, [_vp, C.c_int, C.c_uint]),
+    "nbhip_batch_download": (C.c_int, [_vp, C.c_int, _vp, _sz]),
+    "nbhip_batch_query": (C.c_int, [_vp, C.POINTER(BatchInfo)]),
+    "nbhip_batch_configure": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "nbhip_batch_tune": (C.c_int, [_vp, C.POINTER(C.c_double)]),
     "nbhip_timer_start": (C.c_int, [_vp]),
     "nbhip_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_long)]),
 }
@@ -261,3 +284,60 @@ class Context:
         launches = C.c_long()
         self._check(self.L.nbhip_timer_stop(self.h, C.byref(ms), C.byref(launches)))
         return ms.value, launches.value
+
+
+class Batch:
+    """Many independent direct-sum systems stepped together on one context."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.L = ctx.L
+        h = C.c_void_p()
+        ctx._check(self.L.nbhip_batch_create(ctx.h, C.byref(h)))
+        self.h = h
+        self.sizes: dict[int, int] = {}
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.nbhip_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, bodies: np.ndarray, p: Params) -> int:
+        b = np.ascontiguousarray(bodies, np.float32)
+        if b.ndim != 2 or b.shape[1] != 12:
+            raise ValueError("bodies must have shape (n, 12)")
+        i = C.c_int()
+        self.ctx._check(self.L.nbhip_batch_add(self.h, _ptr(b), b.shape[0], C.byref(p), C.byref(i)))
+        self.sizes[i.value] = b.shape[0]
+        return i.value
+
+    def remove(self, sid: int) -> None:
+        self.ctx._check(self.L.nbhip_batch_remove(self.h, sid))
+        self.sizes.pop(sid, None)
+
+    def step(self, steps: int = 1, flags: int = 0) -> None:
+        self.ctx._check(self.L.nbhip_batch_step(self.h, steps, flags))
+
+    def download(self, sid: int) -> np.ndarray:
+        b = np.zeros((self.sizes[sid], 12), np.float32)
+        self.ctx._check(self.L.nbhip_batch_download(self.h, sid, _ptr(b), b.shape[0]))
+        return b
+
+    def info(self) -> BatchInfo:
+        out = BatchInfo()
+        self.ctx._check(self.L.nbhip_batch_query(self.h, C.byref(out)))
+        return out
+
+    def configure(self, P: int, W: int, persistent: bool = True) -> None:
+        self.ctx._check(self.L.nbhip_batch_configure(self.h, P, W, int(persistent)))
+
+    def tune(self) -> float:
+        ms = C.c_double()
+        self.ctx._check(self.L.nbhip_batch_tune(self.h, C.byref(ms)))
+        return ms.value

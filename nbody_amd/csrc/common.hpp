@@ -65,7 +65,16 @@ struct DirectConfig {
     int P = 4;       // i-bodies per lane
     int W = 4;       // waves per block (all waves share the block's i-set, split j)
     int jsplit = 1;  // blocks along j per i-block (>1 -> partial sums + reduce pass)
-    bool rcp = false;  // |d|^-3 as rsq * rcp instead of rsq^3
+};
+
+// One system's device arrays, as the direct pass sees them.
+struct SysView {
+    const float4* pm;  // current positions + mass
+    float4* pm_next;   // the other ping-pong buffer (written by a fused step)
+    float4* vr;        // velocities + radius (updated in place by a fused step)
+    float4* acc;
+    size_t n;
+    bool any_radius;   // selects the radius-cutoff variant
 };
 
 // One node of the compressed octree, in DFS pre-order (32 bytes, see bh.hip).
@@ -137,12 +146,17 @@ namespace nbhip {
 
 // direct.hip
 void direct_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse_integrate);
+void direct_pass(hipStream_t stream, const SysView& v, const nbhip_params& p, bool fuse,
+                 DevBuf<float>& partial, const DirectConfig& cfg);
 void integrate(nbhip_ctx* c, const nbhip_params& p);
 void diagnostics(nbhip_ctx* c, const nbhip_params& p, double out[9]);
 void force_error_f64(nbhip_ctx* c, const nbhip_params& p, const uint32_t* sample, size_t ns,
                      double* acc64, double* rel_err);
 void ensure_radius_flag(nbhip_ctx* c);
 void aos_to_split(nbhip_ctx* c, size_t n);
+void aos_to_split_raw(hipStream_t s, const float* aos, size_t n, float4* pm, float4* vr, float4* acc);
+void split_to_aos_raw(hipStream_t s, float* aos, size_t n, const float4* pm, const float4* vr,
+                      const float4* acc);
 void split_to_aos(nbhip_ctx* c, size_t n);
 DirectConfig choose_direct_config(size_t n);
 

@@ -1,38 +1,21 @@
-// direct.hip -- direct-summation force pass, kick-drift, diagnostics and layout kernels
-// for gfx950.
+// direct.hip -- direct-summation force pass (single system), kick-drift, diagnostics and
+// layout kernels for gfx950.  The tile itself (force law, LDS staging, compensated
+// summation) is force_tile.hpp, shared with the batched pass.
 //
-// Force law (reference source/detail/physics.h:21-42, GLSL shaders/include/common.glsl:41-56,
-// with a softening extension; eps = 0 is the reference law):
-//     a_i = G * sum_{j : |d|^2 > radius_i^2}  m_j d / (|d|^2 + eps^2)^{3/2},  d = x_j - x_i
-// The self pair has |d| = 0 <= radius_i^2 and drops out by the cutoff, exactly as in the
-// reference (the brute-force loop's j != i skip is only a shortcut there).
-//
-// Kernel shape (k_force):
-//   * bodies are SoA float4 {x, y, z, m} (reference source/gpu.h:32-36), read coalesced;
-//   * a block of W waves owns 64*P i-bodies: lane l holds bodies ibase + p*64 + l,
-//     p < P, in registers, so every j read from LDS feeds P interactions;
-//   * the block stages 64*W j-bodies per tile into LDS with one coalesced float4 load per
-//     thread; wave w consumes its own 64-body slice of the tile, reading each j with a
-//     wave-uniform address (an LDS broadcast), so the W waves split j between them;
-//   * per interaction: 3 sub, 3 FMA (|d|^2 + eps^2), one v_rsq_f32, 3 mul, 3 FMA -- all
-//     FP32 on the vector ALU (no MFMA: a pairwise sum is not a dense contraction);
-//   * each wave sums its 64-body slice in plain FP32, then folds the slice total into a
-//     running (sum, compensation) pair with Knuth's TwoSum -- once per 64 interactions;
-//   * wave partials meet in LDS and are combined in a fixed order; a j-split across
-//     blocks (small N) writes partials that k_reduce combines in a fixed order too, so
-//     results are bit-reproducible and independent of launch configuration timing;
-//   * OUT_STEP fuses the semi-implicit Euler kick-drift + wrap into the epilogue and
-//     writes next positions to the other ping-pong buffer.
+// A launch covers one system: grid.x runs over i-blocks of 64*P bodies, grid.y over
+// j-chunks.  With one j-chunk the epilogue finishes each body in place (acceleration,
+// and for a step the fused kick-drift into the other ping-pong position buffer); with
+// several, each block writes its partial sums and k_reduce combines them in a fixed
+// order before finishing, so results never depend on the launch shape's timing.
 #include <algorithm>
 #include <cstdlib>
 #include <vector>
 
 #include "common.hpp"
+#include "force_tile.hpp"
 
 namespace nbhip {
 namespace {
-
-constexpr int WAVE = 64;
 
 enum : int { OUT_PARTIAL = 0, OUT_ACC = 1, OUT_STEP = 2 };
 
@@ -53,46 +36,9 @@ struct ForceArgs {
     int wrap;
 };
 
-// Knuth's TwoSum: s + c += t with the rounding error of the addition kept in c.
-__device__ __forceinline__ void two_sum(float& s, float& c, float t)
-{
-    const float u = s + t;
-    const float bp = u - s;
-    const float e = (s - (u - bp)) + (t - bp);
-    s = u;
-    c += e;
-}
-
-// 3-torus wrap (reference detail::wrap, source/detail/physics.h:50-57).
-__device__ __forceinline__ float wrap_coord(float x, float size)
-{
-    if (size <= 0.f) return x;
-    const float half = size * .5f;
-    return fmodf(fmodf(x + half, size) + size, size) - half;
-}
-
-// Semi-implicit Euler + wrap (reference detail::integrate_euler, physics.h:60-70), with the
-// reference's operation order and no FMA contraction so it rounds like the CPU build.
-__device__ __forceinline__ void kick_drift(float4& x, float4& v, float ax, float ay, float az,
-                                           float dt, float size, int wrap)
-{
-#pragma clang fp contract(off)
-    v.x = v.x + ax * dt;
-    v.y = v.y + ay * dt;
-    v.z = v.z + az * dt;
-    x.x = x.x + v.x * dt;
-    x.y = x.y + v.y * dt;
-    x.z = x.z + v.z * dt;
-    if (wrap) {
-        x.x = wrap_coord(x.x, size);
-        x.y = wrap_coord(x.y, size);
-        x.z = wrap_coord(x.z, size);
-    }
-}
-
 // Write the final acceleration of body i and, for OUT_STEP, advance it.
-__device__ __forceinline__ void finish_body(const ForceArgs& a, int i, float4 xi, float ax,
-                                            float ay, float az)
+__device__ __forceinline__ void finish_body(const ForceArgs& a, int i, float4 xi, float ax, float ay,
+                                            float az)
 {
     a.acc[i] = make_float4(ax, ay, az, 0.f);
     if (a.out_mode == OUT_STEP) {
@@ -103,113 +49,24 @@ __device__ __forceinline__ void finish_body(const ForceArgs& a, int i, float4 xi
     }
 }
 
-// RCP selects how |d|^-3 is formed: false -> inv * inv * inv with inv = rsq(d2) (three
-// VALU multiplies with the mass); true -> inv * rcp(d2) (the reciprocal goes to the
-// transcendental unit, saving one VALU multiply per interaction).
-template <int P, int W, bool CUT, bool RCP>
+template <int P, int W, bool CUT>
 __global__ __launch_bounds__(WAVE* W) void k_force(ForceArgs a)
 {
-    __shared__ float4 tile[WAVE * W];
-    __shared__ float red[W][P][3][WAVE];
-
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    __shared__ ForceShared<P, W> sh;
     const int n = a.n;
-    const int ibase = blockIdx.x * (WAVE * P);
-
-    float4 xi[P];
-    float r2[P];
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        const int i = ibase + p * WAVE + lane;
-        xi[p] = i < n ? a.pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-        if (CUT) {
-            const float r = i < n ? a.vr[i].w : 0.f;
-            r2[p] = r * r;
-        }
-    }
-
-    float S[P][3], C[P][3];
-#pragma unroll
-    for (int p = 0; p < P; ++p)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) S[p][k] = C[p][k] = 0.f;
-
-    const float eps2 = a.eps2;
     const int j0 = blockIdx.y * a.jchunk;
     const int j1 = min(n, j0 + a.jchunk);
-    for (int t = j0; t < j1; t += WAVE * W) {
-        const int jt = t + threadIdx.x;
-        __syncthreads();  // the previous tile has been consumed
-        // Padding bodies have zero mass at the origin: they add exactly 0 (and the
-        // cutoff masks the 0/0 case when eps == 0).
-        tile[threadIdx.x] = jt < j1 ? a.pm[jt] : make_float4(0.f, 0.f, 0.f, 0.f);
-        __syncthreads();
-
-        float T[P][3];
-#pragma unroll
-        for (int p = 0; p < P; ++p) T[p][0] = T[p][1] = T[p][2] = 0.f;
-
-        const float4* tw = tile + wave * WAVE;
-#pragma unroll 8
-        for (int jj = 0; jj < WAVE; ++jj) {
-            const float4 q = tw[jj];
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                const float dx = q.x - xi[p].x;
-                const float dy = q.y - xi[p].y;
-                const float dz = q.z - xi[p].z;
-                float w;
-                if (CUT) {
-                    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    const float e2 = d2 + eps2;
-                    const float inv = __builtin_amdgcn_rsqf(e2);
-                    const float inv3 = RCP ? inv * __builtin_amdgcn_rcpf(e2) : inv * inv * inv;
-                    w = d2 > r2[p] ? q.w * inv3 : 0.f;
-                } else {
-                    const float e2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-                    const float inv = __builtin_amdgcn_rsqf(e2);
-                    w = RCP ? (q.w * inv) * __builtin_amdgcn_rcpf(e2) : q.w * (inv * inv * inv);
-                }
-                T[p][0] = fmaf(w, dx, T[p][0]);
-                T[p][1] = fmaf(w, dy, T[p][1]);
-                T[p][2] = fmaf(w, dz, T[p][2]);
-            }
-        }
-#pragma unroll
-        for (int p = 0; p < P; ++p)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) two_sum(S[p][k], C[p][k], T[p][k]);
-    }
-
-    // Combine the W wave partials in a fixed order.
-#pragma unroll
-    for (int p = 0; p < P; ++p)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) red[wave][p][k][lane] = S[p][k] + C[p][k];
-    __syncthreads();
-
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        if ((p % W) != wave) continue;
-        const int i = ibase + p * WAVE + lane;
-        if (i >= n) continue;
-        float tot[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            float s = 0.f, c = 0.f;
-#pragma unroll
-            for (int w = 0; w < W; ++w) two_sum(s, c, red[w][p][k][lane]);
-            tot[k] = s + c;
-        }
-        if (a.out_mode == OUT_PARTIAL) {
-            float* dst = a.partial + size_t(blockIdx.y) * 3 * n;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) dst[size_t(k) * n + i] = tot[k];
-        } else {
-            finish_body(a, i, xi[p], a.G * tot[0], a.G * tot[1], a.G * tot[2]);
-        }
-    }
+    force_tile<P, W, CUT>(sh, a.pm, a.vr, n, a.eps2, blockIdx.x * (WAVE * P), j0, j1,
+                          [&](int, int i, float4 xi, float sx, float sy, float sz) {
+                              if (a.out_mode == OUT_PARTIAL) {
+                                  float* dst = a.partial + size_t(blockIdx.y) * 3 * n;
+                                  dst[i] = sx;
+                                  dst[size_t(n) + i] = sy;
+                                  dst[2 * size_t(n) + i] = sz;
+                              } else {
+                                  finish_body(a, i, xi, a.G * sx, a.G * sy, a.G * sz);
+                              }
+                          });
 }
 
 // Combine j-split partials in a fixed order, then finish as OUT_ACC / OUT_STEP.
@@ -402,16 +259,15 @@ __global__ __launch_bounds__(256) void k_split_to_aos(float* aos, int n, const f
 using ForceKernel = void (*)(ForceArgs);
 
 template <int P, int W>
-ForceKernel pick_cut(bool cut, bool rcp)
+ForceKernel pick_cut(bool cut)
 {
-    if (rcp) return cut ? (ForceKernel)k_force<P, W, true, true> : (ForceKernel)k_force<P, W, false, true>;
-    return cut ? (ForceKernel)k_force<P, W, true, false> : (ForceKernel)k_force<P, W, false, false>;
+    return cut ? (ForceKernel)k_force<P, W, true> : (ForceKernel)k_force<P, W, false>;
 }
 
-ForceKernel pick_force(int P, int W, bool cut, bool rcp)
+ForceKernel pick_force(int P, int W, bool cut)
 {
 #define NBHIP_CASE(p, w) \
-    if (P == p && W == w) return pick_cut<p, w>(cut, rcp);
+    if (P == p && W == w) return pick_cut<p, w>(cut);
     NBHIP_CASE(1, 1) NBHIP_CASE(1, 2) NBHIP_CASE(1, 4) NBHIP_CASE(1, 8)
     NBHIP_CASE(2, 1) NBHIP_CASE(2, 2) NBHIP_CASE(2, 4) NBHIP_CASE(2, 8)
     NBHIP_CASE(4, 1) NBHIP_CASE(4, 2) NBHIP_CASE(4, 4) NBHIP_CASE(4, 8)
@@ -433,18 +289,22 @@ unsigned grid1(size_t n) { return unsigned((n + 255) / 256); }
 DirectConfig choose_direct_config(size_t n)
 {
     DirectConfig c;
-    c.P = env_int("NBHIP_DIRECT_P", 4);
-    c.W = env_int("NBHIP_DIRECT_W", 4);
-    c.rcp = env_int("NBHIP_DIRECT_RCP", 0) != 0;
+    // Defaults from a launch-shape sweep at N = 262,144 on MI355X (P in {2,4,8} x W in
+    // {1,2,4,8} x j-split {1,2,4}): two i-bodies per lane keep register use low enough for
+    // high occupancy, eight waves share each j tile, and >= 8192 blocks fill the 256 CUs
+    // evenly.
+    c.P = env_int("NBHIP_DIRECT_P", 2);
+    c.W = env_int("NBHIP_DIRECT_W", 8);
     const size_t per_block = size_t(WAVE) * c.P;
     const size_t iblocks = (n + per_block - 1) / per_block;
     const int forced = env_int("NBHIP_DIRECT_JSPLIT", 0);
     if (forced > 0) {
         c.jsplit = forced;
     } else {
-        // Aim for >= 2048 blocks (8 per CU over 256 CUs) so small systems still fill the
-        // chip; never split below one tile of j per block.
-        const size_t target = 2048;
+        // Aim for >= 8192 blocks (32 per CU over 256 CUs) so the last wave of blocks is a
+        // small tail and small systems still fill the chip; never split below one tile
+        // of j per block.
+        const size_t target = 8192;
         size_t js = iblocks >= target ? 1 : (target + iblocks - 1) / iblocks;
         const size_t tile = size_t(WAVE) * c.W;
         const size_t max_js = std::max<size_t>(1, (n + tile - 1) / tile);
@@ -471,13 +331,12 @@ void ensure_radius_flag(nbhip_ctx* c)
     c->radius_known = true;
 }
 
-void direct_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse)
+void direct_pass(hipStream_t stream, const SysView& v, const nbhip_params& p, bool fuse,
+                 DevBuf<float>& partial, const DirectConfig& cfg)
 {
-    const size_t n = c->count;
-    ensure_radius_flag(c);
-    const bool cut = c->any_radius || p.eps == 0.f;
-    DirectConfig cfg = choose_direct_config(n);
-    ForceKernel k = pick_force(cfg.P, cfg.W, cut, cfg.rcp);
+    const size_t n = v.n;
+    const bool cut = v.any_radius || p.eps == 0.f;
+    ForceKernel k = pick_force(cfg.P, cfg.W, cut);
     if (!k) fail(NBHIP_ERR_INVALID, "no force kernel for P=%d W=%d", cfg.P, cfg.W);
 
     const size_t tile = size_t(WAVE) * cfg.W;
@@ -486,15 +345,15 @@ void direct_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse)
     const int jsplit = int((n + jchunk - 1) / jchunk);
 
     ForceArgs a{};
-    a.pm = c->pm[c->cur].ptr;
-    a.vr = c->vr.ptr;
+    a.pm = v.pm;
+    a.vr = v.vr;
     a.n = int(n);
     a.G = p.G;
     a.eps2 = p.eps * p.eps;
     a.jchunk = int(jchunk);
-    a.acc = c->acc.ptr;
-    a.pm_next = c->pm[1 - c->cur].ptr;
-    a.vr_out = c->vr.ptr;
+    a.acc = v.acc;
+    a.pm_next = v.pm_next;
+    a.vr_out = v.vr;
     a.dt = p.dt;
     a.size = p.size;
     a.wrap = p.wrap;
@@ -504,18 +363,25 @@ void direct_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse)
     const dim3 grid(unsigned((n + per_block - 1) / per_block), unsigned(jsplit));
     if (jsplit == 1) {
         a.out_mode = final_mode;
-        hipLaunchKernelGGL(k, grid, dim3(WAVE * cfg.W), 0, c->stream, a);
+        hipLaunchKernelGGL(k, grid, dim3(WAVE * cfg.W), 0, stream, a);
         NBHIP_CHECK(hipGetLastError());
     } else {
-        c->partial.grow(size_t(jsplit) * 3 * n, 0, c->stream);
-        a.partial = c->partial.ptr;
+        partial.grow(size_t(jsplit) * 3 * n, 0, stream);
+        a.partial = partial.ptr;
         a.out_mode = OUT_PARTIAL;
-        hipLaunchKernelGGL(k, grid, dim3(WAVE * cfg.W), 0, c->stream, a);
+        hipLaunchKernelGGL(k, grid, dim3(WAVE * cfg.W), 0, stream, a);
         NBHIP_CHECK(hipGetLastError());
         a.out_mode = final_mode;
-        hipLaunchKernelGGL(k_reduce, dim3(grid1(n)), dim3(256), 0, c->stream, a, jsplit);
+        hipLaunchKernelGGL(k_reduce, dim3(grid1(n)), dim3(256), 0, stream, a, jsplit);
         NBHIP_CHECK(hipGetLastError());
     }
+}
+
+void direct_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse)
+{
+    ensure_radius_flag(c);
+    SysView v{ c->pm[c->cur].ptr, c->pm[1 - c->cur].ptr, c->vr.ptr, c->acc.ptr, c->count, c->any_radius };
+    direct_pass(c->stream, v, p, fuse, c->partial, choose_direct_config(c->count));
     c->launches += 1;
     if (fuse) c->cur = 1 - c->cur;
 }
@@ -568,6 +434,19 @@ void force_error_f64(nbhip_ctx* c, const nbhip_params& p, const uint32_t* sample
             for (int k = 0; k < 3; ++k) acc64[3 * s + k] = h[4 * s + k];
         if (rel_err) rel_err[s] = h[4 * s + 3];
     }
+}
+
+void aos_to_split_raw(hipStream_t s, const float* aos, size_t n, float4* pm, float4* vr, float4* acc)
+{
+    hipLaunchKernelGGL(k_aos_to_split, dim3(grid1(n)), dim3(256), 0, s, aos, int(n), pm, vr, acc);
+    NBHIP_CHECK(hipGetLastError());
+}
+
+void split_to_aos_raw(hipStream_t s, float* aos, size_t n, const float4* pm, const float4* vr,
+                      const float4* acc)
+{
+    hipLaunchKernelGGL(k_split_to_aos, dim3(grid1(n)), dim3(256), 0, s, aos, int(n), pm, vr, acc);
+    NBHIP_CHECK(hipGetLastError());
 }
 
 void aos_to_split(nbhip_ctx* c, size_t n)
