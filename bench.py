@@ -1,13 +1,20 @@
-"""Benchmark: direct-sum gravity on one MI355X at N = 262,144 (two Plummer spheres).
+"""Benchmark: gravity on one MI355X.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 2] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 2] [--no-cpu] [--only direct|batch|bh]
 
-Prints ONE JSON line.  A step is one fused force + kick-drift pass over all N bodies
-(N^2 pairwise interactions, 20 flops each by the usual convention).  Inputs are resident
-in HBM before the timed region; the region is bracketed by device synchronisation, and
-the force kernel's own duration is measured with HIP events on the library's stream.
-The CPU baseline (oracle restatement of the reference brute-force loop) is timed on a
-bounded subset of rows on this host's cores.
+Prints ONE JSON line.  The headline (value, roofline, cpu_baseline) is the direct sum at
+N = 262,144 (two Plummer spheres): a step is one fused force + kick-drift pass over all N
+bodies (N^2 pairwise interactions, 20 flops each by the usual convention).  Inputs are
+resident in HBM before the timed region; the region is bracketed by device
+synchronisation, and the force kernel's own duration is measured with HIP events on the
+library's stream.  The CPU baseline (oracle restatement of the reference brute-force
+loop) is timed on a bounded subset of rows on this host's cores.
+
+Secondary objects in the same line:
+  "batched"    512 Plummer systems, N log-uniform in [256, 32768], 50 steps replayed from
+               a hipGraph, against the same systems stepped one launch per system;
+  "barnes_hut" N = 2^20 exponential disk + Plummer bulge, theta 0.5, leaf 16, 10 steps,
+               with the reference Barnes-Hut restatement timed on the CPU.
 """
 from __future__ import annotations
 
@@ -150,19 +157,153 @@ def run_direct(args) -> dict:
     return out
 
 
+def run_batch(args) -> dict:
+    """512 independent Plummer systems (N log-uniform in [256, 32768], own seeds), eps 0.01,
+    dt 1e-3: autotuned batched launch, 50 steps captured in a hipGraph and replayed."""
+    from nbody_amd import _lib, ics
+    rng = np.random.default_rng(SEED)
+    ns = np.exp(rng.uniform(np.log(256), np.log(32768), 512)).astype(int)
+    p = _lib.params(dt=1e-3, eps=0.01, size=10000.0, wrap=1)
+    ctx = _lib.Context(0)
+    batch = _lib.Batch(ctx)
+    systems = []
+    for k, n in enumerate(ns):
+        b = ics.plummer(int(n), seed=SEED + k)
+        systems.append(b)
+        batch.add(b, p)
+    tuned_ms = batch.tune()
+    info = batch.info()
+    steps = 50
+    batch.step(steps, _lib.BATCH_GRAPH)          # capture + first replay (untimed)
+    ctx.synchronize()
+    ctx.timer_start()
+    batch.step(steps, _lib.BATCH_GRAPH)
+    ms, _ = ctx.timer_stop()
+    inter = info.interactions_per_step
+    rate = inter * steps / (ms / 1e3)
+    # the unbatched path: the same systems, one launch (+ reduce) per system per step, with
+    # the single-system launch shape (j-split to fill the GPU); and "plain", one launch per
+    # system with no j-split
+    base_steps = 5
+    batch.step(1, _lib.BATCH_PER_SYSTEM)
+    ctx.synchronize()
+    ctx.timer_start()
+    batch.step(base_steps, _lib.BATCH_PER_SYSTEM)
+    ms_base, _ = ctx.timer_stop()
+    rate_base = inter * base_steps / (ms_base / 1e3)
+    os.environ["NBHIP_DIRECT_JSPLIT"] = "1"
+    try:
+        ctx.timer_start()
+        batch.step(base_steps, _lib.BATCH_PER_SYSTEM)
+        ms_plain, _ = ctx.timer_stop()
+    finally:
+        del os.environ["NBHIP_DIRECT_JSPLIT"]
+    rate_plain = inter * base_steps / (ms_plain / 1e3)
+    batch.close()
+    ctx.close()
+    out = {
+        "workload": "512 Plummer systems, N log-uniform in [256, 32768], seeds 2026+k, "
+                    "eps=0.01, dt=1e-3, 50 steps in one hipGraph",
+        "systems": int(info.systems), "bodies": int(info.bodies),
+        "interactions_per_step": inter,
+        "value": rate / 1e9, "unit": "G-interactions/s",
+        "ms_per_step": ms / steps,
+        "pct_fp32_peak": 100.0 * FLOPS_PER_INTERACTION * rate / (FP32_PEAK_TFLOPS * 1e12),
+        "launch": {"P": info.P, "W": info.W, "persistent": bool(info.persistent),
+                   "autotuned_ms_per_step": tuned_ms},
+        "unbatched_value": rate_base / 1e9,
+        "unbatched_ms_per_step": ms_base / base_steps,
+        "speedup_vs_unbatched": rate / rate_base,
+        "unbatched_plain_value": rate_plain / 1e9,
+        "speedup_vs_unbatched_plain": rate / rate_plain,
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = max(1, min(CPU_THREADS_MAX, os.cpu_count() or 1))
+        pick = np.random.default_rng(SEED).choice(len(systems), 64, replace=False)
+        t0 = time.perf_counter()
+        work = 0.0
+        for k in pick:
+            b = systems[k].copy()
+            O.brute_force(b, eps=0.01, threads=threads)
+            work += float(b.shape[0]) ** 2
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": work / t / 1e9, "unit": "G-interactions/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"one full force pass of 64 of the 512 systems (seeded pick), "
+                                         f"{t:.1f} s"}
+    return out
+
+
+def run_bh(args) -> dict:
+    """N = 2^20 exponential disk (R_d 3, z_0 0.3, 80% of the mass) + Plummer bulge (a 0.5),
+    theta 0.5 (reference criterion), leaf 16, eps 0.01, dt 1e-3, 10 steps, root cube 128."""
+    from nbody_amd import _lib, ics
+    n = 1 << 20
+    size = 128.0
+    bodies = ics.disk_bulge(n, seed=SEED)
+    p = _lib.params(mode=_lib.MODE_NLOGN, theta=0.5, leaf_size=16, eps=0.01, dt=1e-3, size=size,
+                    wrap=1)
+    ctx = _lib.Context(0)
+    ctx.upload_bodies(bodies)
+    e0 = ctx.diagnostics(p)
+    _, inter = ctx.bh_interactions(p)
+    ctx.step(p, 1)
+    ctx.synchronize()
+    steps = 10
+    ctx.timer_start()
+    ctx.step(p, steps)
+    ms, _ = ctx.timer_stop()
+    e1 = ctx.diagnostics(p)
+    ctx.close()
+    out = {
+        "workload": "Barnes-Hut N=2^20 disk+bulge, theta=0.5 (reference criterion), leaf 16, "
+                    "eps=0.01, dt=1e-3, 10 steps, tree rebuilt every step",
+        "steps_per_s": steps / (ms / 1e3), "ms_per_step": ms / steps,
+        "interactions_per_step": int(inter),
+        "value": inter * steps / (ms / 1e3) / 1e9, "unit": "G-interactions/s",
+        "energy_drift": float(((e1[0] + e1[1]) - (e0[0] + e0[1])) / abs(e0[0] + e0[1])),
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = max(1, min(CPU_THREADS_MAX, os.cpu_count() or 1))
+        b = bodies.copy()
+        t0 = time.perf_counter()
+        nodes = O.bh_build(b, size)
+        t_build = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, v = O.bh_accelerate(b, size, theta=0.5, eps=0.01, threads=threads, nodes=nodes, visits=True)
+        t_walk = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / (t_build + t_walk), "unit": "steps/s", "cores": threads,
+                               "kind": "port", "build_ms": t_build * 1e3, "walk_ms": t_walk * 1e3,
+                               "interactions_per_step": int(v.sum()),
+                               "sample": "1 force pass of the reference octree restatement "
+                                         "(serial build, leaf 1, threaded walk)"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--only", choices=["direct", "batch", "bh"], default=None,
+                    help="run one workload (the headline fields then describe it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the multi-GPU "
               "sharding of this solver is a design, not a build; see DESIGN.md",
               file=sys.stderr)
         return 2
-    out = run_direct(args)
+    if args.only in (None, "direct"):
+        out = run_direct(args)
+    else:
+        out = {"metric": METRIC, "n_gpus": 1}
+    if args.only in (None, "batch"):
+        out["batched"] = run_batch(args)
+    if args.only in (None, "bh"):
+        out["barnes_hut"] = run_bh(args)
     print(json.dumps(out))
     return 0
 
