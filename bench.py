@@ -89,6 +89,7 @@ def run_direct(args) -> dict:
     p = _lib.params(dt=DT_C2, eps=EPS_C2, size=10000.0, wrap=1, G=1.0)
     ctx = _lib.Context(0)
     ctx.upload_bodies(bodies)
+    _, tuned = ctx.tune(p)          # the autotuner picks the launch shape for this N
     e0 = ctx.diagnostics(p)
     ctx.step(p, args.warmup)
     ctx.synchronize()
@@ -132,6 +133,7 @@ def run_direct(args) -> dict:
             "workload": "direct-sum two-Plummer collision, N=262144, eps=0.005, dt=5e-4, "
                         "fused force+kick-drift step, FP32 with TwoSum-compensated force sum",
             "n_bodies": n, "eps": EPS_C2, "dt": DT_C2, "seed": SEED,
+            "launch": {"P": tuned[0], "W": tuned[1], "jsplit": tuned[2], "autotuned": True},
         },
         "roofline": {
             "bound": "valu",

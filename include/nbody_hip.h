@@ -152,6 +152,26 @@ int nbhip_step(nbhip_ctx* ctx, const nbhip_params* p, int steps);
 /* Block until all work enqueued on the context has finished. */
 int nbhip_synchronize(nbhip_ctx* ctx);
 
+/* Non-blocking: *idle = 1 when all work enqueued on the context has finished, else 0
+ * (hipStreamQuery).  Lets a scheduler keep several contexts busy at once. */
+int nbhip_query(nbhip_ctx* ctx, int* idle);
+
+/* ---- autotuner for the single-system direct sum ---------------------------------------- */
+
+/* One timed launch shape: P bodies per lane, W waves per block, j-split, ms per force pass. */
+typedef struct nbhip_tune_entry {
+    int   P, W, jsplit;
+    float ms;
+} nbhip_tune_entry;
+
+/* Time every compiled (P, W) shape at j-splits 1, 2, 4 and 8 on the current bodies
+ * (accelerations only: positions and velocities do not move), and use the fastest for
+ * this body count from now on.  Up to `cap` timings are written to `out` (may be NULL),
+ * *count receives how many were taken, and *best (may be NULL) the chosen one.
+ * Synchronises. */
+int nbhip_tune(nbhip_ctx* ctx, const nbhip_params* p, nbhip_tune_entry* out, size_t cap,
+               size_t* count, nbhip_tune_entry* best);
+
 /* ---- diagnostics (FP64 reductions on the device) --------------------------------- */
 
 /* out[0] kinetic energy, out[1] potential energy (softened, with the radius cutoff),

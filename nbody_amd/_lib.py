@@ -52,6 +52,11 @@ BATCH_PER_SYSTEM = 2
 BATCH_ACCELERATE = 4
 
 
+class TuneEntry(C.Structure):
+    """nbhip_tune_entry (include/nbody_hip.h)."""
+    _fields_ = [("P", C.c_int), ("W", C.c_int), ("jsplit", C.c_int), ("ms", C.c_float)]
+
+
 class BatchInfo(C.Structure):
     """nbhip_batch_info (include/nbody_hip.h)."""
     _fields_ = [
@@ -80,6 +85,8 @@ SIGNATURES = {
     "nbhip_integrate": (C.c_int, [_vp, C.POINTER(Params)]),
     "nbhip_step": (C.c_int, [_vp, C.POINTER(Params), C.c_int]),
     "nbhip_synchronize": (C.c_int, [_vp]),
+    "nbhip_query": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "nbhip_tune": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, C.POINTER(_sz), C.POINTER(TuneEntry)]),
     "nbhip_diagnostics": (C.c_int, [_vp, C.POINTER(Params), _vp]),
     "nbhip_force_error_f64": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, _vp, _vp]),
     "nbhip_bh_keys": (C.c_int, [_vp, C.POINTER(Params), _vp, _vp]),
@@ -239,6 +246,22 @@ class Context:
 
     def synchronize(self) -> None:
         self._check(self.L.nbhip_synchronize(self.h))
+
+    def idle(self) -> bool:
+        flag = C.c_int()
+        self._check(self.L.nbhip_query(self.h, C.byref(flag)))
+        return bool(flag.value)
+
+    # ---- autotuner ----
+    def tune(self, p: Params) -> tuple[list[tuple[int, int, int, float]], tuple[int, int, int, float]]:
+        """Time the launch shapes at the current N; returns (all timings, chosen)."""
+        cap = 256
+        arr = (TuneEntry * cap)()
+        cnt = C.c_size_t()
+        best = TuneEntry()
+        self._check(self.L.nbhip_tune(self.h, C.byref(p), arr, cap, C.byref(cnt), C.byref(best)))
+        rows = [(e.P, e.W, e.jsplit, e.ms) for e in arr[:cnt.value]]
+        return rows, (best.P, best.W, best.jsplit, best.ms)
 
     # ---- diagnostics ----
     def diagnostics(self, p: Params) -> np.ndarray:

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "nbody_hip.h"
 
@@ -107,6 +108,10 @@ struct nbhip_ctx {
     nbhip::DevBuf<float4> vr;
     nbhip::DevBuf<float4> acc;
 
+    // launch shape chosen by nbhip_tune for one body count (tuned_n == 0: none)
+    size_t tuned_n = 0;
+    nbhip::DirectConfig tuned;
+
     // radius bookkeeping: does any body have radius > 0?  (selects the cutoff kernel)
     bool radius_known = false;
     bool any_radius = false;
@@ -159,6 +164,8 @@ void split_to_aos_raw(hipStream_t s, float* aos, size_t n, const float4* pm, con
                       const float4* acc);
 void split_to_aos(nbhip_ctx* c, size_t n);
 DirectConfig choose_direct_config(size_t n);
+void tune_direct(nbhip_ctx* c, const nbhip_params& p, std::vector<nbhip_tune_entry>& all,
+                 nbhip_tune_entry& best);
 
 // bh.hip
 void bh_build(nbhip_ctx* c, const nbhip_params& p);

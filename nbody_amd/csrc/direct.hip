@@ -377,13 +377,58 @@ void direct_pass(hipStream_t stream, const SysView& v, const nbhip_params& p, bo
     }
 }
 
+// Launch shape for a context: an explicit override in the environment wins, then the
+// autotuner's choice for this body count, then the default heuristic.
+DirectConfig context_config(const nbhip_ctx* c)
+{
+    const bool forced = std::getenv("NBHIP_DIRECT_P") || std::getenv("NBHIP_DIRECT_W") ||
+                        std::getenv("NBHIP_DIRECT_JSPLIT");
+    if (!forced && c->tuned_n == c->count && c->tuned_n != 0) return c->tuned;
+    return choose_direct_config(c->count);
+}
+
 void direct_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse)
 {
     ensure_radius_flag(c);
     SysView v{ c->pm[c->cur].ptr, c->pm[1 - c->cur].ptr, c->vr.ptr, c->acc.ptr, c->count, c->any_radius };
-    direct_pass(c->stream, v, p, fuse, c->partial, choose_direct_config(c->count));
+    direct_pass(c->stream, v, p, fuse, c->partial, context_config(c));
     c->launches += 1;
     if (fuse) c->cur = 1 - c->cur;
+}
+
+void tune_direct(nbhip_ctx* c, const nbhip_params& p, std::vector<nbhip_tune_entry>& all,
+                 nbhip_tune_entry& best)
+{
+    static const int shapes[][2] = { { 1, 4 }, { 1, 8 }, { 2, 2 }, { 2, 4 }, { 2, 8 }, { 4, 2 },
+                                     { 4, 4 }, { 4, 8 }, { 8, 2 }, { 8, 4 } };
+    ensure_radius_flag(c);
+    const SysView v{ c->pm[c->cur].ptr, c->pm[1 - c->cur].ptr, c->vr.ptr, c->acc.ptr, c->count,
+                     c->any_radius };
+    const size_t n = c->count;
+    best = nbhip_tune_entry{ 0, 0, 0, 1e30f };
+    for (const auto& sh : shapes)
+        for (int js : { 1, 2, 4, 8 }) {
+            DirectConfig cfg;
+            cfg.P = sh[0];
+            cfg.W = sh[1];
+            const size_t tile = size_t(WAVE) * cfg.W;
+            if (js > 1 && size_t(js) * tile > n) continue;  // a j-chunk must hold a tile
+            cfg.jsplit = js;
+            direct_pass(c->stream, v, p, false, c->partial, cfg);  // warm
+            NBHIP_CHECK(hipEventRecord(c->ev_start, c->stream));
+            for (int r = 0; r < 2; ++r) direct_pass(c->stream, v, p, false, c->partial, cfg);
+            NBHIP_CHECK(hipEventRecord(c->ev_stop, c->stream));
+            NBHIP_CHECK(hipEventSynchronize(c->ev_stop));
+            float ms = 0.f;
+            NBHIP_CHECK(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+            const nbhip_tune_entry e{ cfg.P, cfg.W, js, ms / 2 };
+            all.push_back(e);
+            if (e.ms < best.ms) best = e;
+        }
+    c->tuned_n = n;
+    c->tuned.P = best.P;
+    c->tuned.W = best.W;
+    c->tuned.jsplit = best.jsplit;
 }
 
 void integrate(nbhip_ctx* c, const nbhip_params& p)

@@ -187,6 +187,7 @@ int nbhip_reserve(nbhip_ctx* c, size_t n)
         c->acc.grow(n, keep, c->stream);
         c->count = n;
         c->radius_known = false;
+        c->tuned_n = 0;
     });
 }
 
@@ -326,6 +327,41 @@ int nbhip_synchronize(nbhip_ctx* c)
         check_ctx(c);
         NBHIP_CHECK(hipSetDevice(c->device));
         NBHIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int nbhip_tune(nbhip_ctx* c, const nbhip_params* p, nbhip_tune_entry* out, size_t cap,
+               size_t* count, nbhip_tune_entry* best)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        check_params(p);
+        if (count) *count = 0;
+        if (c->count == 0) return;
+        NBHIP_CHECK(hipSetDevice(c->device));
+        std::vector<nbhip_tune_entry> all;
+        nbhip_tune_entry b{};
+        nbhip::tune_direct(c, *p, all, b);
+        if (count) *count = all.size();
+        if (out)
+            for (size_t k = 0; k < all.size() && k < cap; ++k) out[k] = all[k];
+        if (best) *best = b;
+    });
+}
+
+int nbhip_query(nbhip_ctx* c, int* idle)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        if (!idle) fail(NBHIP_ERR_INVALID, "null output");
+        NBHIP_CHECK(hipSetDevice(c->device));
+        const hipError_t e = hipStreamQuery(c->stream);
+        if (e == hipSuccess)
+            *idle = 1;
+        else if (e == hipErrorNotReady)
+            *idle = 0;
+        else
+            fail(NBHIP_ERR_HIP, "hipStreamQuery failed: %s", hipGetErrorString(e));
     });
 }
 

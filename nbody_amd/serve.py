@@ -1,0 +1,268 @@
+"""Continuous-batching service for many simulation jobs on one MI355X.
+
+Jobs arrive over time (each: bodies, a step count, a mode).  The server keeps every
+admitted job's state on the device until its last step, and runs:
+  * direct-sum jobs (N below `direct_below`) together in one nbhip_batch -- one launch per
+    step for all of them, with jobs joining and leaving between chunks of steps;
+  * Barnes-Hut jobs each on their own context from a fixed pool, so their streams run
+    concurrently with the batch and with each other.
+Admission is first come, first served, against a device-memory budget (bytes per body
+estimated per mode) and the free contexts of the pool; a job that does not fit waits.
+Each resource advances in chunks of at most `chunk` steps, never past the step count of
+any job it holds, and is polled without blocking (nbhip_query), so a resource that
+finishes early is refilled while the others still run.
+
+A job's result does not depend on what else is running: a batched system sums each
+body's force in a fixed order whatever shares the batch, and a Barnes-Hut job owns its
+context.  Latency is measured from a job's arrival to its result being back on the host.
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+from collections import deque
+
+import numpy as np
+
+from . import _lib
+
+# Device bytes per body, for admission: state arrays (pm x2, vr, acc: 64 B) for a
+# batched job; for a Barnes-Hut job also keys, sorted copies, prefix sums and up to 2N
+# nodes (about 170 B more).
+BYTES_PER_BODY = {_lib.MODE_N2: 64, _lib.MODE_NLOGN: 240}
+
+
+@dataclasses.dataclass
+class Job:
+    bodies: np.ndarray              # reference Body records, float32 (n, 12)
+    steps: int
+    params: _lib.Params
+    arrival: float = 0.0            # seconds after the start of the run
+    tag: str = ""
+    # filled in by the server
+    id: int = -1
+    admitted: float = float("nan")
+    done: float = float("nan")
+    result: np.ndarray | None = None
+    interactions_per_step: float = 0.0
+
+    @property
+    def n(self) -> int:
+        return self.bodies.shape[0]
+
+    @property
+    def mode(self) -> int:
+        return self.params.mode
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+    def device_bytes(self) -> int:
+        return BYTES_PER_BODY[self.mode] * self.n
+
+
+@dataclasses.dataclass
+class Report:
+    jobs: int
+    wall_s: float
+    job_steps: int
+    body_steps: float
+    interactions: float
+    latencies: np.ndarray
+    peak_device_bytes: int
+    max_concurrent: int
+
+    def summary(self) -> dict:
+        lat = self.latencies
+        return {
+            "jobs": self.jobs, "wall_s": self.wall_s,
+            "jobs_per_s": self.jobs / self.wall_s,
+            "steps_per_s": self.job_steps / self.wall_s,
+            "body_steps_per_s": self.body_steps / self.wall_s,
+            "interactions_per_s": self.interactions / self.wall_s,
+            "latency_p50_s": float(np.percentile(lat, 50)) if lat.size else None,
+            "latency_p99_s": float(np.percentile(lat, 99)) if lat.size else None,
+            "latency_max_s": float(lat.max()) if lat.size else None,
+            "peak_device_bytes": self.peak_device_bytes,
+            "max_concurrent_jobs": self.max_concurrent,
+        }
+
+
+class _BhSlot:
+    def __init__(self):
+        self.ctx = _lib.Context(0)
+        self.job: Job | None = None
+        self.remaining = 0
+        self.inflight = 0
+
+
+class Server:
+    def __init__(self, hbm_budget: float = 200e9, bh_contexts: int = 8, chunk: int = 16,
+                 direct_below: int = 16384, keep_results: bool = False, batch_shape=None):
+        self.budget = hbm_budget
+        self.chunk = chunk
+        self.direct_below = direct_below
+        self.keep_results = keep_results
+        self.batch_ctx = _lib.Context(0)
+        self.batch = _lib.Batch(self.batch_ctx)
+        if batch_shape is not None:
+            self.batch.configure(*batch_shape)
+        self.slots = [_BhSlot() for _ in range(bh_contexts)]
+
+    def close(self) -> None:
+        self.batch.close()
+        self.batch_ctx.close()
+        for s in self.slots:
+            s.ctx.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- the loop ------------------------------------------------------------------------
+    def run(self, jobs: list[Job], clock=time.perf_counter) -> Report:
+        for k, j in enumerate(jobs):
+            j.id = k
+            if j.mode == _lib.MODE_N2 and j.n >= self.direct_below:
+                raise ValueError(f"job {k}: direct sum is for N < {self.direct_below}")
+            if j.device_bytes() > self.budget:
+                raise ValueError(f"job {k} needs {j.device_bytes()} bytes, more than the budget")
+        pending = deque(sorted(jobs, key=lambda j: j.arrival))
+        waiting: deque[Job] = deque()
+        direct: dict[int, tuple[Job, int]] = {}   # batch system id -> (job, steps remaining)
+        batch_inflight = 0
+        in_use = 0
+        peak = 0
+        max_conc = 0
+        job_steps = 0
+        body_steps = 0.0
+        interactions = 0.0
+        finished: list[Job] = []
+        t0 = clock()
+
+        def now() -> float:
+            return clock() - t0
+
+        def finish(job: Job, result: np.ndarray) -> None:
+            nonlocal in_use
+            job.done = now()
+            if self.keep_results:
+                job.result = result
+            in_use -= job.device_bytes()
+            finished.append(job)
+
+        while len(finished) < len(jobs):
+            t = now()
+            while pending and pending[0].arrival <= t:
+                waiting.append(pending.popleft())
+
+            # -- the batch: account a finished chunk, retire jobs, admit, launch --------
+            if batch_inflight and self.batch_ctx.idle():
+                for sid in list(direct):
+                    job, rem = direct[sid]
+                    rem -= batch_inflight
+                    job_steps += batch_inflight
+                    body_steps += batch_inflight * job.n
+                    interactions += batch_inflight * float(job.n) ** 2
+                    if rem <= 0:
+                        res = self.batch.download(sid)
+                        self.batch.remove(sid)
+                        del direct[sid]
+                        finish(job, res)
+                    else:
+                        direct[sid] = (job, rem)
+                batch_inflight = 0
+
+            # -- admission, first come first served ---------------------------------------
+            while waiting:
+                job = waiting[0]
+                if in_use + job.device_bytes() > self.budget:
+                    break
+                if job.mode == _lib.MODE_N2:
+                    if batch_inflight:
+                        break        # joins at the next chunk boundary
+                    sid = self.batch.add(job.bodies, job.params)
+                    direct[sid] = (job, job.steps)
+                    job.interactions_per_step = float(job.n) ** 2
+                else:
+                    slot = next((s for s in self.slots if s.job is None), None)
+                    if slot is None:
+                        break
+                    slot.ctx.upload_bodies(job.bodies)
+                    _, inter = slot.ctx.bh_interactions(job.params)
+                    job.interactions_per_step = float(inter)
+                    slot.job, slot.remaining, slot.inflight = job, job.steps, 0
+                waiting.popleft()
+                job.admitted = now()
+                in_use += job.device_bytes()
+                peak = max(peak, in_use)
+
+            if direct and not batch_inflight:
+                k = min(self.chunk, min(rem for _, rem in direct.values()))
+                self.batch.step(k)
+                batch_inflight = k
+
+            # -- Barnes-Hut slots ---------------------------------------------------------
+            for s in self.slots:
+                if s.job is None:
+                    continue
+                if s.inflight and s.ctx.idle():
+                    s.remaining -= s.inflight
+                    job_steps += s.inflight
+                    body_steps += s.inflight * s.job.n
+                    interactions += s.inflight * s.job.interactions_per_step
+                    s.inflight = 0
+                    if s.remaining <= 0:
+                        res = s.ctx.download_bodies()
+                        job, s.job = s.job, None
+                        finish(job, res)
+                        continue
+                if not s.inflight:
+                    k = min(self.chunk, s.remaining)
+                    s.ctx.step(s.job.params, k)
+                    s.inflight = k
+
+            max_conc = max(max_conc, len(direct) + sum(1 for s in self.slots if s.job is not None))
+            if not waiting and pending and not direct and all(s.job is None for s in self.slots):
+                # nothing on the device: wait for the next arrival
+                time.sleep(max(0.0, min(pending[0].arrival - now(), 0.01)))
+            else:
+                time.sleep(20e-6)
+
+        wall = now()
+        lat = np.array([j.latency for j in finished])
+        return Report(len(finished), wall, job_steps, body_steps, interactions, lat, peak, max_conc)
+
+
+def make_jobs(count: int, seed: int, n_range=(1024, 262144), steps_range=(10, 1000),
+              direct_below: int = 16384, eps: float = 0.01, dt: float = 1e-3,
+              theta: float = 0.5, leaf_size: int = 16, size: float = 128.0) -> list[Job]:
+    """The served workload: Plummer or disk+bulge systems, N log-uniform, steps uniform;
+    direct sum below `direct_below` bodies, Barnes-Hut above.  Arrival times are zero;
+    set them with poisson_arrivals()."""
+    from . import ics
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+        steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
+        disk = bool(rng.integers(0, 2))
+        bodies = ics.disk_bulge(n, seed=seed * 7919 + k) if disk else ics.plummer(n, seed=seed * 7919 + k)
+        if n < direct_below:
+            p = _lib.params(dt=dt, eps=eps, size=size, wrap=1)
+        else:
+            p = _lib.params(mode=_lib.MODE_NLOGN, dt=dt, eps=eps, size=size, wrap=1, theta=theta,
+                            leaf_size=leaf_size)
+        out.append(Job(bodies=bodies, steps=steps, params=p, tag="disk" if disk else "plummer"))
+    return out
+
+
+def poisson_arrivals(jobs: list[Job], rate: float, seed: int) -> None:
+    """Arrival times of a Poisson stream of `rate` jobs per second, in list order."""
+    gaps = np.random.default_rng(seed).exponential(1.0 / rate, len(jobs))
+    t = np.cumsum(gaps) - gaps[0]
+    for j, a in zip(jobs, t):
+        j.arrival = float(a)

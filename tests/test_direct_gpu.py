@@ -203,3 +203,15 @@ def test_full_size_c2_properties(ctx):
     acc = ctx.download(_lib.RB_ACCELERATIONS)[2][:, :3].astype(np.float64)
     ma = acc * b[:, 7:8]
     assert np.abs(ma.sum(0)).max() < 1e-5 * np.abs(ma).sum(0).max()
+
+
+def test_autotuner_picks_a_shape_and_keeps_results(ctx, orc):
+    b = ics.plummer(20000, seed=21)
+    ctx.upload_bodies(b)
+    p = _lib.params(eps=0.01)
+    rows, best = ctx.tune(p)
+    assert len(rows) >= 10 and best[3] == min(r[3] for r in rows)
+    ctx.accelerate(p)                      # now with the tuned shape
+    a = ctx.download(_lib.RB_ACCELERATIONS)[2][:, :3]
+    s = np.arange(0, 20000, 101, dtype=np.uint32)
+    assert rel_err(a[s], orc.direct_f64(b, s, eps=0.01)).max() <= 1e-5

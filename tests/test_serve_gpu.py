@@ -1,0 +1,62 @@
+"""The continuous-batching server (nbody_amd/serve.py) on the device.
+
+Bars: every job completes; each job's result is bit-identical to the same job run alone
+(direct-sum jobs in a batch of one, Barnes-Hut jobs on a context of their own); the
+device-memory budget is never exceeded and a small budget still drains the queue.
+"""
+import numpy as np
+import pytest
+
+from nbody_amd import _lib, serve
+
+pytestmark = pytest.mark.gpu
+
+
+def alone(job):
+    with _lib.Context(0) as ctx:
+        if job.mode == _lib.MODE_N2:
+            with _lib.Batch(ctx) as b:
+                sid = b.add(job.bodies, job.params)
+                b.step(job.steps)
+                return b.download(sid)
+        ctx.upload_bodies(job.bodies)
+        ctx.step(job.params, job.steps)
+        return ctx.download_bodies()
+
+
+def small_jobs(count=14, seed=5):
+    jobs = serve.make_jobs(count, seed=seed, n_range=(1024, 60000), steps_range=(3, 40),
+                           direct_below=16384)
+    serve.poisson_arrivals(jobs, rate=200.0, seed=seed)
+    return jobs
+
+
+def test_every_job_same_alone_and_in_the_mix():
+    jobs = small_jobs()
+    assert any(j.mode == _lib.MODE_N2 for j in jobs) and any(j.mode == _lib.MODE_NLOGN for j in jobs)
+    with serve.Server(chunk=8, bh_contexts=3, keep_results=True) as srv:
+        rep = srv.run(jobs)
+    s = rep.summary()
+    print(s)
+    assert rep.jobs == len(jobs) and s["latency_p99_s"] is not None
+    assert rep.job_steps == sum(j.steps for j in jobs)
+    for j in jobs:
+        ref = alone(j)
+        assert np.array_equal(j.result.view(np.uint32), ref.view(np.uint32)), (j.id, j.n, j.mode)
+
+
+def test_budget_is_respected_and_queue_drains():
+    jobs = small_jobs(count=10, seed=9)
+    budget = max(j.device_bytes() for j in jobs) * 1.2
+    with serve.Server(hbm_budget=budget, chunk=8, bh_contexts=2) as srv:
+        rep = srv.run(jobs)
+    assert rep.jobs == len(jobs)
+    assert rep.peak_device_bytes <= budget
+    assert rep.max_concurrent <= 3
+
+
+def test_rejects_impossible_jobs():
+    jobs = small_jobs(count=2, seed=1)
+    with serve.Server(hbm_budget=1000, chunk=8, bh_contexts=1) as srv:
+        with pytest.raises(ValueError):
+            srv.run(jobs)
