@@ -14,7 +14,9 @@ Secondary objects in the same line:
   "batched"    512 Plummer systems, N log-uniform in [256, 32768], 50 steps replayed from
                a hipGraph, against the same systems stepped one launch per system;
   "barnes_hut" N = 2^20 exponential disk + Plummer bulge, theta 0.5, leaf 16, 10 steps,
-               with the reference Barnes-Hut restatement timed on the CPU.
+               with the reference Barnes-Hut restatement timed on the CPU;
+  "served"     a Poisson stream of jobs through the continuous-batching server: served
+               steps/s and p50/p99 latency.
 """
 from __future__ import annotations
 
@@ -284,13 +286,47 @@ def run_bh(args) -> dict:
     return out
 
 
+def run_served(args) -> dict:
+    """Poisson stream of simulation jobs (Plummer or disk+bulge, N log-uniform in [1k, 256k],
+    steps uniform in [10, 1000]; direct sum below 16k bodies, Barnes-Hut theta 0.5 above),
+    device-resident, admitted against a 200 GB budget, continuously batched.  Capacity is
+    measured first with every job queued at t=0; the open stream then runs at 0.7x it."""
+    from nbody_amd import serve
+    count = 160
+    jobs = serve.make_jobs(count, seed=SEED)
+    with serve.Server(hbm_budget=200e9) as srv:
+        sat = srv.run(jobs)
+        capacity = sat.jobs / sat.wall_s
+        rate = 0.7 * capacity
+        serve.poisson_arrivals(jobs, rate=rate, seed=SEED)
+        rep = srv.run(jobs)
+    s = rep.summary()
+    n_direct = sum(1 for j in jobs if j.mode == 0)
+    return {
+        "workload": f"{count} jobs: Plummer or disk+bulge, N log-uniform in [1024, 262144], steps "
+                    "uniform in [10, 1000]; direct sum below 16384 bodies (batched), Barnes-Hut "
+                    "theta 0.5 leaf 16 above (a context each, 8 concurrent); 200 GB budget; "
+                    f"Poisson arrivals at 0.7x the measured capacity ({rate:.1f} jobs/s)",
+        "jobs": s["jobs"], "direct_jobs": n_direct, "barnes_hut_jobs": count - n_direct,
+        "steps_per_s": s["steps_per_s"], "unit": "job-steps/s",
+        "jobs_per_s": s["jobs_per_s"],
+        "body_steps_per_s": s["body_steps_per_s"],
+        "interactions_per_s": s["interactions_per_s"],
+        "latency_p50_s": s["latency_p50_s"], "latency_p99_s": s["latency_p99_s"],
+        "capacity_jobs_per_s": capacity, "arrival_rate_jobs_per_s": rate,
+        "saturated_steps_per_s": sat.job_steps / sat.wall_s,
+        "max_concurrent_jobs": s["max_concurrent_jobs"],
+        "peak_device_bytes": s["peak_device_bytes"],
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--only", choices=["direct", "batch", "bh"], default=None,
+    ap.add_argument("--only", choices=["direct", "batch", "bh", "served"], default=None,
                     help="run one workload (the headline fields then describe it)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -306,6 +342,8 @@ def main() -> int:
         out["batched"] = run_batch(args)
     if args.only in (None, "bh"):
         out["barnes_hut"] = run_bh(args)
+    if args.only in (None, "served"):
+        out["served"] = run_served(args)
     print(json.dumps(out))
     return 0
 

@@ -47,12 +47,14 @@ def test_every_job_same_alone_and_in_the_mix():
 
 def test_budget_is_respected_and_queue_drains():
     jobs = small_jobs(count=10, seed=9)
+    with serve.Server(chunk=8, bh_contexts=2) as srv:
+        free = srv.run(small_jobs(count=10, seed=9))
     budget = max(j.device_bytes() for j in jobs) * 1.2
     with serve.Server(hbm_budget=budget, chunk=8, bh_contexts=2) as srv:
         rep = srv.run(jobs)
-    assert rep.jobs == len(jobs)
-    assert rep.peak_device_bytes <= budget
-    assert rep.max_concurrent <= 3
+    assert rep.jobs == len(jobs) and free.jobs == len(jobs)
+    assert rep.peak_device_bytes <= budget < free.peak_device_bytes
+    assert rep.max_concurrent <= free.max_concurrent
 
 
 def test_rejects_impossible_jobs():
