@@ -93,12 +93,14 @@ def run_direct(args) -> dict:
     ctx.upload_bodies(bodies)
     _, tuned = ctx.tune(p)          # the autotuner picks the launch shape for this N
     e0 = ctx.diagnostics(p)
-    ctx.step(p, args.warmup)
+    for _ in range(args.warmup):
+        ctx.step(p, 1)
     ctx.synchronize()
 
     t0 = time.perf_counter()
     ctx.timer_start()
-    ctx.step(p, args.steps)
+    for _ in range(args.steps):
+        ctx.step(p, 1)
     ms_events, launches = ctx.timer_stop()
     ctx.synchronize()
     wall = time.perf_counter() - t0
@@ -252,18 +254,24 @@ def run_bh(args) -> dict:
     ctx.upload_bodies(bodies)
     e0 = ctx.diagnostics(p)
     _, inter = ctx.bh_interactions(p)
+    steps = 10
     ctx.step(p, 1)
     ctx.synchronize()
-    steps = 10
     ctx.timer_start()
-    ctx.step(p, steps)
+    ctx.step(p, steps)                   # eager: ~20 launches per step
     ms, _ = ctx.timer_stop()
+    ctx.step_graph(p, steps)             # capture (untimed) ...
+    ctx.synchronize()
+    ctx.timer_start()
+    ctx.step_graph(p, steps)             # ... and replay
+    ms_graph, _ = ctx.timer_stop()
     e1 = ctx.diagnostics(p)
     ctx.close()
     out = {
         "workload": "Barnes-Hut N=2^20 disk+bulge, theta=0.5 (reference criterion), leaf 16, "
                     "eps=0.01, dt=1e-3, 10 steps, tree rebuilt every step",
         "steps_per_s": steps / (ms / 1e3), "ms_per_step": ms / steps,
+        "ms_per_step_graph": ms_graph / steps,
         "interactions_per_step": int(inter),
         "value": inter * steps / (ms / 1e3) / 1e9, "unit": "G-interactions/s",
         "energy_drift": float(((e1[0] + e1[1]) - (e0[0] + e0[1])) / abs(e0[0] + e0[1])),

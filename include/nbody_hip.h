@@ -149,6 +149,12 @@ int nbhip_integrate(nbhip_ctx* ctx, const nbhip_params* p);
  * mode the kick-drift is fused into the force pass (ping-pong position buffers). */
 int nbhip_step(nbhip_ctx* ctx, const nbhip_params* p, int steps);
 
+/* The same `steps` steps, captured once into a hipGraph and replayed on later calls with
+ * the same mode, body count, parameters and buffers (results are identical to
+ * nbhip_step).  The first call pays the capture; worth it where launches dominate
+ * (small systems, Barnes-Hut's ~20 kernels per step). */
+int nbhip_step_graph(nbhip_ctx* ctx, const nbhip_params* p, int steps);
+
 /* Block until all work enqueued on the context has finished. */
 int nbhip_synchronize(nbhip_ctx* ctx);
 

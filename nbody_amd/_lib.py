@@ -84,6 +84,7 @@ SIGNATURES = {
     "nbhip_accelerate": (C.c_int, [_vp, C.POINTER(Params)]),
     "nbhip_integrate": (C.c_int, [_vp, C.POINTER(Params)]),
     "nbhip_step": (C.c_int, [_vp, C.POINTER(Params), C.c_int]),
+    "nbhip_step_graph": (C.c_int, [_vp, C.POINTER(Params), C.c_int]),
     "nbhip_synchronize": (C.c_int, [_vp]),
     "nbhip_query": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "nbhip_tune": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, C.POINTER(_sz), C.POINTER(TuneEntry)]),
@@ -243,6 +244,10 @@ class Context:
 
     def step(self, p: Params, steps: int = 1) -> None:
         self._check(self.L.nbhip_step(self.h, C.byref(p), steps))
+
+    def step_graph(self, p: Params, steps: int) -> None:
+        """`steps` steps replayed from a captured hipGraph (captured on first use)."""
+        self._check(self.L.nbhip_step_graph(self.h, C.byref(p), steps))
 
     def synchronize(self) -> None:
         self._check(self.L.nbhip_synchronize(self.h))

@@ -151,9 +151,42 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_apply(const T* in, int n, con
     if (base <= n - 1 && n - 1 < base + SC_IPT) out[n] = run;  // the thread holding the last item
 }
 
+// The whole scan in one block, for short arrays where launches cost more than bytes:
+// tiles of 256 x SC_IPT items in order, with a running carry.  `out` may alias `in`.
+template <typename T>
+__global__ __launch_bounds__(SC_BLOCK) void k_scan_single(const T* in, int n, T* out)
+{
+    __shared__ T sh[SC_BLOCK];
+    T carry{};
+    for (int tile = 0; tile < n; tile += SC_TILE) {
+        const int base = tile + threadIdx.x * SC_IPT;
+        T v[SC_IPT];
+        T sum{};
+        for (int k = 0; k < SC_IPT; ++k) {
+            v[k] = base + k < n ? in[base + k] : T{};
+            sum = sum + v[k];
+        }
+        T total;
+        T run = carry + block_exclusive_scan(sum, sh, total);
+        for (int k = 0; k < SC_IPT; ++k) {
+            if (base + k < n) out[base + k] = run;
+            run = run + v[k];
+        }
+        carry = carry + total;
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+constexpr int SC_SINGLE_MAX = 4 * SC_TILE;  // beyond this the three-phase scan is faster
+
 template <typename T>
 void exclusive_scan(nbhip_ctx* c, const T* in, int n, T* out, DevBuf<T>& part)
 {
+    if (n <= SC_SINGLE_MAX) {
+        hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(SC_BLOCK), 0, c->stream, in, n, out);
+        NBHIP_CHECK(hipGetLastError());
+        return;
+    }
     const int nb = std::max(1, (n + SC_TILE - 1) / SC_TILE);
     part.grow(nb, 0, c->stream);
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(SC_BLOCK), 0, c->stream, in, n, part.ptr);
@@ -295,6 +328,23 @@ struct KeyView {
         }
         return lo;
     }
+    // largest b >= lo with cpl(a, b) >= m, given cpl(a, lo) >= m
+    __device__ int last_from(int a, int lo, int m) const
+    {
+        int step = 1;
+        for (;;) {
+            const int t = lo + step;
+            if (t >= n || cpl(a, t) < m) break;
+            lo = t;
+            step <<= 1;
+        }
+        while (step > 1) {
+            step >>= 1;
+            const int t = lo + step;
+            if (t < n && cpl(a, t) >= m) lo = t;
+        }
+        return lo;
+    }
     // smallest a' <= a with cpl(a', a) >= m
     __device__ int first(int a, int m) const
     {
@@ -314,72 +364,67 @@ struct KeyView {
     }
 };
 
-// The entries position a contributes to the pre-order, shallow to deep: internal nodes
-// and at most one bucket starting at a, then (if it is not inside a bucket) leaf a.
-struct Walk {
-    int cnt;          // internal + bucket entries
-    bool leaf;        // leaf a follows them
-    int lvl[LEVELS + 1];
-    int end[LEVELS + 1];
-    bool bucket[LEVELS + 1];
-};
-
-__device__ void walk_position(const KeyView& kv, int a, int leaf_size, Walk& w)
+// Walks the nodes that start at sorted position a, deepest first, and calls
+// visit(level, b, bucket) for each node the tree keeps: internal nodes, and at most one
+// bucket (the shallowest cell at a holding at most leaf_size bodies, or a 21-level cell,
+// provided its parent is not itself such a cell).  Returns whether leaf a (the body at
+// position a on its own) is kept, i.e. is not inside a bucket.  Each node's end is found
+// by galloping from the previous node's end: cpl(a, b + 1) = d(b) is the next level.
+template <typename Visit>
+__device__ bool walk_position(const KeyView& kv, int a, int leaf_size, Visit&& visit)
 {
-    w.cnt = 0;
-    w.leaf = false;
-    if (kv.n == 1) {
-        w.leaf = true;
-        return;
-    }
+    if (kv.n == 1) return true;
     const int dl = kv.d(a - 1);
-    int tl[LEVELS + 1], te[LEVELS + 1];
-    int t = 0;
-    for (int m = kv.d(a); m > dl && t <= LEVELS;) {
-        const int b = kv.last(a, m);
-        tl[t] = m;
-        te[t] = b;
-        ++t;
+    int m = kv.d(a);
+    if (m <= dl) {
+        // no node starts at a: the deepest node holding a starts before it, at level d(a-1)
+        const int size = kv.last(a, dl) - kv.first(a, dl) + 1;
+        return size > leaf_size && dl < LEVELS;
+    }
+    int lo = a;  // cpl(a, lo) >= m
+    bool deepest_is_bucket = false, first = true;
+    bool held = false;  // a bucket seen but not yet emitted (its parent decides)
+    int held_l = 0, held_b = 0;
+    while (m > dl) {
+        const int b = kv.last_from(a, lo, m);
+        const bool bucket = (b - a + 1) <= leaf_size || m == LEVELS;
+        if (first) {
+            deepest_is_bucket = bucket;
+            first = false;
+        }
+        if (bucket) {
+            held = true;  // a shallower bucket replaces it: deeper cells are inside
+            held_l = m;
+            held_b = b;
+        } else {
+            if (held) {   // its parent is this internal node: keep it
+                visit(held_l, held_b, true);
+                held = false;
+            }
+            visit(m, b, false);
+        }
+        lo = b + 1;
         m = kv.d(b);
     }
-    for (int s = t - 1; s >= 0; --s) {
-        const int size = te[s] - a + 1;
-        if (size <= leaf_size || tl[s] == LEVELS) {
-            bool emit = true;
-            if (s == t - 1) {  // its parent starts before a (or it is the root)
-                const int pl = max(dl, kv.d(te[s]));
-                if (pl >= 0) emit = kv.last(a, pl) - kv.first(a, pl) + 1 > leaf_size;
-            }
-            if (emit) {
-                w.lvl[w.cnt] = tl[s];
-                w.end[w.cnt] = te[s];
-                w.bucket[w.cnt] = true;
-                ++w.cnt;
-            }
-            return;  // everything deeper at a, and body a itself, is inside the bucket
-        }
-        w.lvl[w.cnt] = tl[s];
-        w.end[w.cnt] = te[s];
-        w.bucket[w.cnt] = false;
-        ++w.cnt;
+    if (held) {
+        // the shallowest node at a is a bucket; its parent starts before a, at level dl
+        bool keep = true;
+        if (dl >= 0) keep = kv.last(a, dl) - kv.first(a, dl) + 1 > leaf_size;
+        if (keep) visit(held_l, held_b, true);
     }
-    if (t > 0) {
-        w.leaf = true;  // the deepest node holding a starts at a and is internal
-    } else {
-        // the deepest node holding a starts before a, at level d(a-1)
-        const int size = kv.last(a, dl) - kv.first(a, dl) + 1;
-        w.leaf = size > leaf_size && dl < LEVELS;
-    }
+    return !deepest_is_bucket;
 }
 
+// cnt[a]: entries position a adds to the pre-order (kept nodes + leaf); kept[a]: nodes only
 __global__ __launch_bounds__(256) void k_count(const unsigned long long* keys, int n, int leaf_size,
-                                               unsigned* cnt)
+                                               unsigned* cnt, unsigned char* kept)
 {
     const int a = blockIdx.x * 256 + threadIdx.x;
     if (a >= n) return;
-    Walk w;
-    walk_position(KeyView{ keys, n }, a, leaf_size, w);
-    cnt[a] = unsigned(w.cnt + (w.leaf ? 1 : 0));
+    unsigned nodes = 0;
+    const bool leaf = walk_position(KeyView{ keys, n }, a, leaf_size, [&](int, int, bool) { ++nodes; });
+    cnt[a] = nodes + (leaf ? 1u : 0u);
+    kept[a] = (unsigned char)nodes;
 }
 
 struct EmitArgs {
@@ -387,6 +432,7 @@ struct EmitArgs {
     int n;
     int leaf_size;
     const unsigned* base;  // n + 1 entries
+    const unsigned char* kept;
     const Dbl4* pref;      // n + 1 entries
     const float4* spm;
     float size;
@@ -401,13 +447,16 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
 #pragma clang fp contract(off)
     const int a = blockIdx.x * 256 + threadIdx.x;
     if (a >= e.n) return;
-    Walk w;
-    walk_position(KeyView{ e.keys, e.n }, a, e.leaf_size, w);
-    unsigned slot = e.base[a];
+    const unsigned base = e.base[a];
+    const unsigned kept = e.kept[a];
+    const bool leaf_kept = e.base[a + 1] - base > kept;
     const float theta_sq = e.theta * e.theta;
-    for (int j = 0; j < w.cnt; ++j, ++slot) {
+    // nodes kept at a occupy [base, base + kept) shallow to deep; the walk sees them deep
+    // to shallow, so they are placed from the top of that range down
+    unsigned next_slot = base + kept;
+    walk_position(KeyView{ e.keys, e.n }, a, e.leaf_size, [&](int lvl, int b, bool bucket) {
+        const unsigned slot = --next_slot;
         if (slot >= e.capacity) return;
-        const int b = w.end[j];
         const Dbl4 hi = e.pref[b + 1], lo = e.pref[a];
         const double M = hi.m - lo.m;
         BhNode nd;
@@ -417,22 +466,25 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
         else
             nd.cm = make_float4(0.f, 0.f, 0.f, 0.f);
         float s = e.size;  // the cell edge at this level, halved as the reference does
-        for (int l = 0; l < w.lvl[j]; ++l) s = s * .5f;
+        for (int l = 0; l < lvl; ++l) s = s * .5f;
         const float s2 = s * s;
         nd.thr = e.opening == NBHIP_OPEN_TEXTBOOK ? s2 / theta_sq : s2 * theta_sq;
         nd.next = e.base[b + 1];
         nd.first = unsigned(a);
-        nd.count = w.bucket[j] ? unsigned(b - a + 1) : 0u;
+        nd.count = bucket ? unsigned(b - a + 1) : 0u;
         e.nodes[slot] = nd;
-    }
-    if (w.leaf && slot < e.capacity) {
-        BhNode nd;
-        nd.cm = e.spm[a];
-        nd.thr = 0.f;
-        nd.next = e.base[a + 1];
-        nd.first = unsigned(a);
-        nd.count = 1;
-        e.nodes[slot] = nd;
+    });
+    if (leaf_kept) {
+        const unsigned slot = base + kept;
+        if (slot < e.capacity) {
+            BhNode nd;
+            nd.cm = e.spm[a];
+            nd.thr = 0.f;
+            nd.next = e.base[a + 1];
+            nd.first = unsigned(a);
+            nd.count = 1;
+            e.nodes[slot] = nd;
+        }
     }
 }
 
@@ -583,6 +635,26 @@ unsigned grid(size_t n) { return unsigned((n + 255) / 256); }
 
 }  // namespace
 
+void bh_reserve(nbhip_ctx* c)
+{
+    const int n = int(c->count);
+    hipStream_t s = c->stream;
+    c->bh_keys[0].grow(n, 0, s);
+    c->bh_keys[1].grow(n, 0, s);
+    c->bh_vals[0].grow(n, 0, s);
+    c->bh_vals[1].grow(n, 0, s);
+    c->bh_spm.grow(n, 0, s);
+    c->bh_moments.grow(n, 0, s);
+    c->bh_pref.grow(size_t(n) + 1, 0, s);
+    c->bh_base.grow(size_t(n) + 1, 0, s);
+    c->bh_nodes.grow(2 * size_t(n) + 1, 0, s);
+    c->bh_kept.grow(n, 0, s);
+    const int nb = std::max(1, (n + RS_TILE - 1) / RS_TILE);
+    c->bh_hist.grow(size_t(256) * nb + 1, 0, s);
+    c->bh_uscan.grow(size_t(std::max(1, (std::max(256 * nb, n) + SC_TILE - 1) / SC_TILE)), 0, s);
+    c->bh_dscan.grow(size_t(std::max(1, (n + SC_TILE - 1) / SC_TILE)), 0, s);
+}
+
 void bh_build(nbhip_ctx* c, const nbhip_params& p)
 {
     const int n = int(c->count);
@@ -596,6 +668,7 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     c->bh_pref.grow(size_t(n) + 1, 0, s);
     c->bh_base.grow(size_t(n) + 1, 0, s);
     c->bh_nodes.grow(2 * size_t(n) + 1, 0, s);
+    c->bh_kept.grow(n, 0, s);
 
     hipLaunchKernelGGL(k_keys, dim3(grid(n)), dim3(256), 0, s, c->pm[c->cur].ptr, n, p.size,
                        c->bh_keys[0].ptr, c->bh_vals[0].ptr);
@@ -611,12 +684,13 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     exclusive_scan<Dbl4>(c, c->bh_moments.ptr, n, c->bh_pref.ptr, c->bh_dscan);
 
     const int leaf = std::max(1, p.leaf_size);
-    hipLaunchKernelGGL(k_count, dim3(grid(n)), dim3(256), 0, s, keys, n, leaf, c->bh_base.ptr);
+    hipLaunchKernelGGL(k_count, dim3(grid(n)), dim3(256), 0, s, keys, n, leaf, c->bh_base.ptr,
+                       c->bh_kept.ptr);
     NBHIP_CHECK(hipGetLastError());
     exclusive_scan<unsigned>(c, c->bh_base.ptr, n, c->bh_base.ptr, c->bh_uscan);
 
-    EmitArgs e{ keys, n, leaf, c->bh_base.ptr, c->bh_pref.ptr, c->bh_spm.ptr, p.size, p.theta,
-                p.opening, c->bh_nodes.ptr, unsigned(c->bh_nodes.cap) };
+    EmitArgs e{ keys, n, leaf, c->bh_base.ptr, c->bh_kept.ptr, c->bh_pref.ptr, c->bh_spm.ptr, p.size,
+                p.theta, p.opening, c->bh_nodes.ptr, unsigned(c->bh_nodes.cap) };
     hipLaunchKernelGGL(k_emit, dim3(grid(n)), dim3(256), 0, s, e);
     NBHIP_CHECK(hipGetLastError());
     c->bh_built = true;
@@ -697,6 +771,7 @@ void bh_release(nbhip_ctx* c)
     c->bh_uscan.release();
     c->bh_dscan.release();
     c->bh_visits.release();
+    c->bh_kept.release();
 }
 
 }  // namespace nbhip

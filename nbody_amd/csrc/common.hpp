@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdarg>
 #include <cstddef>
 #include <cstdint>
@@ -29,6 +30,10 @@ struct Error {
             ::nbhip::fail(NBHIP_ERR_HIP, "%s failed: %s (%s:%d)", #expr,                   \
                           hipGetErrorString(_e), __FILE__, __LINE__);                      \
     } while (0)
+
+// Bumped whenever any device buffer moves: captured graphs bake buffer addresses and are
+// replayed only while the epoch they were captured at is current.
+extern std::atomic<unsigned long long> g_buffer_epoch;
 
 // A grow-only device allocation.
 template <typename T>
@@ -58,6 +63,7 @@ struct DevBuf {
         if (ptr) (void)hipFree(ptr);
         ptr = p;
         cap = want;
+        g_buffer_epoch.fetch_add(1);
     }
 };
 
@@ -131,6 +137,7 @@ struct nbhip_ctx {
     nbhip::DevBuf<nbhip::Dbl4> bh_moments;
     nbhip::DevBuf<nbhip::Dbl4> bh_pref;
     nbhip::DevBuf<unsigned> bh_base;
+    nbhip::DevBuf<unsigned char> bh_kept;
     nbhip::DevBuf<nbhip::BhNode> bh_nodes;
     nbhip::DevBuf<unsigned> bh_hist;
     nbhip::DevBuf<unsigned> bh_uscan;
@@ -138,6 +145,11 @@ struct nbhip_ctx {
     nbhip::DevBuf<unsigned> bh_visits;
     int bh_sorted = 0;          // which keys/vals buffer holds the sorted result
     bool bh_built = false;      // a tree of the current count exists on the device
+
+    // captured multi-step sequence (nbhip_step with steps >= 2)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::string graph_key;
 
     // timing
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -172,5 +184,7 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p);
 void bh_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse_integrate, bool count);
 void bh_release(nbhip_ctx* c);
 void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm);
+void bh_reserve(nbhip_ctx* c);
+DirectConfig context_config(const nbhip_ctx* c);
 
 }  // namespace nbhip

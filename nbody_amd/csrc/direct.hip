@@ -377,6 +377,8 @@ void direct_pass(hipStream_t stream, const SysView& v, const nbhip_params& p, bo
     }
 }
 
+std::atomic<unsigned long long> g_buffer_epoch{ 0 };
+
 // Launch shape for a context: an explicit override in the environment wins, then the
 // autotuner's choice for this body count, then the default heuristic.
 DirectConfig context_config(const nbhip_ctx* c)

@@ -69,6 +69,15 @@ void check_params(const nbhip_params* p)
     }
 }
 
+void drop_graph(nbhip_ctx* c)
+{
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->graph_key.clear();
+}
+
 // Is `name` a gfx950 architecture string ("gfx950", "gfx950:sramecc+:xnack-", ...)?
 bool is_gfx950(const char* name) { return std::strncmp(name, "gfx950", 6) == 0; }
 
@@ -162,6 +171,7 @@ void nbhip_destroy(nbhip_ctx* c)
     c->dred.release();
     c->fscratch.release();
     c->uscratch.release();
+    drop_graph(c);
     nbhip::bh_release(c);
     if (c->ev_start) (void)hipEventDestroy(c->ev_start);
     if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
@@ -304,6 +314,71 @@ int nbhip_integrate(nbhip_ctx* c, const nbhip_params* p)
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+void enqueue_steps(nbhip_ctx* c, const nbhip_params& p, int steps)
+{
+    for (int s = 0; s < steps; ++s) {
+        if (p.mode == NBHIP_MODE_N2)
+            nbhip::direct_accelerate(c, p, true);
+        else
+            nbhip::bh_accelerate(c, p, true, false);
+    }
+}
+
+// nbhip_step_graph captures a step sequence once into a hipGraph and replays it: a
+// Barnes-Hut step is ~20 launches, so for small systems the graph removes most of the
+// launch cost.  Every
+// allocation and host query a step makes is done first, outside the capture; the graph
+// is reused while mode, body count, parameters, parity, launch shape and every buffer
+// address (the epoch) are unchanged.
+void step_graph(nbhip_ctx* c, const nbhip_params& p, int steps)
+{
+    nbhip::ensure_radius_flag(c);
+    const nbhip::DirectConfig cfg = nbhip::context_config(c);
+    if (p.mode == NBHIP_MODE_N2) {
+        if (cfg.jsplit > 1) c->partial.grow(size_t(cfg.jsplit) * 3 * c->count, 0, c->stream);
+    } else {
+        nbhip::bh_reserve(c);
+    }
+    char key[256];
+    std::snprintf(key, sizeof key, "%d/%d/%zu/%d/%d/%d/%d/%d/%llu/", p.mode, steps, c->count, c->cur,
+                  cfg.P, cfg.W, cfg.jsplit, int(c->any_radius), nbhip::g_buffer_epoch.load());
+    std::string k(key);
+    k.append(reinterpret_cast<const char*>(&p), sizeof p);
+    const int cur_after = p.mode == NBHIP_MODE_N2 ? (c->cur ^ (steps & 1)) : c->cur;
+    if (!(c->graph_exec && c->graph_key == k)) {
+        drop_graph(c);
+        const int cur0 = c->cur;
+        const long launches0 = c->launches;
+        NBHIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_steps(c, p, steps);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(c->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            c->cur = cur0;
+            c->launches = launches0;
+            throw;
+        }
+        NBHIP_CHECK(hipStreamEndCapture(c->stream, &c->graph));
+        NBHIP_CHECK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+        c->graph_key = k;
+        c->cur = cur0;           // capture recorded the steps; the replay runs them
+        c->launches = launches0;
+    }
+    NBHIP_CHECK(hipGraphLaunch(c->graph_exec, c->stream));
+    c->cur = cur_after;
+    c->launches += steps;
+}
+
+}  // namespace
+
+extern "C" {
+
 int nbhip_step(nbhip_ctx* c, const nbhip_params* p, int steps)
 {
     return guarded(c, [&] {
@@ -312,12 +387,19 @@ int nbhip_step(nbhip_ctx* c, const nbhip_params* p, int steps)
         if (steps < 0) fail(NBHIP_ERR_INVALID, "negative step count");
         if (c->count == 0) return;
         NBHIP_CHECK(hipSetDevice(c->device));
-        for (int s = 0; s < steps; ++s) {
-            if (p->mode == NBHIP_MODE_N2)
-                nbhip::direct_accelerate(c, *p, true);
-            else
-                nbhip::bh_accelerate(c, *p, true, false);
-        }
+        enqueue_steps(c, *p, steps);
+    });
+}
+
+int nbhip_step_graph(nbhip_ctx* c, const nbhip_params* p, int steps)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        check_params(p);
+        if (steps < 0) fail(NBHIP_ERR_INVALID, "negative step count");
+        if (c->count == 0 || steps == 0) return;
+        NBHIP_CHECK(hipSetDevice(c->device));
+        step_graph(c, *p, steps);
     });
 }
 

@@ -215,3 +215,56 @@ def test_autotuner_picks_a_shape_and_keeps_results(ctx, orc):
     a = ctx.download(_lib.RB_ACCELERATIONS)[2][:, :3]
     s = np.arange(0, 20000, 101, dtype=np.uint32)
     assert rel_err(a[s], orc.direct_f64(b, s, eps=0.01)).max() <= 1e-5
+
+
+@pytest.mark.parametrize("mode", [_lib.MODE_N2, _lib.MODE_NLOGN])
+def test_step_graph_replays_equal_eager(ctx, mode):
+    # nbhip_step_graph replays a captured hipGraph; every replay (with eager work and
+    # synchronisations in between) must equal the eager path bit for bit
+    b = ics.disk_bulge(6000, seed=41)
+    p = _lib.params(mode=mode, dt=1e-3, eps=0.01, size=128.0, wrap=1, theta=0.5, leaf_size=8)
+    results = {}
+    for graph in [False, True]:
+        ctx.upload_bodies(b)
+        snaps = []
+        for _ in range(3):
+            if graph:
+                ctx.step_graph(p, 4)       # replayed from the cached graph after the first
+            else:
+                ctx.step(p, 4)
+            snaps.append(ctx.download_bodies())
+            ctx.accelerate(p)              # eager work between replays
+            ctx.synchronize()
+        results[graph] = snaps
+    for e, g in zip(results[False], results[True]):
+        assert np.array_equal(bits(e), bits(g))
+
+
+def test_c1_plain_direct_sum_100_steps(ctx, orc):
+    # The reference's CPU-runnable case: N=4096 Plummer (a=1, M=1, G=1), eps=0.01, dt=1e-3,
+    # 100 integrate steps, GPU vs the reference brute-force restatement stepping alongside.
+    # Bars: positions within 1e-5 (absolute, the cloud has radius ~1) and velocities within
+    # 1e-4 relative after 100 steps; both runs conserve energy to the same 1e-6.
+    n, steps = 4096, 100
+    b = ics.plummer(n, seed=2026)
+    p = _lib.params(dt=1e-3, eps=0.01, size=10000.0, wrap=1)
+    ctx.upload_bodies(b)
+    e0 = ctx.diagnostics(p)
+    for _ in range(steps):
+        ctx.step(p, 1)
+    gpu = ctx.download_bodies()
+    e1 = ctx.diagnostics(p)
+    cpu = b.copy()
+    for _ in range(steps):
+        orc.brute_force(cpu, eps=0.01)
+        orc.integrate(cpu, np.float32(1e-3), 10000.0, True)
+    dx = np.abs(gpu[:, 0:3] - cpu[:, 0:3]).max()
+    dv = rel_err(gpu[:, 4:7], cpu[:, 4:7])
+    ec0 = orc.energy_f64(b, eps=0.01)
+    ec1 = orc.energy_f64(cpu, eps=0.01)
+    drift_gpu = ((e1[0] + e1[1]) - (e0[0] + e0[1])) / abs(e0[0] + e0[1])
+    drift_cpu = ((ec1[0] + ec1[1]) - (ec0[0] + ec0[1])) / abs(ec0[0] + ec0[1])
+    print(f"C1 after {steps} steps: max |dx| {dx:.2e}, max rel dv {dv.max():.2e}, "
+          f"energy drift gpu {drift_gpu:.3e} cpu {drift_cpu:.3e}")
+    assert dx <= 1e-5 and dv.max() <= 1e-4
+    assert abs(drift_gpu - drift_cpu) <= 1e-6
