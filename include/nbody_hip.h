@@ -225,6 +225,39 @@ int nbhip_bh_nodes(nbhip_ctx* ctx, const nbhip_params* p, nbhip_bh_node* nodes, 
 int nbhip_bh_interactions(nbhip_ctx* ctx, const nbhip_params* p, uint32_t* per_body,
                           uint64_t* total);
 
+/* One node of the reference's octree, in its 48-byte layout (include/nbody/bhtree.h:13-22,
+ * bh::Node with Bounds from include/nbody/bounds.h:11-14): a cell with two or more bodies
+ * has 8 contiguous children (empty cells included, mass 0) at children + q, q the
+ * reference octant code; a leaf holds one body (children = 0); next is the index of the
+ * next node at this level, or the parent's next (0 after the last node). */
+typedef struct nbhip_tree_node {
+    float    center[3];
+    float    size;       /* cell edge */
+    float    com[3];     /* one body: its position; more: FP64 centre of mass, rounded */
+    float    mass;
+    uint32_t next;
+    uint32_t children;   /* index of the first of 8 children, 0 for a leaf */
+    uint32_t pad0;
+    uint32_t pad1;
+} nbhip_tree_node;
+
+/* Exports an octree in the reference's bh::Node layout: the tree the reference's
+ * detail::build_tree (source/detail/tree.h:14-29, source/bhtree.cpp:14-95) builds from
+ * the same positions, with the same cells, bounds (bit-exact), node threading and
+ * single-body leaves, numbered differently (internal cells in depth-first order, each
+ * one's children at 1 + 8 * its number) and with centres of mass summed in FP64.  Cells
+ * of 21 levels (edge p->size / 2^21) that still hold several bodies stay leaves, where
+ * the reference would keep splitting them.
+ *   p non-NULL: built from the current positions, root cube of edge p->size at the origin.
+ *   p NULL: the tree of this context's most recent Barnes-Hut build -- the tree the last
+ *           NLOGN accelerate or step used, from the positions at its start, as the
+ *           reference's Solver::tree() returns (source/solvers/gpu_solver_split.h:115-118);
+ *           NBHIP_ERR_INVALID if there is none for the current body count.
+ * Copies out at most `cap` nodes; *count receives the node count (1 + 8 * internal cells;
+ * 1 for N = 0 or 1).  Synchronises. */
+int nbhip_bh_tree(nbhip_ctx* ctx, const nbhip_params* p, nbhip_tree_node* nodes, size_t cap,
+                  size_t* count);
+
 /* ---- batches: many independent direct-sum systems stepped together ------------------
  * No reference counterpart: the reference runs one simulation per Sim.  A batch lives on
  * a context (its stream, its device) and holds systems of any size, each with its own

@@ -37,6 +37,14 @@ BH_NODE_DTYPE = np.dtype([
 ])
 assert BH_NODE_DTYPE.itemsize == 32
 
+# The reference's 48-byte bh::Node (nbhip_tree_node).
+TREE_NODE_DTYPE = np.dtype([
+    ("center", np.float32, 3), ("size", np.float32),
+    ("com", np.float32, 3), ("mass", np.float32),
+    ("next", np.uint32), ("children", np.uint32), ("pad0", np.uint32), ("pad1", np.uint32),
+])
+assert TREE_NODE_DTYPE.itemsize == 48
+
 
 class Params(C.Structure):
     """nbhip_params (include/nbody_hip.h)."""
@@ -93,6 +101,7 @@ SIGNATURES = {
     "nbhip_bh_keys": (C.c_int, [_vp, C.POINTER(Params), _vp, _vp]),
     "nbhip_bh_nodes": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, C.POINTER(_sz)]),
     "nbhip_bh_interactions": (C.c_int, [_vp, C.POINTER(Params), _vp, C.POINTER(C.c_uint64)]),
+    "nbhip_bh_tree": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, C.POINTER(_sz)]),
     "nbhip_batch_create": (C.c_int, [_vp, C.POINTER(_vp)]),
     "nbhip_batch_destroy": (None, [_vp]),
     "nbhip_batch_add": (C.c_int, [_vp, _vp, _sz, C.POINTER(Params), C.POINTER(C.c_int)]),
@@ -295,6 +304,16 @@ class Context:
         self._check(self.L.nbhip_bh_nodes(self.h, C.byref(p), None, 0, C.byref(cnt)))
         nodes = np.zeros(cnt.value, BH_NODE_DTYPE)
         self._check(self.L.nbhip_bh_nodes(self.h, C.byref(p), _ptr(nodes), nodes.size, C.byref(cnt)))
+        return nodes
+
+    def bh_tree(self, p: Params | None = None) -> np.ndarray:
+        """The octree in the reference's bh::Node layout: built from the current positions
+        with p, or (p None) the tree of the last Barnes-Hut build."""
+        pp = C.byref(p) if p is not None else None
+        cnt = C.c_size_t()
+        self._check(self.L.nbhip_bh_tree(self.h, pp, None, 0, C.byref(cnt)))
+        nodes = np.zeros(cnt.value, TREE_NODE_DTYPE)
+        self._check(self.L.nbhip_bh_tree(self.h, pp, _ptr(nodes), nodes.size, C.byref(cnt)))
         return nodes
 
     def bh_interactions(self, p: Params) -> tuple[np.ndarray, int]:

@@ -631,6 +631,128 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a)
     }
 }
 
+// ---- 6. export in the reference's node layout ----------------------------------------------
+// The reference's tree (source/bhtree.cpp:14-95) splits every cell that holds two or more
+// bodies into 8 contiguous children, empty ones included, down to one body per leaf.  Its
+// cells are therefore the cells of the sorted keys: the cell of depth L that starts at
+// sorted position a holds two or more bodies iff d(a) >= L, and starts at a iff
+// d(a-1) < L.  Numbering these internal cells in depth-first order (by start position,
+// then depth) by a scan of per-position counts gives each one's children block directly:
+// internal cell g has its child q at node 1 + 8 g + q, and the root is node 0.
+__device__ __forceinline__ unsigned key_digit(unsigned long long k, int level)
+{
+    return unsigned(k >> (63 - 3 * level)) & 7u;  // level 1 is the root's child
+}
+
+__global__ __launch_bounds__(256) void k_tree_count(const unsigned long long* keys, int n, unsigned* cnt)
+{
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    if (a >= n) return;
+    const KeyView kv{ keys, n };
+    const int lo = kv.d(a - 1);
+    const int hi = min(kv.d(a), LEVELS - 1);
+    cnt[a] = hi > lo ? unsigned(hi - lo) : 0u;
+}
+
+struct TreeArgs {
+    const unsigned long long* keys;
+    int n;
+    const unsigned* ibase;  // n + 1 entries: internal cells starting before each position
+    const Dbl4* pref;       // n + 1 entries
+    const float4* spm;
+    float size;
+    TreeNode* nodes;
+    unsigned capacity;      // nodes allocated; entries past it are never written
+};
+
+// Mass and centre of bodies [a, b] in key order: a single body keeps its own position
+// and mass, as a reference leaf does; more take the FP64 sums.
+__device__ __forceinline__ float4 tree_mass(const TreeArgs& t, int a, int b)
+{
+    if (a == b) return t.spm[a];
+    const Dbl4 hi = t.pref[b + 1], lo = t.pref[a];
+    const double M = hi.m - lo.m;
+    if (M == 0.0) return make_float4(0.f, 0.f, 0.f, 0.f);
+    return make_float4(float((hi.x - lo.x) / M), float((hi.y - lo.y) / M), float((hi.z - lo.z) / M),
+                       float(M));
+}
+
+__device__ __forceinline__ void tree_put(const TreeArgs& t, unsigned idx, float4 cs, float4 cm,
+                                         unsigned next, unsigned children)
+{
+    if (idx >= t.capacity) return;
+    TreeNode* nd = t.nodes + idx;
+    nd->cs = cs;
+    nd->cm = cm;
+    nd->link = make_uint4(next, children, 0u, 0u);
+}
+
+// One thread per sorted position a: writes the children of every internal cell that
+// starts at a (position 0 also writes the root).
+__global__ __launch_bounds__(256) void k_tree_emit(TreeArgs t)
+{
+#pragma clang fp contract(off)
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    if (a >= t.n) return;
+    const KeyView kv{ t.keys, t.n };
+    if (a == 0)
+        tree_put(t, 0, make_float4(0.f, 0.f, 0.f, t.size), tree_mass(t, 0, t.n - 1), 0u,
+                 t.n > 1 ? 1u : 0u);
+    const int lo = kv.d(a - 1);
+    const int hi = min(kv.d(a), LEVELS - 1);
+    if (hi <= lo) return;
+    const unsigned long long ka = t.keys[a];
+
+    // bounds of the cells holding a, descending as the reference's quadrant_bounds does
+    // (include/nbody/bounds.h:40-52)
+    float cx = 0.f, cy = 0.f, cz = 0.f, s = t.size;
+    auto descend = [&](unsigned q) {
+        const float quart = s * .25f;
+        cx = (q & 1u) ? (cx - quart) : (cx + quart);
+        cy = (q & 2u) ? (cy - quart) : (cy + quart);
+        cz = (q & 4u) ? (cz - quart) : (cz + quart);
+        s = s * .5f;
+    };
+    for (int l = 1; l <= lo + 1; ++l) descend(key_digit(ka, l));
+
+    for (int L = lo + 1; L <= hi; ++L) {
+        const unsigned g = t.ibase[a] + unsigned(L - lo - 1);
+        const int b = kv.last(a, L);
+        // this cell's own next: the sibling after its deepest ancestor-or-self that is not
+        // the last child of its parent (0 when there is none)
+        unsigned own_next = 0;
+        for (int j = L; j >= 1; --j) {
+            const unsigned qj = key_digit(ka, j);
+            if (qj < 7u) {
+                const int s0 = kv.first(a, j - 1);
+                const unsigned ga = t.ibase[s0] + unsigned(j - 1 - kv.d(s0 - 1) - 1);
+                own_next = 1u + 8u * ga + qj + 1u;
+                break;
+            }
+        }
+        const float quart = s * .25f, half = s * .5f;
+        const unsigned cb = 1u + 8u * g;
+        int pos = a;
+        for (unsigned q = 0; q < 8u; ++q) {
+            const float4 cs = make_float4((q & 1u) ? (cx - quart) : (cx + quart),
+                                          (q & 2u) ? (cy - quart) : (cy + quart),
+                                          (q & 4u) ? (cz - quart) : (cz + quart), half);
+            const unsigned next = q < 7u ? cb + q + 1u : own_next;
+            if (pos <= b && key_digit(t.keys[pos], L + 1) == q) {
+                const int e = kv.last(pos, L + 1);
+                unsigned children = 0;
+                if (e > pos && L + 1 < LEVELS)
+                    children = 1u + 8u * (t.ibase[pos] + unsigned(L + 1 - kv.d(pos - 1) - 1));
+                tree_put(t, cb + q, cs, tree_mass(t, pos, e), next, children);
+                pos = e + 1;
+            } else {
+                tree_put(t, cb + q, cs, make_float4(0.f, 0.f, 0.f, 0.f), next, 0u);
+            }
+        }
+        if (L < hi) descend(key_digit(ka, L + 1));
+    }
+}
+
 unsigned grid(size_t n) { return unsigned((n + 255) / 256); }
 
 }  // namespace
@@ -694,6 +816,8 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     hipLaunchKernelGGL(k_emit, dim3(grid(n)), dim3(256), 0, s, e);
     NBHIP_CHECK(hipGetLastError());
     c->bh_built = true;
+    c->bh_size = p.size;
+    ++c->bh_builds;
 }
 
 void bh_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse, bool count)
@@ -749,11 +873,42 @@ void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm
         NBHIP_CHECK(hipStreamSynchronize(s));
     }
     const int sorted = radix_sort(c, n, 0);
+    c->bh_built = false;  // the key buffers no longer belong to the last build
     if (perm) {
         NBHIP_CHECK(hipMemcpyAsync(perm, c->bh_vals[sorted].ptr, size_t(n) * sizeof(uint32_t),
                                    hipMemcpyDeviceToHost, s));
     }
     NBHIP_CHECK(hipStreamSynchronize(s));
+}
+
+size_t bh_tree(nbhip_ctx* c, const nbhip_params* p)
+{
+    const int n = int(c->count);
+    if (p) {
+        bh_build(c, *p);
+    } else if (!c->bh_built) {
+        fail(NBHIP_ERR_INVALID, "no Barnes-Hut tree has been built for the current bodies");
+    }
+    if (c->bh_tree_build == c->bh_builds) return c->bh_tree_total;
+    hipStream_t s = c->stream;
+    const unsigned long long* keys = c->bh_keys[c->bh_sorted].ptr;
+    c->bh_icount.grow(size_t(n) + 1, 0, s);
+    hipLaunchKernelGGL(k_tree_count, dim3(grid(n)), dim3(256), 0, s, keys, n, c->bh_icount.ptr);
+    NBHIP_CHECK(hipGetLastError());
+    exclusive_scan<unsigned>(c, c->bh_icount.ptr, n, c->bh_icount.ptr, c->bh_uscan);
+    unsigned internal = 0;
+    NBHIP_CHECK(hipMemcpyAsync(&internal, c->bh_icount.ptr + n, sizeof internal, hipMemcpyDeviceToHost, s));
+    NBHIP_CHECK(hipStreamSynchronize(s));
+    const size_t total = 1 + 8 * size_t(internal);
+    if (total > 0xffffffffull) fail(NBHIP_ERR_UNSUPPORTED, "tree of %zu nodes exceeds 32-bit indices", total);
+    c->bh_tree.grow(total, 0, s);
+    TreeArgs t{ keys, n, c->bh_icount.ptr, c->bh_pref.ptr, c->bh_spm.ptr, c->bh_size, c->bh_tree.ptr,
+                unsigned(c->bh_tree.cap < 0xffffffffull ? c->bh_tree.cap : 0xffffffffull) };
+    hipLaunchKernelGGL(k_tree_emit, dim3(grid(n)), dim3(256), 0, s, t);
+    NBHIP_CHECK(hipGetLastError());
+    c->bh_tree_total = total;
+    c->bh_tree_build = c->bh_builds;
+    return total;
 }
 
 void bh_release(nbhip_ctx* c)
@@ -772,6 +927,10 @@ void bh_release(nbhip_ctx* c)
     c->bh_dscan.release();
     c->bh_visits.release();
     c->bh_kept.release();
+    c->bh_icount.release();
+    c->bh_tree.release();
+    c->bh_built = false;
+    c->bh_tree_build = ~0ull;
 }
 
 }  // namespace nbhip

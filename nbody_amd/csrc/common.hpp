@@ -94,6 +94,14 @@ struct BhNode {
 };
 static_assert(sizeof(BhNode) == sizeof(nbhip_bh_node), "BhNode must match nbhip_bh_node");
 
+// One node in the reference's 48-byte octree layout (nbhip_tree_node).
+struct TreeNode {
+    float4 cs;         // centre xyz, edge
+    float4 cm;         // centre of mass xyz, mass
+    uint4 link;        // next, children, 0, 0
+};
+static_assert(sizeof(TreeNode) == sizeof(nbhip_tree_node), "TreeNode must match nbhip_tree_node");
+
 // FP64 running sums of m, m x, m y, m z over bodies in key order.
 struct Dbl4 {
     double m, x, y, z;
@@ -145,6 +153,14 @@ struct nbhip_ctx {
     nbhip::DevBuf<unsigned> bh_visits;
     int bh_sorted = 0;          // which keys/vals buffer holds the sorted result
     bool bh_built = false;      // a tree of the current count exists on the device
+    float bh_size = 0.f;        // root edge of that tree
+    unsigned long long bh_builds = 0;  // builds enqueued so far (graph replays included)
+
+    // the last build in the reference's node layout (nbhip_bh_tree), made on demand
+    nbhip::DevBuf<unsigned> bh_icount;
+    nbhip::DevBuf<nbhip::TreeNode> bh_tree;
+    unsigned long long bh_tree_build = ~0ull;  // bh_builds when bh_tree was made
+    size_t bh_tree_total = 0;
 
     // captured multi-step sequence (nbhip_step with steps >= 2)
     hipGraph_t graph = nullptr;
@@ -185,6 +201,7 @@ void bh_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse_integrate, boo
 void bh_release(nbhip_ctx* c);
 void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm);
 void bh_reserve(nbhip_ctx* c);
+size_t bh_tree(nbhip_ctx* c, const nbhip_params* p);
 DirectConfig context_config(const nbhip_ctx* c);
 
 }  // namespace nbhip

@@ -195,6 +195,7 @@ int nbhip_reserve(nbhip_ctx* c, size_t n)
         c->pm[1].grow(n, keep, c->stream);
         c->vr.grow(n, keep, c->stream);
         c->acc.grow(n, keep, c->stream);
+        if (n != c->count) c->bh_built = false;
         c->count = n;
         c->radius_known = false;
         c->tuned_n = 0;
@@ -373,6 +374,7 @@ void step_graph(nbhip_ctx* c, const nbhip_params& p, int steps)
     NBHIP_CHECK(hipGraphLaunch(c->graph_exec, c->stream));
     c->cur = cur_after;
     c->launches += steps;
+    if (p.mode != NBHIP_MODE_N2) c->bh_builds += steps;  // the replay rebuilt the tree
 }
 
 }  // namespace
@@ -510,6 +512,39 @@ int nbhip_bh_nodes(nbhip_ctx* c, const nbhip_params* p, nbhip_bh_node* nodes, si
                                        hipMemcpyDeviceToHost, c->stream));
             NBHIP_CHECK(hipStreamSynchronize(c->stream));
         }
+    });
+}
+
+int nbhip_bh_tree(nbhip_ctx* c, const nbhip_params* p, nbhip_tree_node* nodes, size_t cap,
+                  size_t* count)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        if (p) {
+            check_params(p);
+            if (!(p->size > 0.f)) fail(NBHIP_ERR_INVALID, "the tree needs a positive root size");
+        }
+        if (count) *count = 0;
+        if (c->count == 0) {
+            // the reference's cleared tree: the root cell alone, empty
+            if (!p) fail(NBHIP_ERR_INVALID, "no Barnes-Hut tree has been built for the current bodies");
+            if (count) *count = 1;
+            if (cap >= 1) {
+                if (!nodes) fail(NBHIP_ERR_INVALID, "null nodes");
+                nodes[0] = nbhip_tree_node{ { 0.f, 0.f, 0.f }, p->size, { 0.f, 0.f, 0.f }, 0.f, 0u, 0u, 0u, 0u };
+            }
+            return;
+        }
+        NBHIP_CHECK(hipSetDevice(c->device));
+        const size_t total = nbhip::bh_tree(c, p);
+        if (count) *count = total;
+        const size_t k = total < cap ? total : cap;
+        if (k) {
+            if (!nodes) fail(NBHIP_ERR_INVALID, "null nodes");
+            NBHIP_CHECK(hipMemcpyAsync(nodes, c->bh_tree.ptr, k * sizeof(nbhip_tree_node),
+                                       hipMemcpyDeviceToHost, c->stream));
+        }
+        NBHIP_CHECK(hipStreamSynchronize(c->stream));
     });
 }
 

@@ -1,9 +1,19 @@
 #include "hip_solver.h"
 
+#include <cstddef>
+
 #include "context.h"
 
 namespace nbody
 {
+    // The export writes bh::Node records directly.
+    static_assert(sizeof(bh::Node) == sizeof(nbhip_tree_node));
+    static_assert(offsetof(bh::Node, bounds) == offsetof(nbhip_tree_node, center));
+    static_assert(offsetof(bh::Node, com) == offsetof(nbhip_tree_node, com));
+    static_assert(offsetof(bh::Node, mass) == offsetof(nbhip_tree_node, mass));
+    static_assert(offsetof(bh::Node, next) == offsetof(nbhip_tree_node, next));
+    static_assert(offsetof(bh::Node, children) == offsetof(nbhip_tree_node, children));
+
     HipSolver::HipSolver(std::shared_ptr<Context> context, StateRef state, const HipMode mode)
         : Solver(std::move(context), std::move(state))
         , _dev(_context->require_hip())
@@ -23,6 +33,8 @@ namespace nbody
         _state = std::move(state);
         _device_dirty = false;
         _host_dirty = true;
+        _tree_built = false;
+        _tree_stale = true;
     }
 
     void HipSolver::ingest() const
@@ -65,6 +77,7 @@ namespace nbody
         const nbhip_params p = params(0.f);
         _dev->check(nbhip_accelerate(_dev->ctx(), &p), "accelerate");
         _device_dirty = true;
+        dispatched_tree();
     }
 
     void HipSolver::integrate(const float dt)
@@ -89,9 +102,42 @@ namespace nbody
         const nbhip_params p = params(dt);
         _dev->check(nbhip_step(_dev->ctx(), &p, 1), "step");
         _device_dirty = true;
+        dispatched_tree();
     }
 
-    const bh::Tree* HipSolver::tree() const { return nullptr; }
+    void HipSolver::dispatched_tree()
+    {
+        // An NLogN dispatch built a new tree on the device; it is copied out only if asked.
+        if (_mode == HipMode::NLogN) {
+            _tree_built = true;
+            _tree_stale = true;
+        }
+    }
+
+    const bh::Tree* HipSolver::tree() const
+    {
+        // N^2 mode has no tree to offer (source/solvers/gpu_solver_split.h:113-118).
+        if (_mode != HipMode::NLogN)
+            return nullptr;
+        if (_tree_stale) {
+            std::vector<bh::Node>& nodes = _tree.mutable_nodes();
+            if (!_tree_built) {
+                // What adopt() leaves: the root cell alone (gpu_solver_split.h:42).
+                nodes.assign(1, bh::Node{ .bounds = { .center = { 0, 0, 0 }, .size = _state->size } });
+            } else {
+                // The tree the last dispatch used, from the positions at its start, as the
+                // reference's tree() is (gpu_solver_split.h:59-70).
+                size_t count = 0;
+                _dev->check(nbhip_bh_tree(_dev->ctx(), nullptr, nullptr, 0, &count), "export tree");
+                nodes.resize(count);
+                _dev->check(nbhip_bh_tree(_dev->ctx(), nullptr, reinterpret_cast<nbhip_tree_node*>(nodes.data()),
+                                          count, &count),
+                            "export tree");
+            }
+            _tree_stale = false;
+        }
+        return &_tree;
+    }
 
     void HipSolver::materialize() const
     {

@@ -8,6 +8,7 @@
 // authoritative and state() materializes it back into State::bodies.  Stepping never
 // reads back, so a caller that steps without reading moves no data over PCIe.
 #include <memory>
+#include "nbody/bhtree.h"
 #include "solver.h"
 #include "hip_device.h"
 
@@ -42,6 +43,7 @@ namespace nbody
         nbhip_params params(float dt) const;
         void upload_bodies();
         void materialize() const;
+        void dispatched_tree();
 
         std::shared_ptr<HipDevice> _dev;
         HipMode _mode;
@@ -53,5 +55,10 @@ namespace nbody
         mutable bool _device_dirty = false;
         // State::bodies holds data the device has not seen yet.
         mutable bool _host_dirty = true;
+
+        // NLogN: the last dispatch's tree, copied from the device when tree() is called.
+        mutable bh::Tree _tree;
+        mutable bool _tree_stale = true;
+        bool _tree_built = false;  // a dispatch since adopt() built one
     };
 }

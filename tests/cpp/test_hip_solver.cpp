@@ -582,6 +582,51 @@ TEST_CASE("brute force agrees with barnes-hut", "gpu")
     (void)max_relative_acc_error;
 }
 
+TEST_CASE("barnes-hut offers the tree of its last dispatch", "gpu")
+{
+    // source/solvers/gpu_solver_split.h:42 (adopt clears to the root), :59-70 (each
+    // dispatch builds from the positions at its start), :113-118 (tree() in NLogN mode);
+    // mass conservation as tests/source/test_bhtree.cpp:189-212 checks it
+    skip_without(Variant::HipBarnesHut);
+    Sim sim(Variant::HipBarnesHut);
+    sim.set_size(1024.f);
+    seed_disk(sim, 1000);
+    REQUIRE(sim.tree() != nullptr);
+    REQUIRE(sim.nodes().size() == 1);
+    REQUIRE(sim.nodes()[0].mass == 0.f);
+    REQUIRE(sim.nodes()[0].bounds.size == 1024.f);
+
+    for (int step = 0; step < 2; ++step)
+    {
+        INFO("step " << step);
+        const std::vector<Body> start = sim.bodies();
+        sim.update(1.f / 120.f);
+        const std::span<const nbody::bh::Node> nodes = sim.nodes();
+        REQUIRE(nodes.size() > 1);
+        double total = 0;
+        for (const Body& b : start) total += b.mass;
+        REQUIRE(nodes[0].mass == float(total));
+        // every body sits, by the octant descent, in a leaf holding exactly it
+        for (const Body& b : start)
+        {
+            uint32_t i = 0;
+            while (nodes[i].children)
+            {
+                const Vector c = nodes[i].bounds.center;
+                i = nodes[i].children + ((b.pos.x < c.x) | ((b.pos.y < c.y) << 1) | ((b.pos.z < c.z) << 2));
+            }
+            REQUIRE(nodes[i].com == b.pos);
+            REQUIRE(nodes[i].mass == b.mass);
+        }
+        // children-else-next (Tree::query's walk, source/bhtree.cpp:153-173) reaches
+        // every node once and ends at 0
+        size_t seen = 0;
+        uint32_t i = 0;
+        do { ++seen; i = nodes[i].children ? nodes[i].children : nodes[i].next; } while (i != 0 && seen <= nodes.size());
+        REQUIRE(seen == nodes.size());
+    }
+}
+
 int main(int argc, char** argv)
 {
     const std::string tag = argc > 1 ? argv[1] : "";

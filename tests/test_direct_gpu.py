@@ -242,12 +242,14 @@ def test_step_graph_replays_equal_eager(ctx, mode):
 
 def test_c1_plain_direct_sum_100_steps(ctx, orc):
     # The reference's CPU-runnable case: N=4096 Plummer (a=1, M=1, G=1), eps=0.01, dt=1e-3,
-    # 100 integrate steps, GPU vs the reference brute-force restatement stepping alongside.
-    # Bars: positions within 1e-5 (absolute, the cloud has radius ~1) and velocities within
-    # 1e-4 relative after 100 steps; both runs conserve energy to the same 1e-6.
+    # wrap off, 100 steps, GPU vs the reference brute-force restatement stepping alongside.
+    # Two FP32 trajectories that differ only in force rounding (the reference loop vs FP64
+    # forces rounded to FP32) end 2.4e-7 apart in position and 3.3e-7 (relative) in velocity
+    # on this input, so the bars are positions within 1e-5 (absolute, the cloud has radius
+    # ~1), velocities within 1e-5 relative, and energy drifts equal to 1e-6.
     n, steps = 4096, 100
     b = ics.plummer(n, seed=2026)
-    p = _lib.params(dt=1e-3, eps=0.01, size=10000.0, wrap=1)
+    p = _lib.params(dt=1e-3, eps=0.01, size=10000.0, wrap=0)
     ctx.upload_bodies(b)
     e0 = ctx.diagnostics(p)
     for _ in range(steps):
@@ -257,7 +259,7 @@ def test_c1_plain_direct_sum_100_steps(ctx, orc):
     cpu = b.copy()
     for _ in range(steps):
         orc.brute_force(cpu, eps=0.01)
-        orc.integrate(cpu, np.float32(1e-3), 10000.0, True)
+        orc.integrate(cpu, np.float32(1e-3), 10000.0, False)
     dx = np.abs(gpu[:, 0:3] - cpu[:, 0:3]).max()
     dv = rel_err(gpu[:, 4:7], cpu[:, 4:7])
     ec0 = orc.energy_f64(b, eps=0.01)
@@ -266,5 +268,5 @@ def test_c1_plain_direct_sum_100_steps(ctx, orc):
     drift_cpu = ((ec1[0] + ec1[1]) - (ec0[0] + ec0[1])) / abs(ec0[0] + ec0[1])
     print(f"C1 after {steps} steps: max |dx| {dx:.2e}, max rel dv {dv.max():.2e}, "
           f"energy drift gpu {drift_gpu:.3e} cpu {drift_cpu:.3e}")
-    assert dx <= 1e-5 and dv.max() <= 1e-4
+    assert dx <= 1e-5 and dv.max() <= 1e-5
     assert abs(drift_gpu - drift_cpu) <= 1e-6
