@@ -297,11 +297,13 @@ def run_bh(args) -> dict:
 def run_served(args) -> dict:
     """Poisson stream of simulation jobs (Plummer or disk+bulge, N log-uniform in [1k, 256k],
     steps uniform in [10, 1000]; direct sum below 16k bodies, Barnes-Hut theta 0.5 above),
-    device-resident, admitted against a 200 GB budget, continuously batched.  Capacity is
-    measured first with every job queued at t=0; the open stream then runs at 0.7x it."""
+    device-resident, admitted against a 200 GB budget, continuously batched; each job's
+    bodies are drawn on the device at admission (nbhip_generate), so none crosses PCIe on
+    the way in.  Capacity is measured first with every job queued at t=0; the open stream
+    then runs at 0.7x it."""
     from nbody_amd import serve
     count = 160
-    jobs = serve.make_jobs(count, seed=SEED)
+    jobs = serve.make_jobs(count, seed=SEED, device_ics=True)
     with serve.Server(hbm_budget=200e9) as srv:
         sat = srv.run(jobs)
         capacity = sat.jobs / sat.wall_s
@@ -313,7 +315,8 @@ def run_served(args) -> dict:
     return {
         "workload": f"{count} jobs: Plummer or disk+bulge, N log-uniform in [1024, 262144], steps "
                     "uniform in [10, 1000]; direct sum below 16384 bodies (batched), Barnes-Hut "
-                    "theta 0.5 leaf 16 above (a context each, 8 concurrent); 200 GB budget; "
+                    "theta 0.5 leaf 16 above (a context each, 8 concurrent); bodies drawn on "
+                    "the device at admission; 200 GB budget; "
                     f"Poisson arrivals at 0.7x the measured capacity ({rate:.1f} jobs/s)",
         "jobs": s["jobs"], "direct_jobs": n_direct, "barnes_hut_jobs": count - n_direct,
         "steps_per_s": s["steps_per_s"], "unit": "job-steps/s",

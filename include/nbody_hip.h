@@ -258,6 +258,65 @@ typedef struct nbhip_tree_node {
 int nbhip_bh_tree(nbhip_ctx* ctx, const nbhip_params* p, nbhip_tree_node* nodes, size_t cap,
                   size_t* count);
 
+/* ---- initial conditions generated on the device --------------------------------------
+ * No byte-for-byte reference counterpart: the reference draws util::disk / util::cube
+ * (source/util.cpp:14-126) from one sequential std::default_random_engine, which cannot be
+ * split across threads.  These generators draw body i's numbers from Philox4x64-10
+ * (Salmon et al., SC 2011) at counter (i, block, component, 0) under key (seed, 0), so a
+ * body depends only on (seed, i), and every distribution is sampled without rejection
+ * (inverse CDFs, Box-Muller, Gamma sums).  Arithmetic is FP64, rounded to FP32 at the end.
+ *   PLUMMER     Plummer sphere, Aarseth-Henon-Wielen 1974 (radii by inverse CDF truncated
+ *               at 0.999 of the mass; q^2 ~ Beta(3/2, 9/2) for v / v_esc); equal masses
+ *               mass / n; scale = a; centre-of-mass frame, then shifted by center / vel.
+ *   DISK_BULGE  Exponential disk (scale length `scale`, radius truncated at rcut * scale,
+ *               sech^2 height z0, a disk_frac share of the bodies and mass, circular speeds
+ *               from the model's enclosed mass at each body's spherical radius) followed
+ *               by a Plummer bulge (scale bulge_a, the rest); equal masses mass / n;
+ *               centre-of-mass frame, then shifted.
+ *   DISK        util::disk's shape (source/util.cpp:14-96): body 0 the central mass `mass`,
+ *               bodies 1..n-1 stars of star_mass at angle 2 pi i / (n-1) about `axis`
+ *               (a unit vector), radius inner + sqrt(u) (outer - inner) with outer = scale,
+ *               height ((outer - r) / outer) * thickness * r_c * N(0, 0.5), speeds
+ *               sqrt(G M / |com - x|) from the total mass and centre of mass (what the
+ *               reference's temporary-tree walk accumulates); radii compute_radius(m, 100).
+ *   CUBE        util::cube's shape (source/util.cpp:98-126): uniform in a cube of edge
+ *               `scale` about the origin (center is ignored, as the reference ignores
+ *               CubeArgs::center), velocity vel, masses `mass`, radii compute_radius. */
+#define NBHIP_IC_PLUMMER    0
+#define NBHIP_IC_DISK_BULGE 1
+#define NBHIP_IC_DISK       2
+#define NBHIP_IC_CUBE       3
+
+typedef struct nbhip_ics {
+    int      kind;
+    int      reserved;
+    uint64_t seed;
+    float    center[3];
+    float    vel[3];
+    float    axis[3];    /* DISK */
+    float    G;
+    float    mass;       /* total (PLUMMER, DISK_BULGE), central (DISK), each (CUBE) */
+    float    star_mass;  /* DISK */
+    float    scale;      /* PLUMMER a, DISK_BULGE disk scale length, DISK outer radius, CUBE edge */
+    float    inner;      /* DISK inner radius */
+    float    thickness;  /* DISK, in central-body radii */
+    float    z0;         /* DISK_BULGE */
+    float    disk_frac;  /* DISK_BULGE */
+    float    bulge_a;    /* DISK_BULGE */
+    float    rcut;       /* DISK_BULGE, in scale lengths */
+} nbhip_ics;
+
+/* Defaults of a kind: PLUMMER a = M = G = 1; DISK_BULGE scale 3, z0 0.3, disk_frac 0.8,
+ * bulge_a 0.5, rcut 10, M = G = 1; DISK and CUBE the reference's DiskArgs / CubeArgs
+ * (include/nbody/util.h:13-45: outer 250, thickness 1, central 4.1e6, stars 1; edge 500,
+ * mass 1) with G = 1 (include/nbody/constants.h:8).  Seed 0. */
+int nbhip_default_ics(int kind, nbhip_ics* out);
+
+/* Replace the context's bodies by n bodies drawn from `spec` (as nbhip_upload_bodies
+ * does, accelerations zero), without a host copy.  Stream-ordered; does not synchronise
+ * unless the body arrays grow. */
+int nbhip_generate(nbhip_ctx* ctx, const nbhip_ics* spec, size_t n);
+
 /* ---- batches: many independent direct-sum systems stepped together ------------------
  * No reference counterpart: the reference runs one simulation per Sim.  A batch lives on
  * a context (its stream, its device) and holds systems of any size, each with its own
@@ -285,6 +344,11 @@ void nbhip_batch_destroy(nbhip_batch* b);
 
 /* Add a system of n reference `Body` records; *id receives its handle.  Uploads. */
 int nbhip_batch_add(nbhip_batch* b, const void* bodies, size_t n, const nbhip_params* p, int* id);
+
+/* Add a system of n bodies drawn on the device from `spec` (nbhip_generate's generators),
+ * without a host copy.  Stream-ordered. */
+int nbhip_batch_add_generated(nbhip_batch* b, const nbhip_ics* spec, size_t n, const nbhip_params* p,
+                              int* id);
 
 /* Remove a system and free its device storage. */
 int nbhip_batch_remove(nbhip_batch* b, int id);

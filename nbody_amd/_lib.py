@@ -74,6 +74,22 @@ class BatchInfo(C.Structure):
     ]
 
 
+IC_PLUMMER = 0
+IC_DISK_BULGE = 1
+IC_DISK = 2
+IC_CUBE = 3
+
+
+class Ics(C.Structure):
+    """nbhip_ics (include/nbody_hip.h): an initial-condition generator spec."""
+    _fields_ = [("kind", C.c_int), ("reserved", C.c_int), ("seed", C.c_uint64),
+                ("center", C.c_float * 3), ("vel", C.c_float * 3), ("axis", C.c_float * 3),
+                ("G", C.c_float), ("mass", C.c_float), ("star_mass", C.c_float),
+                ("scale", C.c_float), ("inner", C.c_float), ("thickness", C.c_float),
+                ("z0", C.c_float), ("disk_frac", C.c_float), ("bulge_a", C.c_float),
+                ("rcut", C.c_float)]
+
+
 # Every symbol the header declares, with (restype, argtypes).
 _vp = C.c_void_p
 _sz = C.c_size_t
@@ -102,6 +118,9 @@ SIGNATURES = {
     "nbhip_bh_nodes": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, C.POINTER(_sz)]),
     "nbhip_bh_interactions": (C.c_int, [_vp, C.POINTER(Params), _vp, C.POINTER(C.c_uint64)]),
     "nbhip_bh_tree": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, C.POINTER(_sz)]),
+    "nbhip_default_ics": (C.c_int, [C.c_int, C.POINTER(Ics)]),
+    "nbhip_generate": (C.c_int, [_vp, C.POINTER(Ics), _sz]),
+    "nbhip_batch_add_generated": (C.c_int, [_vp, C.POINTER(Ics), _sz, C.POINTER(Params), C.POINTER(C.c_int)]),
     "nbhip_batch_create": (C.c_int, [_vp, C.POINTER(_vp)]),
     "nbhip_batch_destroy": (None, [_vp]),
     "nbhip_batch_add": (C.c_int, [_vp, _vp, _sz, C.POINTER(Params), C.POINTER(C.c_int)]),
@@ -150,6 +169,21 @@ def params(**kw) -> Params:
             raise KeyError(k)
         setattr(p, k, v)
     return p
+
+
+def ics(kind: int, **kw) -> Ics:
+    """A generator spec: the kind's defaults (nbhip_default_ics), then `kw`."""
+    s = Ics()
+    if load().nbhip_default_ics(kind, C.byref(s)) != OK:
+        raise ValueError(f"unknown initial-condition kind {kind}")
+    for k, v in kw.items():
+        if not hasattr(s, k):
+            raise KeyError(k)
+        if k in ("center", "vel", "axis"):
+            getattr(s, k)[:] = [float(x) for x in v]
+        else:
+            setattr(s, k, v)
+    return s
 
 
 def probe() -> tuple[bool, str]:
@@ -236,6 +270,10 @@ class Context:
         if b.ndim != 2 or b.shape[1] != 12:
             raise ValueError("bodies must have shape (n, 12)")
         self._check(self.L.nbhip_upload_bodies(self.h, _ptr(b), b.shape[0]))
+
+    def generate(self, spec: Ics, n: int) -> None:
+        """Replace the bodies by n drawn on the device from `spec`."""
+        self._check(self.L.nbhip_generate(self.h, C.byref(spec), n))
 
     def download_bodies(self, n: int | None = None) -> np.ndarray:
         if n is None:
@@ -362,6 +400,12 @@ class Batch:
         i = C.c_int()
         self.ctx._check(self.L.nbhip_batch_add(self.h, _ptr(b), b.shape[0], C.byref(p), C.byref(i)))
         self.sizes[i.value] = b.shape[0]
+        return i.value
+
+    def add_generated(self, spec: Ics, n: int, p: Params) -> int:
+        i = C.c_int()
+        self.ctx._check(self.L.nbhip_batch_add_generated(self.h, C.byref(spec), n, C.byref(p), C.byref(i)))
+        self.sizes[i.value] = n
         return i.value
 
     def remove(self, sid: int) -> None:

@@ -388,6 +388,34 @@ int nbhip_batch_add(nbhip_batch* b, const void* bodies, size_t n, const nbhip_pa
     });
 }
 
+int nbhip_batch_add_generated(nbhip_batch* b, const nbhip_ics* spec, size_t n, const nbhip_params* p,
+                              int* id)
+{
+    return batch_guarded(b, [&] {
+        nbhip::check_ics(spec, n);
+        if (!p || n == 0) fail(NBHIP_ERR_INVALID, "a batch system needs bodies and params");
+        if (n > (size_t(1) << 30)) fail(NBHIP_ERR_INVALID, "system of %zu bodies too large", n);
+        if (p->mode != NBHIP_MODE_N2) fail(NBHIP_ERR_UNSUPPORTED, "the batch runs direct-sum systems only");
+        if (!(p->eps >= 0.f)) fail(NBHIP_ERR_INVALID, "eps must be >= 0");
+        hipStream_t s = b->ctx->stream;
+        auto sy = std::make_unique<BatchSystem>();
+        sy->n = n;
+        sy->p = *p;
+        sy->cut = p->eps == 0.f || nbhip::ics_have_radius(*spec);
+        sy->pm[0].grow(n, 0, s);
+        sy->pm[1].grow(n, 0, s);
+        sy->vr.grow(n, 0, s);
+        sy->acc.grow(n, 0, s);
+        sy->cur = b->parity;  // as nbhip_batch_add: drawn into pm[0]
+        nbhip::generate(b->ctx, *spec, n, sy->pm[0].ptr, sy->vr.ptr, sy->acc.ptr);
+        sy->id = b->next_id++;
+        b->device_bytes += 64 * sy->pm[0].cap;
+        if (id) *id = sy->id;
+        b->systems.emplace(sy->id, std::move(sy));
+        b->dirty = true;
+    });
+}
+
 int nbhip_batch_remove(nbhip_batch* b, int id)
 {
     return batch_guarded(b, [&] {

@@ -137,6 +137,10 @@ struct nbhip_ctx {
     nbhip::DevBuf<float> fscratch;
     nbhip::DevBuf<unsigned> uscratch;
 
+    // initial conditions drawn on the device: FP64 bodies, per-block sums
+    nbhip::DevBuf<double> ic_scratch;
+    nbhip::DevBuf<double> ic_red;
+
     // Barnes-Hut: keys + permutation (double-buffered for the radix sort), bodies in key
     // order, FP64 prefix sums, per-position node counts / pre-order bases, the nodes
     nbhip::DevBuf<unsigned long long> bh_keys[2];
@@ -202,6 +206,9 @@ void bh_release(nbhip_ctx* c);
 void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm);
 void bh_reserve(nbhip_ctx* c);
 size_t bh_tree(nbhip_ctx* c, const nbhip_params* p);
+void check_ics(const nbhip_ics* s, size_t n);
+bool ics_have_radius(const nbhip_ics& s);
+void generate(nbhip_ctx* c, const nbhip_ics& s, size_t n, float4* pm, float4* vr, float4* acc);
 DirectConfig context_config(const nbhip_ctx* c);
 
 }  // namespace nbhip

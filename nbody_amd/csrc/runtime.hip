@@ -171,6 +171,8 @@ void nbhip_destroy(nbhip_ctx* c)
     c->dred.release();
     c->fscratch.release();
     c->uscratch.release();
+    c->ic_scratch.release();
+    c->ic_red.release();
     drop_graph(c);
     nbhip::bh_release(c);
     if (c->ev_start) (void)hipEventDestroy(c->ev_start);
@@ -272,6 +274,22 @@ int nbhip_upload_bodies(nbhip_ctx* c, const void* bodies, size_t n)
         nbhip::aos_to_split(c, n);
         NBHIP_CHECK(hipStreamSynchronize(c->stream));
         c->radius_known = false;
+    });
+}
+
+int nbhip_generate(nbhip_ctx* c, const nbhip_ics* spec, size_t n)
+{
+    int st = guarded(c, [&] {
+        check_ctx(c);
+        nbhip::check_ics(spec, n);
+    });
+    if (st == NBHIP_OK) st = nbhip_reserve(c, n);
+    if (st != NBHIP_OK) return st;
+    return guarded(c, [&] {
+        NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::generate(c, *spec, n, c->pm[c->cur].ptr, c->vr.ptr, c->acc.ptr);
+        c->radius_known = true;
+        c->any_radius = nbhip::ics_have_radius(*spec);
     });
 }
 

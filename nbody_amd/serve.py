@@ -8,6 +8,8 @@ admitted job's state on the device until its last step, and runs:
     concurrently with the batch and with each other.
 Admission is first come, first served, against a device-memory budget (bytes per body
 estimated per mode) and the free contexts of the pool; a job that does not fit waits.
+A job brings its bodies either as host records (uploaded at admission) or as a generator
+spec (nbhip_ics), in which case its bodies are drawn on the device and never cross PCIe.
 Each resource advances in chunks of at most `chunk` steps, never past the step count of
 any job it holds, and is polled without blocking (nbhip_query), so a resource that
 finishes early is refilled while the others still run.
@@ -28,17 +30,19 @@ from . import _lib
 
 # Device bytes per body, for admission: state arrays (pm x2, vr, acc: 64 B) for a
 # batched job; for a Barnes-Hut job also keys, sorted copies, prefix sums and up to 2N
-# nodes (about 170 B more).
-BYTES_PER_BODY = {_lib.MODE_N2: 64, _lib.MODE_NLOGN: 240}
+# nodes (about 170 B more) and the FP64 scratch of bodies drawn on the device (48 B).
+BYTES_PER_BODY = {_lib.MODE_N2: 64, _lib.MODE_NLOGN: 288}
 
 
 @dataclasses.dataclass
 class Job:
-    bodies: np.ndarray              # reference Body records, float32 (n, 12)
+    bodies: np.ndarray | None       # reference Body records, float32 (n, 12), or None ...
     steps: int
     params: _lib.Params
     arrival: float = 0.0            # seconds after the start of the run
     tag: str = ""
+    ics: _lib.Ics | None = None     # ... and the bodies are drawn on the device from this
+    n_bodies: int = 0               # with this many bodies
     # filled in by the server
     id: int = -1
     admitted: float = float("nan")
@@ -48,7 +52,7 @@ class Job:
 
     @property
     def n(self) -> int:
-        return self.bodies.shape[0]
+        return self.bodies.shape[0] if self.bodies is not None else self.n_bodies
 
     @property
     def mode(self) -> int:
@@ -184,14 +188,18 @@ class Server:
                 if job.mode == _lib.MODE_N2:
                     if batch_inflight:
                         break        # joins at the next chunk boundary
-                    sid = self.batch.add(job.bodies, job.params)
+                    sid = (self.batch.add(job.bodies, job.params) if job.bodies is not None
+                           else self.batch.add_generated(job.ics, job.n, job.params))
                     direct[sid] = (job, job.steps)
                     job.interactions_per_step = float(job.n) ** 2
                 else:
                     slot = next((s for s in self.slots if s.job is None), None)
                     if slot is None:
                         break
-                    slot.ctx.upload_bodies(job.bodies)
+                    if job.bodies is not None:
+                        slot.ctx.upload_bodies(job.bodies)
+                    else:
+                        slot.ctx.generate(job.ics, job.n)
                     _, inter = slot.ctx.bh_interactions(job.params)
                     job.interactions_per_step = float(inter)
                     slot.job, slot.remaining, slot.inflight = job, job.steps, 0
@@ -239,10 +247,12 @@ class Server:
 
 def make_jobs(count: int, seed: int, n_range=(1024, 262144), steps_range=(10, 1000),
               direct_below: int = 16384, eps: float = 0.01, dt: float = 1e-3,
-              theta: float = 0.5, leaf_size: int = 16, size: float = 128.0) -> list[Job]:
+              theta: float = 0.5, leaf_size: int = 16, size: float = 128.0,
+              device_ics: bool = False) -> list[Job]:
     """The served workload: Plummer or disk+bulge systems, N log-uniform, steps uniform;
-    direct sum below `direct_below` bodies, Barnes-Hut above.  Arrival times are zero;
-    set them with poisson_arrivals()."""
+    direct sum below `direct_below` bodies, Barnes-Hut above.  Bodies are host arrays
+    (numpy's generators, nbody_amd.ics) or, with device_ics, generator specs drawn on the
+    device at admission.  Arrival times are zero; set them with poisson_arrivals()."""
     from . import ics
     rng = np.random.default_rng(seed)
     out = []
@@ -250,13 +260,20 @@ def make_jobs(count: int, seed: int, n_range=(1024, 262144), steps_range=(10, 10
         n = int(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
         steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
         disk = bool(rng.integers(0, 2))
-        bodies = ics.disk_bulge(n, seed=seed * 7919 + k) if disk else ics.plummer(n, seed=seed * 7919 + k)
+        job_seed = seed * 7919 + k
+        spec = None
+        if device_ics:
+            spec = _lib.ics(_lib.IC_DISK_BULGE if disk else _lib.IC_PLUMMER, seed=job_seed)
+            bodies = None
+        else:
+            bodies = ics.disk_bulge(n, seed=job_seed) if disk else ics.plummer(n, seed=job_seed)
         if n < direct_below:
             p = _lib.params(dt=dt, eps=eps, size=size, wrap=1)
         else:
             p = _lib.params(mode=_lib.MODE_NLOGN, dt=dt, eps=eps, size=size, wrap=1, theta=theta,
                             leaf_size=leaf_size)
-        out.append(Job(bodies=bodies, steps=steps, params=p, tag="disk" if disk else "plummer"))
+        out.append(Job(bodies=bodies, steps=steps, params=p, tag="disk" if disk else "plummer",
+                       ics=spec, n_bodies=n))
     return out
 
 

@@ -21,6 +21,9 @@
  *   orc_stable_sort      stable ascending sort of keys, ties by index (std::stable_sort)
  *   orc_direct_f64       FP64 direct sum (no reference counterpart: the accuracy yardstick)
  *   orc_energy_f64       FP64 kinetic/potential energy, momentum, angular momentum
+ *   orc_philox4x64       Philox4x64-10 counter-based generator (Salmon et al., SC 2011)
+ *   orc_generate         initial conditions as the library's nbhip_generate defines them
+ *                        (Plummer, disk + bulge, util::disk-style, util::cube-style)
  *
  * Bodies are the reference's 48-byte AoS `nbody::Body` (include/nbody/body.h:6-19):
  *   float pos[3], radius, vel[3], mass, acc[3], pad   -> 12 floats per body.
@@ -632,3 +635,239 @@ void orc_bh_accelerate_f64com(float* bodies, size_t n, const orc_node* nodes, si
 }
 
 size_t orc_node_size(void) { return sizeof(orc_node); }
+
+/* ------------------------------------------------------------------------------- */
+/* Initial conditions (no reference counterpart: the reference's util::disk / cube  */
+/* draw from one sequential std::default_random_engine, source/util.cpp:14-126).    */
+/* Restates the generators the library's nbhip_generate documents, from their       */
+/* definitions: Philox4x64-10 (Salmon, Moraes, Dror & Shaw, SC 2011) at counter     */
+/* (i, block, component, 0), key (seed, 0); FP64 arithmetic, FP32 at the end.        */
+/* ------------------------------------------------------------------------------- */
+typedef struct {
+    int kind, reserved;
+    uint64_t seed;
+    float center[3], vel[3], axis[3];
+    float G, mass, star_mass, scale, inner, thickness, z0, disk_frac, bulge_a, rcut;
+} orc_ics;
+
+void orc_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4])
+{
+    uint64_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B97F4A7C15ull; /* golden ratio */
+            k1 += 0xBB67AE8584CAA73Bull; /* sqrt(3) - 1   */
+        }
+        const unsigned __int128 p0 = (unsigned __int128)0xD2E7470EE14C6C93ull * c0;
+        const unsigned __int128 p1 = (unsigned __int128)0xCA5A826395121157ull * c2;
+        const uint64_t n0 = (uint64_t)(p1 >> 64) ^ c1 ^ k0;
+        const uint64_t n2 = (uint64_t)(p0 >> 64) ^ c3 ^ k1;
+        c1 = (uint64_t)p1;
+        c3 = (uint64_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static void orc_draw(uint64_t seed, uint64_t i, uint64_t block, uint64_t tag, uint64_t out[4])
+{
+    const uint64_t ctr[4] = { i, block, tag, 0 }, key[2] = { seed, 0 };
+    orc_philox4x64(ctr, key, out);
+}
+
+static double orc_u_co(uint64_t x) { return (double)(x >> 11) * 0x1.0p-53; }
+static double orc_u_oc(uint64_t x) { return (double)((x >> 11) + 1) * 0x1.0p-53; }
+static double orc_u_oo(uint64_t x) { return ((double)(x >> 11) + 0.5) * 0x1.0p-53; }
+
+#define ORC_TWO_PI 6.283185307179586476925286766559
+#define ORC_PI 3.1415926535897932384626433832795
+#define ORC_PLUMMER_CUT 0.999
+
+/* Plummer sphere of unit a, M, G (Aarseth, Henon & Wielen 1974): r from the enclosed
+ * mass fraction X = r^3/(1+r^2)^{3/2} (X uniform on (0, 0.999]); isotropic direction;
+ * v = q v_esc with v_esc = sqrt(2) (1+r^2)^{-1/4} and q^2 ~ Beta(3/2, 9/2), drawn as
+ * G1/(G1+G2), G1 = Exp + Z1^2/2 ~ Gamma(3/2), G2 = 4 Exp + Z2^2/2 ~ Gamma(9/2). */
+static void orc_plummer_unit(uint64_t seed, uint64_t tag, uint64_t i, double p[3], double v[3])
+{
+    uint64_t b0[4], b1[4], b2[4];
+    orc_draw(seed, i, 0, tag, b0);
+    orc_draw(seed, i, 1, tag, b1);
+    orc_draw(seed, i, 2, tag, b2);
+    const double X = ORC_PLUMMER_CUT * orc_u_oc(b0[0]);
+    const double r = 1.0 / sqrt(pow(X, -2.0 / 3.0) - 1.0);
+    const double ct = 2.0 * orc_u_co(b0[1]) - 1.0, st = sqrt(fmax(0.0, 1.0 - ct * ct));
+    const double ph = ORC_TWO_PI * orc_u_co(b0[2]);
+    p[0] = r * st * cos(ph);
+    p[1] = r * st * sin(ph);
+    p[2] = r * ct;
+    const double rho = sqrt(-2.0 * log(orc_u_oc(b2[0]))), th = ORC_TWO_PI * orc_u_co(b2[1]);
+    const double z1 = rho * cos(th), z2 = rho * sin(th);
+    const double g1 = -log(orc_u_oc(b0[3])) + 0.5 * z1 * z1;
+    const double g2 = -log(orc_u_oc(b1[0])) - log(orc_u_oc(b1[1])) - log(orc_u_oc(b1[2])) -
+                      log(orc_u_oc(b1[3])) + 0.5 * z2 * z2;
+    const double q = sqrt(g1 / (g1 + g2));
+    const double vv = q * sqrt(2.0) * pow(1.0 + r * r, -0.25);
+    const double vct = 2.0 * orc_u_co(b2[2]) - 1.0, vst = sqrt(fmax(0.0, 1.0 - vct * vct));
+    const double vph = ORC_TWO_PI * orc_u_co(b2[3]);
+    v[0] = vv * vst * cos(vph);
+    v[1] = vv * vst * sin(vph);
+    v[2] = vv * vct;
+}
+
+static double orc_disk_cdf(double s) { return 1.0 - (1.0 + s) * exp(-s); }
+
+static double orc_compute_radius(double m) /* source/util.cpp:9-12, star density 100 */
+{
+    return pow((3.0 * ORC_PI * m) / (4.0 * 100.0), 1.0 / 3.0);
+}
+
+typedef struct {
+    const orc_ics* s;
+    size_t n, n_disk;
+    double m_each, fmax;
+    double* w; /* 6 doubles per body */
+} orc_ics_arg;
+
+static void orc_ics_mass(const orc_ics_arg* a, size_t i, double* m, double* rad)
+{
+    if (a->s->kind == 2) {
+        *m = i == 0 ? (double)a->s->mass : (double)a->s->star_mass;
+        *rad = orc_compute_radius(*m);
+    } else if (a->s->kind == 3) {
+        *m = a->m_each;
+        *rad = orc_compute_radius(*m);
+    } else {
+        *m = a->m_each;
+        *rad = 0.0;
+    }
+}
+
+static void orc_ics_block(void* arg, size_t begin, size_t end)
+{
+    const orc_ics_arg* a = (const orc_ics_arg*)arg;
+    const orc_ics* s = a->s;
+    for (size_t i = begin; i < end; ++i) {
+        double p[3] = { 0, 0, 0 }, v[3] = { 0, 0, 0 };
+        if (s->kind == 0) {
+            orc_plummer_unit(s->seed, 1, i, p, v);
+            const double sv = sqrt((double)s->G * (double)s->mass / (double)s->scale);
+            for (int k = 0; k < 3; ++k) { p[k] *= (double)s->scale; v[k] *= sv; }
+        } else if (s->kind == 1) {
+            const double rd = s->scale, ab = s->bulge_a, G = s->G;
+            const double md = a->m_each * (double)a->n_disk, mb = a->m_each * (double)(a->n - a->n_disk);
+            if (i < a->n_disk) {
+                uint64_t b[4];
+                orc_draw(s->seed, i, 0, 2, b);
+                /* R: the truncated Gamma(2) (exponential disk) CDF inverted by 64 bisections */
+                const double y = orc_u_co(b[0]) * a->fmax;
+                double lo = 0.0, hi = (double)s->rcut;
+                for (int it = 0; it < 64; ++it) {
+                    const double mid = 0.5 * (lo + hi);
+                    if (orc_disk_cdf(mid) < y) lo = mid; else hi = mid;
+                }
+                const double R = rd * (0.5 * (lo + hi));
+                const double ph = ORC_TWO_PI * orc_u_co(b[1]);
+                const double z = (double)s->z0 * atanh(2.0 * orc_u_oo(b[2]) - 1.0); /* sech^2 */
+                p[0] = R * cos(ph); p[1] = R * sin(ph); p[2] = z;
+                const double r = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+                double menc = md * orc_disk_cdf(fmin(r / rd, (double)s->rcut)) / a->fmax;
+                if (mb > 0.0) {
+                    const double x = r / ab;
+                    menc += mb * fmin(1.0, x * x * x / pow(1.0 + x * x, 1.5) / ORC_PLUMMER_CUT);
+                }
+                const double vc = r > 0.0 ? sqrt(G * menc / r) : 0.0;
+                v[0] = -vc * sin(ph); v[1] = vc * cos(ph);
+            } else {
+                orc_plummer_unit(s->seed, 3, i - a->n_disk, p, v);
+                const double sv = sqrt(G * mb / ab);
+                for (int k = 0; k < 3; ++k) { p[k] *= ab; v[k] *= sv; }
+            }
+        } else if (s->kind == 2) {
+            /* util::disk's construction (source/util.cpp:22-74) */
+            const double rc = orc_compute_radius(s->mass);
+            const double outer = fmax((double)s->scale, rc), inner = fmax((double)s->inner, rc);
+            const double ax = s->axis[0], ay = s->axis[1], az = s->axis[2];
+            if (i == 0) {
+                for (int k = 0; k < 3; ++k) p[k] = s->center[k];
+            } else {
+                uint64_t b[4];
+                orc_draw(s->seed, i, 0, 4, b);
+                const double angle = ORC_TWO_PI * ((double)i / (double)(a->n - 1));
+                const double dist = inner + sqrt(orc_u_co(b[0])) * (outer - inner);
+                const double gauss = 0.5 * sqrt(-2.0 * log(orc_u_oc(b[1]))) * cos(ORC_TWO_PI * orc_u_co(b[2]));
+                const double disp = ((outer - dist) / outer) * ((double)s->thickness * rc) * gauss;
+                const double c0[3] = { copysign(az, ax), copysign(az, ay), -copysign(ax, az) - copysign(ay, az) };
+                const double c1[3] = { ay * c0[2] - az * c0[1], az * c0[0] - ax * c0[2], ax * c0[1] - ay * c0[0] };
+                const double sa = sin(angle), ca = cos(angle);
+                double s0[3] = { sa * c0[0] + ca * c1[0], sa * c0[1] + ca * c1[1], sa * c0[2] + ca * c1[2] };
+                const double l0 = sqrt(s0[0] * s0[0] + s0[1] * s0[1] + s0[2] * s0[2]);
+                for (int k = 0; k < 3; ++k) s0[k] /= l0;
+                const double s1[3] = { ay * s0[2] - az * s0[1], az * s0[0] - ax * s0[2], ax * s0[1] - ay * s0[0] };
+                const double l1 = sqrt(s1[0] * s1[0] + s1[1] * s1[1] + s1[2] * s1[2]);
+                const double axis[3] = { ax, ay, az };
+                for (int k = 0; k < 3; ++k) {
+                    p[k] = (double)s->center[k] + s0[k] * dist + axis[k] * disp;
+                    v[k] = s1[k] / l1;
+                }
+            }
+        } else {
+            uint64_t b[4];
+            orc_draw(s->seed, i, 0, 5, b);
+            for (int k = 0; k < 3; ++k) p[k] = (orc_u_co(b[k]) - 0.5) * (double)s->scale;
+        }
+        for (int k = 0; k < 3; ++k) { a->w[6 * i + k] = p[k]; a->w[6 * i + 3 + k] = v[k]; }
+    }
+}
+
+/* n bodies of spec s as reference Body records (12 floats each).  Returns 0 on success. */
+int orc_generate(const orc_ics* s, size_t n, float* bodies, int nthreads)
+{
+    if (n == 0) return 0;
+    orc_ics_arg a;
+    a.s = s;
+    a.n = n;
+    a.n_disk = s->kind == 1 ? (size_t)llround((double)n * (double)s->disk_frac) : 0;
+    if (a.n_disk > n) a.n_disk = n;
+    a.m_each = (s->kind == 0 || s->kind == 1) ? (double)s->mass / (double)n : (double)s->mass;
+    a.fmax = 1.0 - (1.0 + (double)s->rcut) * exp(-(double)s->rcut);
+    a.w = (double*)malloc(6 * n * sizeof(double));
+    if (!a.w) return 1;
+    orc_parallel_blocks(n, nthreads, orc_ics_block, &a);
+    double tot[7] = { 0, 0, 0, 0, 0, 0, 0 };
+    for (size_t i = 0; i < n; ++i) {
+        double m, rad;
+        orc_ics_mass(&a, i, &m, &rad);
+        tot[0] += m;
+        for (int k = 0; k < 6; ++k) tot[1 + k] += m * a.w[6 * i + k];
+    }
+    const double M = tot[0];
+    for (size_t i = 0; i < n; ++i) {
+        double m, rad, p[3], v[3];
+        const double* w = a.w + 6 * i;
+        orc_ics_mass(&a, i, &m, &rad);
+        if (s->kind == 0 || s->kind == 1) {
+            for (int k = 0; k < 3; ++k) {
+                p[k] = (double)s->center[k] + (w[k] - tot[1 + k] / M);
+                v[k] = (double)s->vel[k] + (w[3 + k] - tot[4 + k] / M);
+            }
+        } else if (s->kind == 2) {
+            const double d[3] = { tot[1] / M - w[0], tot[2] / M - w[1], tot[3] / M - w[2] };
+            const double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            const int star = i > 0 && dist > 0.0;
+            const double speed = star ? sqrt((double)s->G * M / dist) : 0.0;
+            for (int k = 0; k < 3; ++k) {
+                p[k] = w[k];
+                v[k] = (double)s->vel[k] + (star ? w[3 + k] * speed : 0.0);
+            }
+        } else {
+            for (int k = 0; k < 3; ++k) { p[k] = w[k]; v[k] = s->vel[k]; }
+        }
+        float* b = bodies + BF * i;
+        b[0] = (float)p[0]; b[1] = (float)p[1]; b[2] = (float)p[2]; b[3] = (float)rad;
+        b[4] = (float)v[0]; b[5] = (float)v[1]; b[6] = (float)v[2]; b[7] = (float)m;
+        b[8] = b[9] = b[10] = b[11] = 0.f;
+    }
+    free(a.w);
+    return 0;
+}

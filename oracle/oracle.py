@@ -78,6 +78,9 @@ def lib():
     L.orc_bh_accelerate_f64com.argtypes = [_f32p, sz, _nodep, sz, C.c_float, C.c_double,
                                            C.c_double, C.c_void_p, C.c_int]
     L.orc_node_size.restype = sz
+    L.orc_philox4x64.argtypes = [_u64p, _u64p, _u64p]
+    L.orc_generate.argtypes = [C.c_void_p, sz, _f32p, C.c_int]
+    L.orc_generate.restype = C.c_int
     assert L.orc_node_size() == 48
     _lib = L
     return L
@@ -285,3 +288,33 @@ def ref_cube(n: int) -> np.ndarray:
     b = np.zeros((n, 12), np.float32)
     ref().ref_cube(b, n)
     return b
+
+
+# ---- initial conditions ------------------------------------------------------------------
+
+class Ics(C.Structure):
+    """The generator spec, field for field the layout of the library's nbhip_ics."""
+    _fields_ = [("kind", C.c_int), ("reserved", C.c_int), ("seed", C.c_uint64),
+                ("center", C.c_float * 3), ("vel", C.c_float * 3), ("axis", C.c_float * 3),
+                ("G", C.c_float), ("mass", C.c_float), ("star_mass", C.c_float),
+                ("scale", C.c_float), ("inner", C.c_float), ("thickness", C.c_float),
+                ("z0", C.c_float), ("disk_frac", C.c_float), ("bulge_a", C.c_float),
+                ("rcut", C.c_float)]
+
+
+def philox4x64(counter, key) -> np.ndarray:
+    c = np.ascontiguousarray(counter, np.uint64)
+    k = np.ascontiguousarray(key, np.uint64)
+    out = np.zeros(4, np.uint64)
+    lib().orc_philox4x64(c, k, out)
+    return out
+
+
+def generate(spec, n: int, threads=0) -> np.ndarray:
+    """n reference Body records drawn from `spec` (any ctypes struct with Ics's layout)."""
+    s = Ics.from_buffer_copy(bytes(spec))
+    out = np.zeros((n, 12), np.float32)
+    if lib().orc_generate(C.byref(s), n, out, _t(threads)) != 0:
+        raise MemoryError("orc_generate")
+    return out
+

@@ -27,3 +27,16 @@ def test_poisson_arrivals():
     gaps = np.diff(t)
     assert abs(gaps.mean() - 1 / 50.0) < 0.1 / 50.0     # mean inter-arrival 1/rate
     assert abs(gaps.std() / gaps.mean() - 1.0) < 0.1     # exponential: cv = 1
+
+
+def test_device_ics_jobs_carry_specs(orc):
+    jobs = serve.make_jobs(12, seed=4, n_range=(1024, 60000), device_ics=True)
+    host = serve.make_jobs(12, seed=4, n_range=(1024, 60000))
+    for j, h in zip(jobs, host):
+        assert j.bodies is None and j.n == h.n and j.mode == h.mode and j.steps == h.steps
+        assert j.ics.kind == (_lib.IC_DISK_BULGE if j.tag == "disk" else _lib.IC_PLUMMER)
+        assert j.device_bytes() == h.device_bytes()
+    for j in jobs[:4]:
+        b = orc.generate(j.ics, j.n)   # what the device draws, within the root cube
+        assert np.abs(b[:, 0:3]).max() < j.params.size / 2
+

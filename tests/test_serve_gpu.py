@@ -14,25 +14,30 @@ pytestmark = pytest.mark.gpu
 
 def alone(job):
     with _lib.Context(0) as ctx:
+        bodies = job.bodies
+        if bodies is None:  # drawn on the device: the same draw, by itself
+            ctx.generate(job.ics, job.n)
+            bodies = ctx.download_bodies()
         if job.mode == _lib.MODE_N2:
             with _lib.Batch(ctx) as b:
-                sid = b.add(job.bodies, job.params)
+                sid = b.add(bodies, job.params)
                 b.step(job.steps)
                 return b.download(sid)
-        ctx.upload_bodies(job.bodies)
+        ctx.upload_bodies(bodies)
         ctx.step(job.params, job.steps)
         return ctx.download_bodies()
 
 
-def small_jobs(count=14, seed=5):
+def small_jobs(count=14, seed=5, device_ics=False):
     jobs = serve.make_jobs(count, seed=seed, n_range=(1024, 60000), steps_range=(3, 40),
-                           direct_below=16384)
+                           direct_below=16384, device_ics=device_ics)
     serve.poisson_arrivals(jobs, rate=200.0, seed=seed)
     return jobs
 
 
-def test_every_job_same_alone_and_in_the_mix():
-    jobs = small_jobs()
+@pytest.mark.parametrize("device_ics", [False, True])
+def test_every_job_same_alone_and_in_the_mix(device_ics):
+    jobs = small_jobs(device_ics=device_ics)
     assert any(j.mode == _lib.MODE_N2 for j in jobs) and any(j.mode == _lib.MODE_NLOGN for j in jobs)
     with serve.Server(chunk=8, bh_contexts=3, keep_results=True) as srv:
         rep = srv.run(jobs)
