@@ -162,6 +162,22 @@ int nbhip_synchronize(nbhip_ctx* ctx);
  * (hipStreamQuery).  Lets a scheduler keep several contexts busy at once. */
 int nbhip_query(nbhip_ctx* ctx, int* idle);
 
+/* ---- predicted readback (reference Readback mask on a dispatch, source/gpu.h:52-70,
+ *      recorded by record_readback_split, source/gpu.cpp:678-712, and requested by
+ *      GpuSolverSplit::wanted_readback, source/solvers/gpu_solver_split.h:120-132) ---- */
+
+/* Enqueue, behind the work already on the stream, the interleave of every body into
+ * reference Body records and their copy into the context's pinned host staging memory.
+ * Does not wait.  A caller that expects to read after a dispatch issues this right after
+ * it, so the copy overlaps the host's own work instead of starting when it reads. */
+int nbhip_readback(nbhip_ctx* ctx);
+
+/* min(n, count) Body records as of the last nbhip_readback, if no call since then has
+ * changed the bodies (a dispatch, upload, generate or reserve): waits for that copy only
+ * and copies from the staging memory; otherwise downloads afresh, as
+ * nbhip_download_bodies.  *staged (may be NULL) is 1 when the staged copy was used. */
+int nbhip_collect_bodies(nbhip_ctx* ctx, void* bodies, size_t n, int* staged);
+
 /* ---- autotuner for the single-system direct sum ---------------------------------------- */
 
 /* One timed launch shape: P bodies per lane, W waves per block, j-split, ms per force pass. */

@@ -111,6 +111,8 @@ SIGNATURES = {
     "nbhip_step_graph": (C.c_int, [_vp, C.POINTER(Params), C.c_int]),
     "nbhip_synchronize": (C.c_int, [_vp]),
     "nbhip_query": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "nbhip_readback": (C.c_int, [_vp]),
+    "nbhip_collect_bodies": (C.c_int, [_vp, _vp, _sz, C.POINTER(C.c_int)]),
     "nbhip_tune": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, C.POINTER(_sz), C.POINTER(TuneEntry)]),
     "nbhip_diagnostics": (C.c_int, [_vp, C.POINTER(Params), _vp]),
     "nbhip_force_error_f64": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, _vp, _vp]),
@@ -270,6 +272,17 @@ class Context:
         if b.ndim != 2 or b.shape[1] != 12:
             raise ValueError("bodies must have shape (n, 12)")
         self._check(self.L.nbhip_upload_bodies(self.h, _ptr(b), b.shape[0]))
+
+    def readback(self) -> None:
+        """Start copying the bodies to pinned host memory behind the enqueued work."""
+        self._check(self.L.nbhip_readback(self.h))
+
+    def collect_bodies(self) -> tuple[np.ndarray, bool]:
+        """The bodies (from the last readback when still current) and whether it was used."""
+        b = np.zeros((self.count, 12), np.float32)
+        st = C.c_int()
+        self._check(self.L.nbhip_collect_bodies(self.h, _ptr(b), b.shape[0], C.byref(st)))
+        return b, bool(st.value)
 
     def generate(self, spec: Ics, n: int) -> None:
         """Replace the bodies by n drawn on the device from `spec`."""

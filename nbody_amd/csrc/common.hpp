@@ -176,6 +176,16 @@ struct nbhip_ctx {
     long launches = 0;
     long launches_at_start = 0;
 
+    // predicted readback (nbhip_readback / nbhip_collect_bodies): Body records staged in
+    // pinned host memory, valid while no call has changed the bodies since
+    unsigned long long body_version = 0;  // bumped by every call that changes the bodies
+    float* staging = nullptr;             // 12 floats per body, hipHostMalloc
+    size_t staging_cap = 0;               // bodies
+    hipEvent_t ev_staged = nullptr;
+    bool staged = false;
+    unsigned long long staged_version = 0;
+    size_t staged_n = 0;
+
     std::string err;
 };
 

@@ -144,6 +144,7 @@ int nbhip_create(int device, nbhip_ctx** out)
         NBHIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         NBHIP_CHECK(hipEventCreate(&c->ev_start));
         NBHIP_CHECK(hipEventCreate(&c->ev_stop));
+        NBHIP_CHECK(hipEventCreateWithFlags(&c->ev_staged, hipEventDisableTiming));
         *out = c;
         return NBHIP_OK;
     } catch (const Error& e) {
@@ -175,6 +176,8 @@ void nbhip_destroy(nbhip_ctx* c)
     c->ic_red.release();
     drop_graph(c);
     nbhip::bh_release(c);
+    if (c->staging) (void)hipHostFree(c->staging);
+    if (c->ev_staged) (void)hipEventDestroy(c->ev_staged);
     if (c->ev_start) (void)hipEventDestroy(c->ev_start);
     if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -190,6 +193,7 @@ int nbhip_reserve(nbhip_ctx* c, size_t n)
 {
     return guarded(c, [&] {
         check_ctx(c);
+        ++c->body_version;  // the staged readback no longer matches
         if (n > (size_t(1) << 31)) fail(NBHIP_ERR_INVALID, "body count %zu too large", n);
         NBHIP_CHECK(hipSetDevice(c->device));
         const size_t keep = c->count;
@@ -211,6 +215,7 @@ int nbhip_upload(nbhip_ctx* c, const float* pos_mass, const float* vel_radius, c
 {
     return guarded(c, [&] {
         check_ctx(c);
+        ++c->body_version;  // the staged readback no longer matches
         if (offset > c->count || n > c->count - offset)
             fail(NBHIP_ERR_INVALID, "upload range [%zu, %zu) beyond body count %zu", offset,
                  offset + n, c->count);
@@ -293,6 +298,55 @@ int nbhip_generate(nbhip_ctx* c, const nbhip_ics* spec, size_t n)
     });
 }
 
+int nbhip_readback(nbhip_ctx* c)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        NBHIP_CHECK(hipSetDevice(c->device));
+        const size_t n = c->count;
+        if (n > c->staging_cap) {
+            // the old staging memory may still be the target of an earlier copy
+            NBHIP_CHECK(hipStreamSynchronize(c->stream));
+            if (c->staging) NBHIP_CHECK(hipHostFree(c->staging));
+            c->staging = nullptr;
+            c->staging_cap = 0;
+            c->staged = false;
+            size_t want = 1024;
+            while (want < n) want *= 2;
+            void* p = nullptr;
+            if (hipHostMalloc(&p, want * 48, hipHostMallocDefault) != hipSuccess)
+                fail(NBHIP_ERR_NOMEM, "pinned staging of %zu bytes failed", want * 48);
+            c->staging = static_cast<float*>(p);
+            c->staging_cap = want;
+        }
+        if (n) {
+            c->aos.grow(12 * n, 0, c->stream);
+            nbhip::split_to_aos(c, n);
+            NBHIP_CHECK(hipMemcpyAsync(c->staging, c->aos.ptr, 48 * n, hipMemcpyDeviceToHost, c->stream));
+        }
+        NBHIP_CHECK(hipEventRecord(c->ev_staged, c->stream));
+        c->staged = true;
+        c->staged_version = c->body_version;
+        c->staged_n = n;
+    });
+}
+
+int nbhip_collect_bodies(nbhip_ctx* c, void* bodies, size_t n, int* staged)
+{
+    if (staged) *staged = 0;
+    if (c && c->staged && c->staged_version == c->body_version && c->staged_n == c->count) {
+        return guarded(c, [&] {
+            NBHIP_CHECK(hipSetDevice(c->device));
+            const size_t k = n < c->count ? n : c->count;
+            if (k && !bodies) fail(NBHIP_ERR_INVALID, "null bodies");
+            NBHIP_CHECK(hipEventSynchronize(c->ev_staged));
+            if (k) std::memcpy(bodies, c->staging, 48 * k);
+            if (staged) *staged = 1;
+        });
+    }
+    return nbhip_download_bodies(c, bodies, n);
+}
+
 int nbhip_download_bodies(nbhip_ctx* c, void* bodies, size_t n)
 {
     return guarded(c, [&] {
@@ -312,6 +366,7 @@ int nbhip_accelerate(nbhip_ctx* c, const nbhip_params* p)
 {
     return guarded(c, [&] {
         check_ctx(c);
+        ++c->body_version;  // the staged readback no longer matches
         check_params(p);
         if (c->count == 0) return;  // nothing to do (source/solvers/gpu_solver_split.h:81-82)
         NBHIP_CHECK(hipSetDevice(c->device));
@@ -326,6 +381,7 @@ int nbhip_integrate(nbhip_ctx* c, const nbhip_params* p)
 {
     return guarded(c, [&] {
         check_ctx(c);
+        ++c->body_version;  // the staged readback no longer matches
         check_params(p);
         if (c->count == 0) return;
         NBHIP_CHECK(hipSetDevice(c->device));
@@ -403,6 +459,7 @@ int nbhip_step(nbhip_ctx* c, const nbhip_params* p, int steps)
 {
     return guarded(c, [&] {
         check_ctx(c);
+        ++c->body_version;  // the staged readback no longer matches
         check_params(p);
         if (steps < 0) fail(NBHIP_ERR_INVALID, "negative step count");
         if (c->count == 0) return;
@@ -415,6 +472,7 @@ int nbhip_step_graph(nbhip_ctx* c, const nbhip_params* p, int steps)
 {
     return guarded(c, [&] {
         check_ctx(c);
+        ++c->body_version;  // the staged readback no longer matches
         check_params(p);
         if (steps < 0) fail(NBHIP_ERR_INVALID, "negative step count");
         if (c->count == 0 || steps == 0) return;
@@ -437,6 +495,7 @@ int nbhip_tune(nbhip_ctx* c, const nbhip_params* p, nbhip_tune_entry* out, size_
 {
     return guarded(c, [&] {
         check_ctx(c);
+        ++c->body_version;  // the staged readback no longer matches
         check_params(p);
         if (count) *count = 0;
         if (c->count == 0) return;
@@ -571,6 +630,7 @@ int nbhip_bh_interactions(nbhip_ctx* c, const nbhip_params* p, uint32_t* per_bod
 {
     return guarded(c, [&] {
         check_ctx(c);
+        ++c->body_version;  // the staged readback no longer matches
         check_params(p);
         if (total) *total = 0;
         if (c->count == 0) return;

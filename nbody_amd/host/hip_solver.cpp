@@ -103,6 +103,17 @@ namespace nbody
         _dev->check(nbhip_step(_dev->ctx(), &p, 1), "step");
         _device_dirty = true;
         dispatched_tree();
+        predicted_readback();
+    }
+
+    void HipSolver::predicted_readback()
+    {
+        // A caller that read after the last step will read after this one too: start the
+        // copy now, behind the step (source/solvers/gpu_solver_split.h:54-70, 120-132;
+        // accelerate() and integrate() fetch nothing, as there).
+        if (_materialize_expected)
+            _dev->check(nbhip_readback(_dev->ctx()), "readback");
+        _materialize_expected = false;
     }
 
     void HipSolver::dispatched_tree()
@@ -141,12 +152,15 @@ namespace nbody
 
     void HipSolver::materialize() const
     {
+        // Set even when there is nothing to fetch: what matters is that this caller reads.
+        _materialize_expected = true;
         // A pending host write wins: downloading now would overwrite it.
         if (!_device_dirty || _host_dirty)
             return;
-        // Only what both sides hold: the device count can exceed a shrunk body array.
+        // Only what both sides hold: the device count can exceed a shrunk body array.  The
+        // copy started after the dispatch is used when it is still current.
         std::vector<Body>& b = _state->bodies;
-        _dev->check(nbhip_download_bodies(_dev->ctx(), b.data(), b.size()), "download bodies");
+        _dev->check(nbhip_collect_bodies(_dev->ctx(), b.data(), b.size(), nullptr), "collect bodies");
         _device_dirty = false;
     }
 }

@@ -5,8 +5,9 @@
 // converts at the Solver sync points exactly as the reference's GpuSolverSplit does
 // (source/solvers/gpu_solver_split.h:18-237): adopt() and ingest() mark the host copy
 // authoritative and the next dispatch uploads it; a dispatch marks the device
-// authoritative and state() materializes it back into State::bodies.  Stepping never
-// reads back, so a caller that steps without reading moves no data over PCIe.
+// authoritative and state() materializes it back into State::bodies.  A caller that steps
+// without reading moves no data over PCIe; one that reads after every dispatch gets the
+// copy started right behind the dispatch (the reference's wanted_readback prediction).
 #include <memory>
 #include "nbody/bhtree.h"
 #include "solver.h"
@@ -44,6 +45,7 @@ namespace nbody
         void upload_bodies();
         void materialize() const;
         void dispatched_tree();
+        void predicted_readback();
 
         std::shared_ptr<HipDevice> _dev;
         HipMode _mode;
@@ -55,6 +57,9 @@ namespace nbody
         mutable bool _device_dirty = false;
         // State::bodies holds data the device has not seen yet.
         mutable bool _host_dirty = true;
+
+        // A materialize() followed the last dispatch: the next one is read too.
+        mutable bool _materialize_expected = false;
 
         // NLogN: the last dispatch's tree, copied from the device when tree() is called.
         mutable bh::Tree _tree;

@@ -270,3 +270,33 @@ def test_c1_plain_direct_sum_100_steps(ctx, orc):
           f"energy drift gpu {drift_gpu:.3e} cpu {drift_cpu:.3e}")
     assert dx <= 1e-5 and dv.max() <= 1e-5
     assert abs(drift_gpu - drift_cpu) <= 1e-6
+
+
+def test_predicted_readback_is_current_or_refreshed(ctx):
+    # nbhip_readback stages the bodies behind the enqueued work; nbhip_collect_bodies uses
+    # that copy only while nothing has changed the bodies since, else downloads afresh
+    b = ics.plummer(3000, seed=41)
+    p = _lib.params(dt=1e-3, eps=0.01)
+    ctx.upload_bodies(b)
+    ctx.step(p, 2)
+    ctx.readback()
+    got, staged = ctx.collect_bodies()
+    assert staged and np.array_equal(bits(got), bits(ctx.download_bodies()))
+    ctx.readback()
+    ctx.step(p, 1)                       # a dispatch after the readback: stale
+    got, staged = ctx.collect_bodies()
+    assert not staged and np.array_equal(bits(got), bits(ctx.download_bodies()))
+    ctx.readback()
+    ctx.upload_bodies(b)                 # an upload after the readback: stale
+    got, staged = ctx.collect_bodies()
+    assert not staged and np.array_equal(bits(got), bits(b))
+    ctx.upload_bodies(ics.plummer(70000, seed=42))   # the staging memory grows
+    ctx.step(p, 1)
+    ctx.readback()
+    got, staged = ctx.collect_bodies()
+    assert staged and got.shape == (70000, 12)
+    assert np.array_equal(bits(got), bits(ctx.download_bodies()))
+    ctx.reserve(0)
+    ctx.readback()
+    got, staged = ctx.collect_bodies()
+    assert staged and got.shape == (0, 12)
