@@ -38,6 +38,7 @@
 // only the lanes sitting on it advance.  Every lane therefore sees exactly its own
 // reference traversal while the node fetches stay uniform.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "common.hpp"
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(256) void k_keys(const float4* pm, int n, float siz
     vals[i] = unsigned(i);
 }
 
-// ---- scans (3-phase: tile totals, scan of totals, tile scans) -------------------------------
+// ---- scans (two phases: tile totals, then tile scans that add the totals before them) -----
 constexpr int SC_BLOCK = 256;
 constexpr int SC_IPT = 8;
 constexpr int SC_TILE = SC_BLOCK * SC_IPT;
@@ -114,27 +115,17 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_reduce(const T* in, int n, T*
     if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
-// Exclusive scan of the nb tile totals, in place, by one block.
-template <typename T>
-__global__ __launch_bounds__(SC_BLOCK) void k_scan_part(T* part, int nb)
-{
-    __shared__ T sh[SC_BLOCK];
-    T carry{};
-    for (int base = 0; base < nb; base += SC_BLOCK) {
-        const int i = base + threadIdx.x;
-        const T v = i < nb ? part[i] : T{};
-        T total;
-        const T ex = block_exclusive_scan(v, sh, total);
-        if (i < nb) part[i] = carry + ex;
-        carry = carry + total;
-    }
-}
-
-// out[i] = sum of in[0..i-1]; out[n] = total.  `out` may alias `in`.
+// out[i] = sum of in[0..i-1]; out[n] = total.  `out` may alias `in`.  Each block first
+// sums the totals of the tiles before it (part[0 .. blockIdx.x), a few KB read from L2 by
+// every block) in a fixed order, so no separate pass scans the tile totals.
 template <typename T>
 __global__ __launch_bounds__(SC_BLOCK) void k_scan_apply(const T* in, int n, const T* part, T* out)
 {
     __shared__ T sh[SC_BLOCK];
+    T pre{};
+    for (int b = threadIdx.x; b < int(blockIdx.x); b += SC_BLOCK) pre = pre + part[b];
+    T before;
+    (void)block_exclusive_scan(pre, sh, before);
     const int base = blockIdx.x * SC_TILE + threadIdx.x * SC_IPT;
     T v[SC_IPT];
     T sum{};
@@ -143,7 +134,7 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_apply(const T* in, int n, con
         sum = sum + v[k];
     }
     T total;
-    T run = part[blockIdx.x] + block_exclusive_scan(sum, sh, total);
+    T run = before + block_exclusive_scan(sum, sh, total);
     for (int k = 0; k < SC_IPT; ++k) {
         if (base + k < n) out[base + k] = run;
         run = run + v[k];
@@ -177,7 +168,7 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_single(const T* in, int n, T*
     if (threadIdx.x == 0) out[n] = carry;
 }
 
-constexpr int SC_SINGLE_MAX = 4 * SC_TILE;  // beyond this the three-phase scan is faster
+constexpr int SC_SINGLE_MAX = 4 * SC_TILE;  // beyond this the two-phase scan is faster
 
 template <typename T>
 void exclusive_scan(nbhip_ctx* c, const T* in, int n, T* out, DevBuf<T>& part)
@@ -191,54 +182,60 @@ void exclusive_scan(nbhip_ctx* c, const T* in, int n, T* out, DevBuf<T>& part)
     part.grow(nb, 0, c->stream);
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(SC_BLOCK), 0, c->stream, in, n, part.ptr);
     NBHIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_scan_part<T>, dim3(1), dim3(SC_BLOCK), 0, c->stream, part.ptr, nb);
-    NBHIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_scan_apply<T>, dim3(nb), dim3(SC_BLOCK), 0, c->stream, in, n, part.ptr, out);
     NBHIP_CHECK(hipGetLastError());
 }
 
-// ---- 2. LSD radix sort, 8-bit digits, stable ----------------------------------------------
-constexpr int RS_BLOCK = 256;
-constexpr int RS_WAVES = RS_BLOCK / 64;
-constexpr int RS_IPT = 16;
-constexpr int RS_TILE = RS_BLOCK * RS_IPT;
+// The apply phase alone, for a producer that already wrote the tile totals.
+template <typename T>
+void exclusive_scan_apply(nbhip_ctx* c, const T* in, int n, T* out, const T* part)
+{
+    const int nb = std::max(1, (n + SC_TILE - 1) / SC_TILE);
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3(nb), dim3(SC_BLOCK), 0, c->stream, in, n, part, out);
+    NBHIP_CHECK(hipGetLastError());
+}
 
-__global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const unsigned long long* keys, int n, int shift,
-                                                      unsigned* hist)
+// ---- 2. LSD radix sort, 8-bit digits, stable ----------------------------------------------
+// A block of RB threads sorts tiles of RB * IPT items: a histogram pass, a scan of the
+// digit-major block histograms, then a scatter that ranks items stably within the tile.
+template <int RB, int IPT>
+__global__ __launch_bounds__(RB) void k_rs_hist(const unsigned long long* keys, int n, int shift, unsigned* hist)
 {
     __shared__ unsigned h[256];
-    h[threadIdx.x] = 0;
+    for (int i = threadIdx.x; i < 256; i += RB) h[i] = 0;
     __syncthreads();
-    const int base = blockIdx.x * RS_TILE;
-    for (int r = 0; r < RS_IPT; ++r) {
-        const int k = base + r * RS_BLOCK + threadIdx.x;
+    const int base = blockIdx.x * (RB * IPT);
+    for (int r = 0; r < IPT; ++r) {
+        const int k = base + r * RB + threadIdx.x;
         if (k < n) atomicAdd(&h[unsigned(keys[k] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];  // digit-major
+    for (int i = threadIdx.x; i < 256; i += RB) hist[i * gridDim.x + blockIdx.x] = h[i];  // digit-major
 }
 
-// Scatter with stable ranks: items are taken in index order, 256 per round; within a
-// wave, lanes holding the same digit are found with 8 ballots; wave counts are combined
-// in wave order through LDS.
-__global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const unsigned long long* kin, const unsigned* vin,
-                                                         unsigned long long* kout, unsigned* vout, int n,
-                                                         int shift, const unsigned* offs)
+// Scatter with stable ranks: items are taken in index order, RB per round; within a wave,
+// lanes holding the same digit are found with 8 ballots; wave counts are combined in wave
+// order through LDS.
+template <int RB, int IPT>
+__global__ __launch_bounds__(RB) void k_rs_scatter(const unsigned long long* kin, const unsigned* vin,
+                                                   unsigned long long* kout, unsigned* vout, int n, int shift,
+                                                   const unsigned* offs)
 {
+    constexpr int WAVES = RB / 64;
     __shared__ unsigned running[256];
-    __shared__ unsigned wcnt[RS_WAVES][256];
+    __shared__ unsigned wcnt[WAVES][256];
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wave = t >> 6;
-    running[t] = offs[t * gridDim.x + blockIdx.x];
+    for (int i = t; i < 256; i += RB) running[i] = offs[i * gridDim.x + blockIdx.x];
     const unsigned long long below = (1ull << lane) - 1ull;
-    const int base = blockIdx.x * RS_TILE;
-    for (int r = 0; r < RS_IPT; ++r) {
-        const int k = base + r * RS_BLOCK + t;
+    const int base = blockIdx.x * (RB * IPT);
+    for (int r = 0; r < IPT; ++r) {
+        const int k = base + r * RB + t;
         const bool valid = k < n;
         const unsigned long long key = valid ? kin[k] : 0ull;
         const unsigned digit = unsigned(key >> shift) & 255u;
-        for (int w = 0; w < RS_WAVES; ++w) wcnt[w][t] = 0;
+        for (int i = t; i < WAVES * 256; i += RB) (&wcnt[0][0])[i] = 0;
         __syncthreads();
         unsigned long long peers = __ballot(valid);
         for (int b = 0; b < 8; ++b) {
@@ -258,27 +255,64 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const unsigned long lon
             }
         }
         __syncthreads();
-        unsigned add = 0;
-        for (int w = 0; w < RS_WAVES; ++w) add += wcnt[w][t];
-        running[t] += add;
+        for (int i = t; i < 256; i += RB) {
+            unsigned add = 0;
+            for (int w = 0; w < WAVES; ++w) add += wcnt[w][i];
+            running[i] += add;
+        }
         __syncthreads();
     }
+}
+
+struct SortShape {
+    int rb, ipt;
+    void (*hist)(const unsigned long long*, int, int, unsigned*);
+    void (*scatter)(const unsigned long long*, const unsigned*, unsigned long long*, unsigned*, int, int,
+                    const unsigned*);
+};
+
+template <int RB, int IPT>
+constexpr SortShape sort_shape()
+{
+    return SortShape{ RB, IPT, k_rs_hist<RB, IPT>, k_rs_scatter<RB, IPT> };
+}
+
+// Block shape of the radix sort (NBHIP_RS_SHAPE = index into this table overrides).  At
+// 2^20 keys 1024 x 4 measured fastest (16 waves per block hide the scatter's latency):
+// 0.883 ms per Barnes-Hut step against 0.894 (512 x 8), 0.931 (256 x 16), 0.943 (1024 x 8)
+// and 0.990 (512 x 16).
+const SortShape& sort_config()
+{
+    static const SortShape shapes[] = { sort_shape<1024, 4>(), sort_shape<512, 8>(), sort_shape<256, 16>(),
+                                        sort_shape<512, 16>(), sort_shape<1024, 8>() };
+    static const int pick = [] {
+        const char* e = std::getenv("NBHIP_RS_SHAPE");
+        const int k = e ? std::atoi(e) : 0;
+        return (k >= 0 && k < int(sizeof shapes / sizeof shapes[0])) ? k : 0;
+    }();
+    return shapes[pick];
+}
+
+int sort_blocks(int n)
+{
+    const SortShape& sh = sort_config();
+    return std::max(1, (n + sh.rb * sh.ipt - 1) / (sh.rb * sh.ipt));
 }
 
 // Sorts keys[src]/vals[src] over 63 key bits; returns the buffer index of the result.
 int radix_sort(nbhip_ctx* c, int n, int src)
 {
-    const int nb = std::max(1, (n + RS_TILE - 1) / RS_TILE);
+    const SortShape& sh = sort_config();
+    const int nb = sort_blocks(n);
     c->bh_hist.grow(size_t(256) * nb + 1, 0, c->stream);
     for (int shift = 0; shift < 64; shift += 8) {
         const int dst = 1 - src;
-        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_BLOCK), 0, c->stream, c->bh_keys[src].ptr, n,
-                           shift, c->bh_hist.ptr);
+        hipLaunchKernelGGL(sh.hist, dim3(nb), dim3(sh.rb), 0, c->stream, c->bh_keys[src].ptr, n, shift,
+                           c->bh_hist.ptr);
         NBHIP_CHECK(hipGetLastError());
         exclusive_scan<unsigned>(c, c->bh_hist.ptr, 256 * nb, c->bh_hist.ptr, c->bh_uscan);
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_BLOCK), 0, c->stream, c->bh_keys[src].ptr,
-                           c->bh_vals[src].ptr, c->bh_keys[dst].ptr, c->bh_vals[dst].ptr, n, shift,
-                           c->bh_hist.ptr);
+        hipLaunchKernelGGL(sh.scatter, dim3(nb), dim3(sh.rb), 0, c->stream, c->bh_keys[src].ptr,
+                           c->bh_vals[src].ptr, c->bh_keys[dst].ptr, c->bh_vals[dst].ptr, n, shift, c->bh_hist.ptr);
         NBHIP_CHECK(hipGetLastError());
         src = dst;
     }
@@ -286,16 +320,28 @@ int radix_sort(nbhip_ctx* c, int n, int src)
 }
 
 // ---- 3. bodies in key order + FP64 moments -------------------------------------------------
-__global__ __launch_bounds__(256) void k_gather(const unsigned* perm, const float4* pm, int n, float4* spm,
-                                                Dbl4* mom)
+// Each block gathers one scan tile (coalesced: position tile + i * SC_BLOCK + thread) and
+// writes the tile's moment total, so the prefix scan that follows needs only its apply phase.
+__global__ __launch_bounds__(SC_BLOCK) void k_gather(const unsigned* perm, const float4* pm, int n, float4* spm,
+                                                     Dbl4* mom, Dbl4* part)
 {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
-    const unsigned j = perm[k];
-    const float4 p = j < unsigned(n) ? pm[j] : make_float4(0.f, 0.f, 0.f, 0.f);
-    spm[k] = p;
-    const double m = p.w;
-    mom[k] = Dbl4{ m, m * double(p.x), m * double(p.y), m * double(p.z) };
+    __shared__ Dbl4 sh[SC_BLOCK];
+    const int base = blockIdx.x * SC_TILE + threadIdx.x;
+    Dbl4 sum{};
+    for (int i = 0; i < SC_IPT; ++i) {
+        const int k = base + i * SC_BLOCK;
+        if (k >= n) break;
+        const unsigned j = perm[k];
+        const float4 p = j < unsigned(n) ? pm[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        spm[k] = p;
+        const double m = p.w;
+        const Dbl4 d{ m, m * double(p.x), m * double(p.y), m * double(p.z) };
+        mom[k] = d;
+        sum = sum + d;
+    }
+    Dbl4 total;
+    (void)block_exclusive_scan(sum, sh, total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
 // ---- 4. node structure from adjacent-key prefixes ---------------------------------------------
@@ -560,10 +606,21 @@ struct Accum {
     }
 };
 
+// Workgroups are handed to the 8 XCDs round-robin (block b to XCD b mod 8).  Renumbering
+// them so that XCD x takes one contiguous run of the key order keeps the nodes and bucket
+// bodies a region walks in that XCD's own L2, instead of every L2 caching every region.
+__device__ __forceinline__ unsigned xcd_block()
+{
+    constexpr unsigned XCDS = 8;
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    const unsigned x = b % XCDS, q = b / XCDS;
+    return x * (nb / XCDS) + min(x, nb % XCDS) + q;
+}
+
 template <bool CUT, bool STEP, bool COUNT>
 __global__ __launch_bounds__(256) void k_walk(WalkArgs a)
 {
-    const int k = blockIdx.x * 256 + threadIdx.x;
+    const int k = int(xcd_block()) * 256 + threadIdx.x;
     const bool valid = k < a.n;
     const unsigned total = min(*a.total, a.capacity);
     const float4 x = valid ? a.spm[k] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -576,34 +633,53 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a)
     unsigned idx = valid ? 0u : total;
     Accum acc;
     unsigned applied = 0;
-    for (;;) {
-        const unsigned cur = __ockl_wfred_min_u32(idx);  // wave-uniform
-        if (cur >= total) break;
+    // After a step every lane's pointer is past cur, so when any lane sits on cur + 1 (an
+    // opened node's first child, next in pre-order) that is the new minimum and the
+    // wave-wide reduction is skipped.  The per-lane work sits under `if (idx == cur)`, so
+    // the node stays wave-uniform, in scalar registers.
+    unsigned cur = __ockl_wfred_min_u32(idx);
+    while (cur < total) {
         const BhNode nd = a.nodes[cur];
-        if (idx != cur) continue;
-        unsigned nxt = nd.next;
-        if (nd.cm.w != 0.f) {  // empty nodes are skipped (reference bhtree.cpp:108-113)
-            bool agg = nd.count == 1;
-            if (!agg) {
+        if (idx == cur) {
+            unsigned nxt = nd.next;
+            if (nd.cm.w != 0.f) {  // empty nodes are skipped (reference bhtree.cpp:108-113)
+                bool agg = nd.count == 1;
+                if (!agg) {
 #pragma clang fp contract(off)
-                const float dx = nd.cm.x - x.x, dy = nd.cm.y - x.y, dz = nd.cm.z - x.z;
-                agg = ((dx * dx) + (dy * dy)) + (dz * dz) > nd.thr;
+                    const float dx = nd.cm.x - x.x, dy = nd.cm.y - x.y, dz = nd.cm.z - x.z;
+                    agg = ((dx * dx) + (dy * dy)) + (dz * dz) > nd.thr;
+                }
+                if (agg) {
+                    interact<CUT>(x, r2, nd.cm, a.eps2, acc.tx, acc.ty, acc.tz);
+                    acc.tick();
+                    if (COUNT) ++applied;
+                } else if (nd.count == 0) {
+                    nxt = cur + 1;  // open: the first child follows in pre-order
+                } else {
+                    // a bucket, body by body; four bodies' loads in flight at a time
+                    const unsigned stop = min(nd.first + nd.count, unsigned(a.n));
+                    unsigned b = nd.first;
+                    for (; b + 4 <= stop; b += 4) {
+                        const float4 q0 = a.spm[b], q1 = a.spm[b + 1], q2 = a.spm[b + 2], q3 = a.spm[b + 3];
+                        interact<CUT>(x, r2, q0, a.eps2, acc.tx, acc.ty, acc.tz);
+                        interact<CUT>(x, r2, q1, a.eps2, acc.tx, acc.ty, acc.tz);
+                        interact<CUT>(x, r2, q2, a.eps2, acc.tx, acc.ty, acc.tz);
+                        interact<CUT>(x, r2, q3, a.eps2, acc.tx, acc.ty, acc.tz);
+                    }
+                    for (; b < stop; ++b)
+                        interact<CUT>(x, r2, a.spm[b], a.eps2, acc.tx, acc.ty, acc.tz);
+                    acc.tick(nd.count);
+                    if (COUNT) applied += nd.count;
+                }
             }
-            if (agg) {
-                interact<CUT>(x, r2, nd.cm, a.eps2, acc.tx, acc.ty, acc.tz);
-                acc.tick();
-                if (COUNT) ++applied;
-            } else if (nd.count == 0) {
-                nxt = cur + 1;  // open: the first child follows in pre-order
-            } else {
-                const unsigned stop = min(nd.first + nd.count, unsigned(a.n));
-                for (unsigned b = nd.first; b < stop; ++b)
-                    interact<CUT>(x, r2, a.spm[b], a.eps2, acc.tx, acc.ty, acc.tz);
-                acc.tick(nd.count);
-                if (COUNT) applied += nd.count;
-            }
+            idx = max(nxt, cur + 1);  // pre-order pointers only move forward
         }
-        idx = max(nxt, cur + 1);  // pre-order pointers only move forward
+        // every pointer is now > cur: if a lane is on cur + 1, that is the minimum
+        const unsigned step = cur + 1;
+        if (__ballot(idx == step) != 0ull)
+            cur = step;
+        else
+            cur = __ockl_wfred_min_u32(idx);
     }
     if (!valid) return;
     const float3 t = acc.total();
@@ -771,7 +847,7 @@ void bh_reserve(nbhip_ctx* c)
     c->bh_base.grow(size_t(n) + 1, 0, s);
     c->bh_nodes.grow(2 * size_t(n) + 1, 0, s);
     c->bh_kept.grow(n, 0, s);
-    const int nb = std::max(1, (n + RS_TILE - 1) / RS_TILE);
+    const int nb = sort_blocks(n);
     c->bh_hist.grow(size_t(256) * nb + 1, 0, s);
     c->bh_uscan.grow(size_t(std::max(1, (std::max(256 * nb, n) + SC_TILE - 1) / SC_TILE)), 0, s);
     c->bh_dscan.grow(size_t(std::max(1, (n + SC_TILE - 1) / SC_TILE)), 0, s);
@@ -800,10 +876,12 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     const unsigned long long* keys = c->bh_keys[sorted].ptr;
     const unsigned* perm = c->bh_vals[sorted].ptr;
 
-    hipLaunchKernelGGL(k_gather, dim3(grid(n)), dim3(256), 0, s, perm, c->pm[c->cur].ptr, n, c->bh_spm.ptr,
-                       c->bh_moments.ptr);
+    const int tiles = std::max(1, (n + SC_TILE - 1) / SC_TILE);
+    c->bh_dscan.grow(size_t(tiles), 0, s);
+    hipLaunchKernelGGL(k_gather, dim3(tiles), dim3(SC_BLOCK), 0, s, perm, c->pm[c->cur].ptr, n, c->bh_spm.ptr,
+                       c->bh_moments.ptr, c->bh_dscan.ptr);
     NBHIP_CHECK(hipGetLastError());
-    exclusive_scan<Dbl4>(c, c->bh_moments.ptr, n, c->bh_pref.ptr, c->bh_dscan);
+    exclusive_scan_apply<Dbl4>(c, c->bh_moments.ptr, n, c->bh_pref.ptr, c->bh_dscan.ptr);
 
     const int leaf = std::max(1, p.leaf_size);
     hipLaunchKernelGGL(k_count, dim3(grid(n)), dim3(256), 0, s, keys, n, leaf, c->bh_base.ptr,
