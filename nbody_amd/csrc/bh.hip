@@ -410,6 +410,184 @@ struct KeyView {
     }
 };
 
+// Nearest-smaller-value searches over d.  A node's end and a cell's start are the first
+// (last) position at or after (before) some point where d drops below the node's level:
+//   last(a, m)  = next_lt(a, m),   first(a, m) = prev_lt(a - 1, m) + 1,
+// with next_lt(f, m) = first j >= f with d(j) < m and prev_lt(f, m) = last j <= f with
+// d(j) < m (-1 if none).  k_levels stores e = d + 1 as one byte per position (e = 0 at the
+// last position, 255 as padding past it) and the minima of 16, 256 and 4096 positions, so
+// a search reads one 16-byte chunk per level, up and then down, instead of chasing keys.
+constexpr int LV_LEVELS = 4;
+
+struct DView {
+    const unsigned char* L[LV_LEVELS];
+    int len[LV_LEVELS];  // entries per level, multiples of 16
+
+    // Bytes of a 16-byte chunk with value < M, as bit 7 of each byte of two words (the
+    // values are below 0x80 or 0xFF, so the subtraction never borrows across bytes).
+    __device__ static void lt_bits(const unsigned char* L, int base, unsigned M, unsigned long long& w0,
+                                   unsigned long long& w1)
+    {
+        constexpr unsigned long long HI = 0x8080808080808080ull, ONES = 0x0101010101010101ull;
+        const uint4 v = *reinterpret_cast<const uint4*>(L + base);
+        const unsigned long long a0 = (unsigned long long)v.x | ((unsigned long long)v.y << 32);
+        const unsigned long long a1 = (unsigned long long)v.z | ((unsigned long long)v.w << 32);
+        const unsigned long long rep = ONES * M;
+        w0 = ~(((a0 | HI) - rep) & HI) & HI;
+        w1 = ~(((a1 | HI) - rep) & HI) & HI;
+    }
+    // first i in [idx, end of idx's chunk] with L[i] < M, or -1
+    __device__ static int scan_from(const unsigned char* L, int idx, unsigned M)
+    {
+        const int base = idx & ~15, s = idx & 15;
+        unsigned long long w0, w1;
+        lt_bits(L, base, M, w0, w1);
+        if (s < 8) w0 &= ~0ull << (8 * s);
+        else {
+            w0 = 0;
+            w1 &= ~0ull << (8 * (s - 8));
+        }
+        if (w0) return base + (__ffsll((long long)w0) - 1) / 8;
+        if (w1) return base + 8 + (__ffsll((long long)w1) - 1) / 8;
+        return -1;
+    }
+    // last i in [start of idx's chunk, idx] with L[i] < M, or -1
+    __device__ static int scan_back(const unsigned char* L, int idx, unsigned M)
+    {
+        const int base = idx & ~15, s = idx & 15;
+        unsigned long long w0, w1;
+        lt_bits(L, base, M, w0, w1);
+        if (s < 8) {
+            w0 &= ~0ull >> (8 * (7 - s));
+            w1 = 0;
+        } else {
+            w1 &= ~0ull >> (8 * (15 - s));
+        }
+        if (w1) return base + 8 + (63 - __clzll((long long)w1)) / 8;
+        if (w0) return base + (63 - __clzll((long long)w0)) / 8;
+        return -1;
+    }
+    __device__ int next_lt(int from, int m) const
+    {
+        const unsigned M = unsigned(m + 1);
+        int lvl = 0, idx = from;
+        for (;;) {
+            const int f = scan_from(L[lvl], idx, M);
+            if (f >= 0) {
+                idx = f;
+                break;
+            }
+            if (lvl == LV_LEVELS - 1) {
+                idx = ((idx >> 4) + 1) << 4;  // the top level is scanned chunk by chunk
+                if (idx >= len[lvl]) return len[0] - 1;  // unreachable: e(n-1) = 0
+                continue;
+            }
+            idx = (idx >> 4) + 1;
+            ++lvl;
+            if (idx >= len[lvl]) return len[0] - 1;  // unreachable: e(n-1) = 0
+        }
+        while (lvl > 0) {
+            --lvl;
+            idx = scan_from(L[lvl], idx << 4, M);  // the chunk's minimum is below M: found
+            if (idx < 0) return len[0] - 1;         // unreachable; keeps every read in bounds
+        }
+        return idx;
+    }
+    __device__ int prev_lt(int from, int m) const
+    {
+        if (from < 0) return -1;
+        const unsigned M = unsigned(m + 1);
+        int lvl = 0, idx = from;
+        for (;;) {
+            const int f = scan_back(L[lvl], idx, M);
+            if (f >= 0) {
+                idx = f;
+                break;
+            }
+            const int c = idx >> 4;
+            if (c == 0) return -1;
+            if (lvl == LV_LEVELS - 1) {
+                idx = (c << 4) - 1;
+                continue;
+            }
+            idx = c - 1;
+            ++lvl;
+        }
+        while (lvl > 0) {
+            --lvl;
+            idx = scan_back(L[lvl], (idx << 4) + 15, M);  // found, as above
+            if (idx < 0) return -1;                        // unreachable
+        }
+        return idx;
+    }
+};
+
+// One block per 4096 positions: e for its positions (16 per thread), then the minima of its
+// 256 chunks of 16, of its 16 chunks of 256, and of all 4096.
+__global__ __launch_bounds__(256) void k_levels(const unsigned long long* K, int n, unsigned char* L0,
+                                                unsigned char* L1, unsigned char* L2, unsigned char* L3, int l3len)
+{
+    __shared__ unsigned char m1[256];
+    __shared__ unsigned char m2[16];
+    const int t = threadIdx.x;
+    const int p0 = blockIdx.x * 4096 + t * 16;
+    unsigned w[4] = { 0u, 0u, 0u, 0u };
+    unsigned mn = 255u;
+    unsigned long long kp = p0 < n ? K[p0] : 0ull;
+    for (int i = 0; i < 16; ++i) {
+        const int p = p0 + i;
+        unsigned e;
+        if (p < n - 1) {
+            const unsigned long long kq = K[p + 1];
+            const unsigned long long x = kp ^ kq;
+            e = unsigned(x == 0ull ? LEVELS : (__clzll((long long)x) - 1) / 3) + 1u;
+            kp = kq;
+        } else {
+            e = p == n - 1 ? 0u : 255u;
+        }
+        w[i >> 2] |= e << (8 * (i & 3));
+        mn = min(mn, e);
+    }
+    *reinterpret_cast<uint4*>(L0 + p0) = make_uint4(w[0], w[1], w[2], w[3]);
+    m1[t] = (unsigned char)mn;
+    L1[blockIdx.x * 256 + t] = (unsigned char)mn;
+    __syncthreads();
+    if (t < 16) {
+        unsigned v = 255u;
+        for (int i = 0; i < 16; ++i) v = min(v, unsigned(m1[t * 16 + i]));
+        m2[t] = (unsigned char)v;
+        L2[blockIdx.x * 16 + t] = (unsigned char)v;
+    }
+    __syncthreads();
+    if (t == 0) {
+        unsigned v = 255u;
+        for (int i = 0; i < 16; ++i) v = min(v, unsigned(m2[i]));
+        L3[blockIdx.x] = (unsigned char)v;
+    }
+    // the top level's padding never compares below anything
+    if (blockIdx.x == gridDim.x - 1 && t > 0 && int(gridDim.x) + t - 1 < l3len) L3[gridDim.x + t - 1] = 255u;
+}
+
+// Layout of the level arrays for n positions: one block of k_levels per 4096.
+struct LevelSizes {
+    int nb, len[LV_LEVELS];
+    size_t off[LV_LEVELS], total;
+    explicit LevelSizes(int n)
+    {
+        nb = std::max(1, (n + 4095) / 4096);
+        len[0] = nb * 4096;
+        len[1] = nb * 256;
+        len[2] = nb * 16;
+        len[3] = (nb + 15) / 16 * 16;
+        size_t o = 0;
+        for (int l = 0; l < LV_LEVELS; ++l) {
+            off[l] = o;
+            o += size_t(len[l]);
+        }
+        total = o;
+    }
+};
+
 // Walks the nodes that start at sorted position a, deepest first, and calls
 // visit(level, b, bucket) for each node the tree keeps: internal nodes, and at most one
 // bucket (the shallowest cell at a holding at most leaf_size bodies, or a 21-level cell,
@@ -417,14 +595,14 @@ struct KeyView {
 // position a on its own) is kept, i.e. is not inside a bucket.  Each node's end is found
 // by galloping from the previous node's end: cpl(a, b + 1) = d(b) is the next level.
 template <typename Visit>
-__device__ bool walk_position(const KeyView& kv, int a, int leaf_size, Visit&& visit)
+__device__ bool walk_position(const KeyView& kv, const DView& dv, int a, int leaf_size, Visit&& visit)
 {
     if (kv.n == 1) return true;
     const int dl = kv.d(a - 1);
     int m = kv.d(a);
     if (m <= dl) {
         // no node starts at a: the deepest node holding a starts before it, at level d(a-1)
-        const int size = kv.last(a, dl) - kv.first(a, dl) + 1;
+        const int size = dv.next_lt(a, dl) - dv.prev_lt(a - 1, dl);
         return size > leaf_size && dl < LEVELS;
     }
     int lo = a;  // cpl(a, lo) >= m
@@ -432,7 +610,7 @@ __device__ bool walk_position(const KeyView& kv, int a, int leaf_size, Visit&& v
     bool held = false;  // a bucket seen but not yet emitted (its parent decides)
     int held_l = 0, held_b = 0;
     while (m > dl) {
-        const int b = kv.last_from(a, lo, m);
+        const int b = dv.next_lt(lo, m);  // the node's last position: d drops below m there
         const bool bucket = (b - a + 1) <= leaf_size || m == LEVELS;
         if (first) {
             deepest_is_bucket = bucket;
@@ -455,7 +633,7 @@ __device__ bool walk_position(const KeyView& kv, int a, int leaf_size, Visit&& v
     if (held) {
         // the shallowest node at a is a bucket; its parent starts before a, at level dl
         bool keep = true;
-        if (dl >= 0) keep = kv.last(a, dl) - kv.first(a, dl) + 1 > leaf_size;
+        if (dl >= 0) keep = dv.next_lt(a, dl) - dv.prev_lt(a - 1, dl) > leaf_size;
         if (keep) visit(held_l, held_b, true);
     }
     return !deepest_is_bucket;
@@ -463,12 +641,12 @@ __device__ bool walk_position(const KeyView& kv, int a, int leaf_size, Visit&& v
 
 // cnt[a]: entries position a adds to the pre-order (kept nodes + leaf); kept[a]: nodes only
 __global__ __launch_bounds__(256) void k_count(const unsigned long long* keys, int n, int leaf_size,
-                                               unsigned* cnt, unsigned char* kept)
+                                               unsigned* cnt, unsigned char* kept, DView dv)
 {
     const int a = blockIdx.x * 256 + threadIdx.x;
     if (a >= n) return;
     unsigned nodes = 0;
-    const bool leaf = walk_position(KeyView{ keys, n }, a, leaf_size, [&](int, int, bool) { ++nodes; });
+    const bool leaf = walk_position(KeyView{ keys, n }, dv, a, leaf_size, [&](int, int, bool) { ++nodes; });
     cnt[a] = nodes + (leaf ? 1u : 0u);
     kept[a] = (unsigned char)nodes;
 }
@@ -486,6 +664,7 @@ struct EmitArgs {
     int opening;
     BhNode* nodes;
     unsigned capacity;     // nodes allocated; entries past it are never written
+    DView dv;
 };
 
 __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
@@ -500,7 +679,7 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
     // nodes kept at a occupy [base, base + kept) shallow to deep; the walk sees them deep
     // to shallow, so they are placed from the top of that range down
     unsigned next_slot = base + kept;
-    walk_position(KeyView{ e.keys, e.n }, a, e.leaf_size, [&](int lvl, int b, bool bucket) {
+    walk_position(KeyView{ e.keys, e.n }, e.dv, a, e.leaf_size, [&](int lvl, int b, bool bucket) {
         const unsigned slot = --next_slot;
         if (slot >= e.capacity) return;
         const Dbl4 hi = e.pref[b + 1], lo = e.pref[a];
@@ -847,6 +1026,7 @@ void bh_reserve(nbhip_ctx* c)
     c->bh_base.grow(size_t(n) + 1, 0, s);
     c->bh_nodes.grow(2 * size_t(n) + 1, 0, s);
     c->bh_kept.grow(n, 0, s);
+    c->bh_levels.grow(LevelSizes(n).total, 0, s);
     const int nb = sort_blocks(n);
     c->bh_hist.grow(size_t(256) * nb + 1, 0, s);
     c->bh_uscan.grow(size_t(std::max(1, (std::max(256 * nb, n) + SC_TILE - 1) / SC_TILE)), 0, s);
@@ -883,14 +1063,26 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     NBHIP_CHECK(hipGetLastError());
     exclusive_scan_apply<Dbl4>(c, c->bh_moments.ptr, n, c->bh_pref.ptr, c->bh_dscan.ptr);
 
+    const LevelSizes ls(n);
+    c->bh_levels.grow(ls.total, 0, s);
+    DView dv{};
+    for (int l = 0; l < LV_LEVELS; ++l) {
+        dv.L[l] = c->bh_levels.ptr + ls.off[l];
+        dv.len[l] = ls.len[l];
+    }
+    hipLaunchKernelGGL(k_levels, dim3(ls.nb), dim3(256), 0, s, keys, n, c->bh_levels.ptr + ls.off[0],
+                       c->bh_levels.ptr + ls.off[1], c->bh_levels.ptr + ls.off[2], c->bh_levels.ptr + ls.off[3],
+                       ls.len[3]);
+    NBHIP_CHECK(hipGetLastError());
+
     const int leaf = std::max(1, p.leaf_size);
     hipLaunchKernelGGL(k_count, dim3(grid(n)), dim3(256), 0, s, keys, n, leaf, c->bh_base.ptr,
-                       c->bh_kept.ptr);
+                       c->bh_kept.ptr, dv);
     NBHIP_CHECK(hipGetLastError());
     exclusive_scan<unsigned>(c, c->bh_base.ptr, n, c->bh_base.ptr, c->bh_uscan);
 
     EmitArgs e{ keys, n, leaf, c->bh_base.ptr, c->bh_kept.ptr, c->bh_pref.ptr, c->bh_spm.ptr, p.size,
-                p.theta, p.opening, c->bh_nodes.ptr, unsigned(c->bh_nodes.cap) };
+                p.theta, p.opening, c->bh_nodes.ptr, unsigned(c->bh_nodes.cap), dv };
     hipLaunchKernelGGL(k_emit, dim3(grid(n)), dim3(256), 0, s, e);
     NBHIP_CHECK(hipGetLastError());
     c->bh_built = true;
@@ -1007,6 +1199,7 @@ void bh_release(nbhip_ctx* c)
     c->bh_kept.release();
     c->bh_icount.release();
     c->bh_tree.release();
+    c->bh_levels.release();
     c->bh_built = false;
     c->bh_tree_build = ~0ull;
 }

@@ -150,6 +150,7 @@ struct nbhip_ctx {
     nbhip::DevBuf<nbhip::Dbl4> bh_pref;
     nbhip::DevBuf<unsigned> bh_base;
     nbhip::DevBuf<unsigned char> bh_kept;
+    nbhip::DevBuf<unsigned char> bh_levels;  // d + 1 per sorted position and its chunk minima
     nbhip::DevBuf<nbhip::BhNode> bh_nodes;
     nbhip::DevBuf<unsigned> bh_hist;
     nbhip::DevBuf<unsigned> bh_uscan;
