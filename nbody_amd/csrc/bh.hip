@@ -199,7 +199,8 @@ void exclusive_scan_apply(nbhip_ctx* c, const T* in, int n, T* out, const T* par
 // A block of RB threads sorts tiles of RB * IPT items: a histogram pass, a scan of the
 // digit-major block histograms, then a scatter that ranks items stably within the tile.
 template <int RB, int IPT>
-__global__ __launch_bounds__(RB) void k_rs_hist(const unsigned long long* keys, int n, int shift, unsigned* hist)
+__global__ __launch_bounds__(RB) void k_rs_hist(const unsigned long long* keys, int n, int shift, unsigned* hist,
+                                                unsigned* totals)
 {
     __shared__ unsigned h[256];
     for (int i = threadIdx.x; i < 256; i += RB) h[i] = 0;
@@ -210,7 +211,35 @@ __global__ __launch_bounds__(RB) void k_rs_hist(const unsigned long long* keys, 
         if (k < n) atomicAdd(&h[unsigned(keys[k] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < 256; i += RB) hist[i * gridDim.x + blockIdx.x] = h[i];  // digit-major
+    for (int i = threadIdx.x; i < 256; i += RB) {
+        hist[i * gridDim.x + blockIdx.x] = h[i];  // digit-major
+        if (h[i]) atomicAdd(&totals[i], h[i]);    // the digit's count over all blocks
+    }
+}
+
+// Scatter offsets of one pass, in place: block d turns row d of the digit-major block
+// histograms into (keys of smaller digits) + (keys of digit d in earlier blocks).  It also
+// clears the other totals buffer, which the next pass's histogram fills.
+__global__ __launch_bounds__(256) void k_rs_offsets(unsigned* hist, int nb, const unsigned* totals,
+                                                    unsigned* next_totals)
+{
+    __shared__ unsigned sh[256];
+    const int d = blockIdx.x;
+    unsigned below = 0;
+    for (int i = threadIdx.x; i < d; i += 256) below += totals[i];
+    unsigned sum_below;
+    (void)block_exclusive_scan(below, sh, sum_below);
+    unsigned carry = sum_below;
+    unsigned* row = hist + size_t(d) * nb;
+    for (int b0 = 0; b0 < nb; b0 += 256) {
+        const int b = b0 + threadIdx.x;
+        const unsigned v = b < nb ? row[b] : 0u;
+        unsigned tile;
+        const unsigned ex = block_exclusive_scan(v, sh, tile);
+        if (b < nb) row[b] = carry + ex;
+        carry += tile;
+    }
+    if (d == 0) next_totals[threadIdx.x] = 0u;
 }
 
 // Scatter with stable ranks: items are taken in index order, RB per round; within a wave,
@@ -266,7 +295,7 @@ __global__ __launch_bounds__(RB) void k_rs_scatter(const unsigned long long* kin
 
 struct SortShape {
     int rb, ipt;
-    void (*hist)(const unsigned long long*, int, int, unsigned*);
+    void (*hist)(const unsigned long long*, int, int, unsigned*, unsigned*);
     void (*scatter)(const unsigned long long*, const unsigned*, unsigned long long*, unsigned*, int, int,
                     const unsigned*);
 };
@@ -300,17 +329,29 @@ int sort_blocks(int n)
 }
 
 // Sorts keys[src]/vals[src] over 63 key bits; returns the buffer index of the result.
+// bh_rs_totals holds two 256-entry digit-count buffers: pass p fills buffer p & 1, which
+// is zero when it starts (each pass's offsets kernel clears the other one; eight passes
+// per sort leave buffer 0 clear for the next sort).
 int radix_sort(nbhip_ctx* c, int n, int src)
 {
     const SortShape& sh = sort_config();
     const int nb = sort_blocks(n);
     c->bh_hist.grow(size_t(256) * nb + 1, 0, c->stream);
-    for (int shift = 0; shift < 64; shift += 8) {
+    if (!c->bh_rs_totals.ptr) {
+        c->bh_rs_totals.grow(512, 0, c->stream);
+        NBHIP_CHECK(hipMemsetAsync(c->bh_rs_totals.ptr, 0, 512 * sizeof(unsigned), c->stream));
+    }
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 8 * pass;
         const int dst = 1 - src;
+        unsigned* totals = c->bh_rs_totals.ptr + 256 * (pass & 1);
+        unsigned* next_totals = c->bh_rs_totals.ptr + 256 * ((pass + 1) & 1);
         hipLaunchKernelGGL(sh.hist, dim3(nb), dim3(sh.rb), 0, c->stream, c->bh_keys[src].ptr, n, shift,
-                           c->bh_hist.ptr);
+                           c->bh_hist.ptr, totals);
         NBHIP_CHECK(hipGetLastError());
-        exclusive_scan<unsigned>(c, c->bh_hist.ptr, 256 * nb, c->bh_hist.ptr, c->bh_uscan);
+        hipLaunchKernelGGL(k_rs_offsets, dim3(256), dim3(256), 0, c->stream, c->bh_hist.ptr, nb, totals,
+                           next_totals);
+        NBHIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(sh.scatter, dim3(nb), dim3(sh.rb), 0, c->stream, c->bh_keys[src].ptr,
                            c->bh_vals[src].ptr, c->bh_keys[dst].ptr, c->bh_vals[dst].ptr, n, shift, c->bh_hist.ptr);
         NBHIP_CHECK(hipGetLastError());
@@ -1027,6 +1068,10 @@ void bh_reserve(nbhip_ctx* c)
     c->bh_nodes.grow(2 * size_t(n) + 1, 0, s);
     c->bh_kept.grow(n, 0, s);
     c->bh_levels.grow(LevelSizes(n).total, 0, s);
+    if (!c->bh_rs_totals.ptr) {
+        c->bh_rs_totals.grow(512, 0, s);
+        NBHIP_CHECK(hipMemsetAsync(c->bh_rs_totals.ptr, 0, 512 * sizeof(unsigned), s));
+    }
     const int nb = sort_blocks(n);
     c->bh_hist.grow(size_t(256) * nb + 1, 0, s);
     c->bh_uscan.grow(size_t(std::max(1, (std::max(256 * nb, n) + SC_TILE - 1) / SC_TILE)), 0, s);
@@ -1200,6 +1245,7 @@ void bh_release(nbhip_ctx* c)
     c->bh_icount.release();
     c->bh_tree.release();
     c->bh_levels.release();
+    c->bh_rs_totals.release();
     c->bh_built = false;
     c->bh_tree_build = ~0ull;
 }

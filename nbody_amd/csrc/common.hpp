@@ -153,6 +153,7 @@ struct nbhip_ctx {
     nbhip::DevBuf<unsigned char> bh_levels;  // d + 1 per sorted position and its chunk minima
     nbhip::DevBuf<nbhip::BhNode> bh_nodes;
     nbhip::DevBuf<unsigned> bh_hist;
+    nbhip::DevBuf<unsigned> bh_rs_totals;  // radix sort: two 256-entry digit-count buffers
     nbhip::DevBuf<unsigned> bh_uscan;
     nbhip::DevBuf<nbhip::Dbl4> bh_dscan;
     nbhip::DevBuf<unsigned> bh_visits;
