@@ -787,6 +787,23 @@ __device__ __forceinline__ void interact(const float4 x, const float r2, const f
     az = fmaf(w, dz, az);
 }
 
+// The same for every lane of the wave, adding exactly zero where `on` is false (the weight
+// is selected, not multiplied, so an infinite 1/|d|^3 never reaches the sum).
+template <bool CUT>
+__device__ __forceinline__ void interact_if(bool on, const float4 x, const float r2, const float4 q,
+                                            const float eps2, float& ax, float& ay, float& az)
+{
+    const float dx = q.x - x.x, dy = q.y - x.y, dz = q.z - x.z;
+    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
+    float w = q.w * (inv * inv * inv);
+    if (CUT) on = on && d2 > r2;
+    w = on ? w : 0.f;
+    ax = fmaf(w, dx, ax);
+    ay = fmaf(w, dy, ay);
+    az = fmaf(w, dz, az);
+}
+
 // Knuth's TwoSum: s + c += t with the rounding error of the addition kept in c.
 __device__ __forceinline__ void two_sum(float& s, float& c, float t)
 {
@@ -798,13 +815,11 @@ __device__ __forceinline__ void two_sum(float& s, float& c, float t)
 }
 
 // Per-body force sum: interactions accumulate in plain FP32 and the partial is folded
-// into a compensated running sum every FOLD interactions.
+// into a compensated running sum (the walk folds every 8 of its steps).
 struct Accum {
-    static constexpr unsigned FOLD = 32;
     float tx = 0.f, ty = 0.f, tz = 0.f;      // current partial
     float sx = 0.f, sy = 0.f, sz = 0.f;      // running sums
     float cx = 0.f, cy = 0.f, cz = 0.f;      // their compensation
-    unsigned k = 0;
 
     __device__ void fold()
     {
@@ -812,12 +827,6 @@ struct Accum {
         two_sum(sy, cy, ty);
         two_sum(sz, cz, tz);
         tx = ty = tz = 0.f;
-        k = 0;
-    }
-    __device__ void tick(unsigned m = 1)
-    {
-        k += m;
-        if (k >= FOLD) fold();
     }
     __device__ float3 total()
     {
@@ -855,45 +864,47 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a)
     unsigned applied = 0;
     // After a step every lane's pointer is past cur, so when any lane sits on cur + 1 (an
     // opened node's first child, next in pre-order) that is the new minimum and the
-    // wave-wide reduction is skipped.  The per-lane work sits under `if (idx == cur)`, so
-    // the node stays wave-uniform, in scalar registers.
+    // wave-wide reduction is skipped.  The node is wave-uniform (scalar registers); the
+    // per-lane choices -- apply it, open it, apply a bucket's bodies -- are predicates on
+    // straight-line code rather than branches, so the accumulators are never copied
+    // between paths.  Partial sums fold into the compensated totals every 8 steps of the
+    // wave (at most 8 nodes or buckets per lane in between).
     unsigned cur = __ockl_wfred_min_u32(idx);
+    unsigned steps = 0;
     while (cur < total) {
         const BhNode nd = a.nodes[cur];
-        if (idx == cur) {
-            unsigned nxt = nd.next;
-            if (nd.cm.w != 0.f) {  // empty nodes are skipped (reference bhtree.cpp:108-113)
-                bool agg = nd.count == 1;
-                if (!agg) {
+        const bool at = idx == cur;
+        // empty nodes are skipped (reference bhtree.cpp:108-113); a leaf is applied
+        // directly; otherwise the reference's inverted criterion decides (bhtree.cpp:125-128)
+        const bool on = at && nd.cm.w != 0.f;
+        bool agg;
+        {
 #pragma clang fp contract(off)
-                    const float dx = nd.cm.x - x.x, dy = nd.cm.y - x.y, dz = nd.cm.z - x.z;
-                    agg = ((dx * dx) + (dy * dy)) + (dz * dz) > nd.thr;
-                }
-                if (agg) {
-                    interact<CUT>(x, r2, nd.cm, a.eps2, acc.tx, acc.ty, acc.tz);
-                    acc.tick();
-                    if (COUNT) ++applied;
-                } else if (nd.count == 0) {
-                    nxt = cur + 1;  // open: the first child follows in pre-order
-                } else {
-                    // a bucket, body by body; four bodies' loads in flight at a time
-                    const unsigned stop = min(nd.first + nd.count, unsigned(a.n));
-                    unsigned b = nd.first;
-                    for (; b + 4 <= stop; b += 4) {
-                        const float4 q0 = a.spm[b], q1 = a.spm[b + 1], q2 = a.spm[b + 2], q3 = a.spm[b + 3];
-                        interact<CUT>(x, r2, q0, a.eps2, acc.tx, acc.ty, acc.tz);
-                        interact<CUT>(x, r2, q1, a.eps2, acc.tx, acc.ty, acc.tz);
-                        interact<CUT>(x, r2, q2, a.eps2, acc.tx, acc.ty, acc.tz);
-                        interact<CUT>(x, r2, q3, a.eps2, acc.tx, acc.ty, acc.tz);
-                    }
-                    for (; b < stop; ++b)
-                        interact<CUT>(x, r2, a.spm[b], a.eps2, acc.tx, acc.ty, acc.tz);
-                    acc.tick(nd.count);
-                    if (COUNT) applied += nd.count;
-                }
-            }
-            idx = max(nxt, cur + 1);  // pre-order pointers only move forward
+            const float dx = nd.cm.x - x.x, dy = nd.cm.y - x.y, dz = nd.cm.z - x.z;
+            agg = (nd.count == 1) | (((dx * dx) + (dy * dy)) + (dz * dz) > nd.thr);
         }
+        interact_if<CUT>(on && agg, x, r2, nd.cm, a.eps2, acc.tx, acc.ty, acc.tz);
+        if (COUNT) applied += (on && agg) ? 1u : 0u;
+        const bool opened = on && !agg;
+        unsigned nxt = nd.next;
+        if (nd.count == 0) {
+            if (opened) nxt = cur + 1;  // open: the first child follows in pre-order
+        } else if (nd.count > 1 && __ballot(opened) != 0ull) {
+            // an opened bucket, body by body; four bodies' loads in flight at a time
+            const unsigned stop = min(nd.first + nd.count, unsigned(a.n));
+            unsigned b = nd.first;
+            for (; b + 4 <= stop; b += 4) {
+                const float4 q0 = a.spm[b], q1 = a.spm[b + 1], q2 = a.spm[b + 2], q3 = a.spm[b + 3];
+                interact_if<CUT>(opened, x, r2, q0, a.eps2, acc.tx, acc.ty, acc.tz);
+                interact_if<CUT>(opened, x, r2, q1, a.eps2, acc.tx, acc.ty, acc.tz);
+                interact_if<CUT>(opened, x, r2, q2, a.eps2, acc.tx, acc.ty, acc.tz);
+                interact_if<CUT>(opened, x, r2, q3, a.eps2, acc.tx, acc.ty, acc.tz);
+            }
+            for (; b < stop; ++b) interact_if<CUT>(opened, x, r2, a.spm[b], a.eps2, acc.tx, acc.ty, acc.tz);
+            if (COUNT) applied += opened ? nd.count : 0u;
+        }
+        if (at) idx = max(nxt, cur + 1);  // pre-order pointers only move forward
+        if ((++steps & 7u) == 0u) acc.fold();
         // every pointer is now > cur: if a lane is on cur + 1, that is the minimum
         const unsigned step = cur + 1;
         if (__ballot(idx == step) != 0ull)
