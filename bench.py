@@ -315,8 +315,9 @@ def run_served(args) -> dict:
     return {
         "workload": f"{count} jobs: Plummer or disk+bulge, N log-uniform in [1024, 262144], steps "
                     "uniform in [10, 1000]; direct sum below 16384 bodies (batched), Barnes-Hut "
-                    "theta 0.5 leaf 16 above (a context each, 8 concurrent); bodies drawn on "
-                    "the device at admission; 200 GB budget; "
+                    "theta 0.5 leaf 16 above (a context each, 8 concurrent, chunks of 16 steps "
+                    "replayed from captured graphs); bodies drawn on the device at admission; "
+                    "200 GB budget; "
                     f"Poisson arrivals at 0.7x the measured capacity ({rate:.1f} jobs/s)",
         "jobs": s["jobs"], "direct_jobs": n_direct, "barnes_hut_jobs": count - n_direct,
         "steps_per_s": s["steps_per_s"], "unit": "job-steps/s",

@@ -103,9 +103,13 @@ class _BhSlot:
 
 class Server:
     def __init__(self, hbm_budget: float = 200e9, bh_contexts: int = 8, chunk: int = 16,
-                 direct_below: int = 16384, keep_results: bool = False, batch_shape=None):
+                 direct_below: int = 16384, keep_results: bool = False, batch_shape=None,
+                 bh_graph: bool = True):
         self.budget = hbm_budget
         self.chunk = chunk
+        # Barnes-Hut chunks replayed from a captured graph (nbhip_step_graph): one launch per
+        # chunk instead of ~20 per step; captured again when a chunk's length changes.
+        self.bh_graph = bh_graph
         self.direct_below = direct_below
         self.keep_results = keep_results
         self.batch_ctx = _lib.Context(0)
@@ -230,7 +234,10 @@ class Server:
                         continue
                 if not s.inflight:
                     k = min(self.chunk, s.remaining)
-                    s.ctx.step(s.job.params, k)
+                    if self.bh_graph:
+                        s.ctx.step_graph(s.job.params, k)
+                    else:
+                        s.ctx.step(s.job.params, k)
                     s.inflight = k
 
             max_conc = max(max_conc, len(direct) + sum(1 for s in self.slots if s.job is not None))
