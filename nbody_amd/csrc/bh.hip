@@ -464,6 +464,9 @@ struct DView {
     const unsigned char* L[LV_LEVELS];
     int len[LV_LEVELS];  // entries per level, multiples of 16
 
+    // d(k) for -1 <= k <= n-1 (e(n-1) = 0 stores d = -1 past the last pair)
+    __device__ int d(int k) const { return k < 0 ? -1 : int(L[0][k]) - 1; }
+
     // Bytes of a 16-byte chunk with value < M, as bit 7 of each byte of two words (the
     // values are below 0x80 or 0xFF, so the subtraction never borrows across bytes).
     __device__ static void lt_bits(const unsigned char* L, int base, unsigned M, unsigned long long& w0,
@@ -639,8 +642,8 @@ template <typename Visit>
 __device__ bool walk_position(const KeyView& kv, const DView& dv, int a, int leaf_size, Visit&& visit)
 {
     if (kv.n == 1) return true;
-    const int dl = kv.d(a - 1);
-    int m = kv.d(a);
+    const int dl = dv.d(a - 1);
+    int m = dv.d(a);
     if (m <= dl) {
         // no node starts at a: the deepest node holding a starts before it, at level d(a-1)
         const int size = dv.next_lt(a, dl) - dv.prev_lt(a - 1, dl);
@@ -669,7 +672,7 @@ __device__ bool walk_position(const KeyView& kv, const DView& dv, int a, int lea
             visit(m, b, false);
         }
         lo = b + 1;
-        m = kv.d(b);
+        m = dv.d(b);
     }
     if (held) {
         // the shallowest node at a is a bucket; its parent starts before a, at level dl

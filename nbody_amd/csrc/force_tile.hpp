@@ -14,8 +14,8 @@
 // address (an LDS broadcast), so the W waves split j between them.  Per interaction:
 // 3 sub, 3 FMA (|d|^2 + eps^2), one v_rsq_f32, 3 mul, 3 FMA, all FP32 on the vector ALU
 // (no MFMA: a pairwise sum is not a dense contraction).  Each wave sums its 64-body
-// slice in plain FP32 and folds the slice total into a (sum, compensation) pair with
-// Knuth's TwoSum, once per 64 interactions; the W wave totals are then combined in a
+// slices in plain FP32 and folds the partial into a (sum, compensation) pair with
+// Knuth's TwoSum once every FOLD_TILES slices; the W wave totals are then combined in a
 // fixed order through LDS, so results do not depend on timing or on other work in
 // flight.
 #pragma once
@@ -24,6 +24,10 @@
 namespace nbhip {
 
 constexpr int WAVE = 64;
+
+// A wave's plain FP32 partial over this many j tiles (64 bodies each) is folded into the
+// compensated (sum, compensation) pair at a time.
+constexpr int FOLD_TILES = 2;
 
 // Knuth's TwoSum: s + c += t with the rounding error of the addition kept in c.
 __device__ __forceinline__ void two_sum(float& s, float& c, float t)
@@ -93,12 +97,13 @@ __device__ __forceinline__ void force_tile(ForceShared<P, W>& sh, const float4* 
         }
     }
 
-    float S[P][3], C[P][3];
+    float S[P][3], C[P][3], T[P][3];
 #pragma unroll
     for (int p = 0; p < P; ++p)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) S[p][k] = C[p][k] = 0.f;
+        for (int k = 0; k < 3; ++k) S[p][k] = C[p][k] = T[p][k] = 0.f;
 
+    int tiles = 0;
     for (int t = j0; t < j1; t += WAVE * W) {
         const int jt = t + threadIdx.x;
         __syncthreads();  // the previous tile (or a previous call's epilogue) is done
@@ -106,10 +111,6 @@ __device__ __forceinline__ void force_tile(ForceShared<P, W>& sh, const float4* 
         // cutoff masks the 0/0 case when eps == 0).
         sh.tile[threadIdx.x] = jt < j1 ? pm[jt] : make_float4(0.f, 0.f, 0.f, 0.f);
         __syncthreads();
-
-        float T[P][3];
-#pragma unroll
-        for (int p = 0; p < P; ++p) T[p][0] = T[p][1] = T[p][2] = 0.f;
 
         const float4* tw = sh.tile + wave * WAVE;
 #pragma unroll 8
@@ -135,11 +136,21 @@ __device__ __forceinline__ void force_tile(ForceShared<P, W>& sh, const float4* 
                 T[p][2] = fmaf(w, dz, T[p][2]);
             }
         }
+        if (++tiles == FOLD_TILES) {  // block-uniform
+            tiles = 0;
 #pragma unroll
-        for (int p = 0; p < P; ++p)
+            for (int p = 0; p < P; ++p)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) two_sum(S[p][k], C[p][k], T[p][k]);
+                for (int k = 0; k < 3; ++k) {
+                    two_sum(S[p][k], C[p][k], T[p][k]);
+                    T[p][k] = 0.f;
+                }
+        }
     }
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) two_sum(S[p][k], C[p][k], T[p][k]);
 
     // Combine the W wave partials in a fixed order.
     __syncthreads();  // nobody still reads red from a previous call
