@@ -66,10 +66,11 @@ __device__ __forceinline__ void kick_drift(float4& x, float4& v, float ax, float
     }
 }
 
-// LDS of one tile pass.
+// LDS of one tile pass: two j tiles (one being read, the next being written) and the
+// cross-wave reduction.
 template <int P, int W>
 struct ForceShared {
-    float4 tile[WAVE * W];
+    float4 tile[2][WAVE * W];
     float red[W][P][3][WAVE];
 };
 
@@ -103,16 +104,22 @@ __device__ __forceinline__ void force_tile(ForceShared<P, W>& sh, const float4* 
 #pragma unroll
         for (int k = 0; k < 3; ++k) S[p][k] = C[p][k] = T[p][k] = 0.f;
 
-    int tiles = 0;
-    for (int t = j0; t < j1; t += WAVE * W) {
-        const int jt = t + threadIdx.x;
-        __syncthreads();  // the previous tile (or a previous call's epilogue) is done
-        // Padding bodies have zero mass at the origin: they add exactly 0 (and the
-        // cutoff masks the 0/0 case when eps == 0).
-        sh.tile[threadIdx.x] = jt < j1 ? pm[jt] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // Tile k is read from tile[k & 1] while tile k + 1 is staged into the other buffer:
+    // its body is loaded into a register before tile k's arithmetic and written to LDS
+    // after it, so the load's latency hides behind the arithmetic and one barrier per tile
+    // suffices (it also guarantees nobody still reads the buffer being overwritten).
+    // Padding bodies have zero mass at the origin: they add exactly 0 (and the cutoff
+    // masks the 0/0 case when eps == 0).
+    constexpr int TILE = WAVE * W;
+    float4 stage = j0 + int(threadIdx.x) < j1 ? pm[j0 + threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();  // a previous call on this LDS is done with it
+    sh.tile[0][threadIdx.x] = stage;
+    int tiles = 0, kt = 0;
+    for (int t = j0; t < j1; t += TILE, ++kt) {
+        const int jn = t + TILE + threadIdx.x;
+        stage = jn < j1 ? pm[jn] : make_float4(0.f, 0.f, 0.f, 0.f);
         __syncthreads();
-
-        const float4* tw = sh.tile + wave * WAVE;
+        const float4* tw = sh.tile[kt & 1] + wave * WAVE;
 #pragma unroll 8
         for (int jj = 0; jj < WAVE; ++jj) {
             const float4 q = tw[jj];
@@ -136,6 +143,7 @@ __device__ __forceinline__ void force_tile(ForceShared<P, W>& sh, const float4* 
                 T[p][2] = fmaf(w, dz, T[p][2]);
             }
         }
+        sh.tile[(kt + 1) & 1][threadIdx.x] = stage;
         if (++tiles == FOLD_TILES) {  // block-uniform
             tiles = 0;
 #pragma unroll
