@@ -144,6 +144,71 @@ def test_forces_match_reference_barnes_hut(ctx, orc, n, size, theta):
     assert total == visits.sum()
 
 
+def test_theta_zero_accepts_the_root(ctx):
+    # Reference theta = 0 accepts a node whenever |com - x|^2 > 0, so every body sees the
+    # root alone: a_i = G M (com - x_i) / (|com - x_i|^2 + eps^2)^{3/2}, its own mass included
+    b = ics.plummer(3000, seed=12)
+    ctx.upload_bodies(b)
+    ctx.accelerate(bh_params(size=64.0, theta=0.0, eps=0.01, G=2.0))
+    a = ctx.download_bodies()[:, 8:11].astype(np.float64)
+    m = b[:, 7].astype(np.float64)
+    x = b[:, 0:3].astype(np.float64)
+    M = m.sum()
+    com = (m[:, None] * x).sum(0) / M
+    d = com - x
+    want = 2.0 * M * d / ((d * d).sum(1) + 1e-4)[:, None] ** 1.5
+    assert rel_err(a, want).max() <= 1e-5
+    _, total = ctx.bh_interactions(bh_params(size=64.0, theta=0.0, eps=0.01))
+    assert total == 3000
+
+
+def test_coincident_bodies_match_fp64_centre_oracle(ctx, orc):
+    # 600 bodies sharing one position: one 21-level cell, which the GPU keeps as a bucket
+    # and the reference splits down to FLT_EPSILON; the coincident bodies open it, and
+    # both add exactly zero for it (d = 0): the GPU body by body (599 more visits), the
+    # reference through one merged leaf
+    b = ics.plummer(3000, seed=13)
+    b[:600, 0:3] = b[0, 0:3]
+    ctx.upload_bodies(b)
+    p = bh_params(size=64.0, theta=0.5, eps=0.01)
+    visits, _ = ctx.bh_interactions(p)
+    out = ctx.download_bodies()
+    x64 = b.copy()
+    _, xv = orc.bh_accelerate_f64com(x64, 64.0, theta=0.5, eps=0.01, visits=True)
+    shared = np.zeros(len(b), bool)
+    shared[:600] = True
+    assert np.all(visits[shared] - xv[shared] == 599)
+    same = visits == xv
+    check = same | shared
+    e = rel_err(out[check, 8:11], x64[check, 8:11])
+    print(f"coincident: same traversal {same[~shared].mean():.5f}, max err {e.max():.2e}")
+    assert same[~shared].mean() >= 0.999 and e.max() <= 1e-4
+
+
+def test_bodies_outside_the_root_cube_match_reference(ctx, orc):
+    # A Plummer sphere in a root cube of edge 8: 6% of bodies lie outside and go to edge
+    # octants, as the reference's comparisons send them; a few beyond a corner in all
+    # three coordinates share one 21-level key.  The reference merges such bodies into a
+    # leaf at the first one's position (source/bhtree.cpp:37-42), which the FP64-centre
+    # oracle (centres rebuilt from leaves) then spreads to every ancestor, so this case is
+    # checked against the reference's own FP32 tree, whose ancestors accumulate the true
+    # positions: identical traversals, forces within its FP32 centre rounding (a CPU
+    # emulation of the GPU tree measured 7e-5 at 99.9%, 1.5e-4 max).
+    b = ics.plummer(4000, seed=14)
+    size = 8.0
+    assert np.mean(np.abs(b[:, 0:3]).max(1) > size / 2) > 0.01
+    ctx.upload_bodies(b)
+    visits, _ = ctx.bh_interactions(bh_params(size=size, theta=0.5, eps=0.01))
+    out = ctx.download_bodies()
+    ref = b.copy()
+    _, rv = orc.bh_accelerate(ref, size, theta=0.5, eps=0.01, visits=True)
+    same = visits == rv
+    e = rel_err(out[same, 8:11], ref[same, 8:11])
+    print(f"outside: same traversal {same.mean():.5f}, 99.9% {np.quantile(e, 0.999):.2e}, max {e.max():.2e}")
+    assert same.mean() >= 0.999
+    assert np.quantile(e, 0.999) <= 2e-4 and e.max() <= 1e-3
+
+
 def test_all_cells_open_equals_direct_sum(ctx, orc):
     b = ics.plummer(4000, seed=8)
     ctx.upload_bodies(b)
