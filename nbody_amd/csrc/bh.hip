@@ -19,15 +19,19 @@
 //
 // Build (all on the device, no host synchronisation):
 //   1. k_keys      63-bit key per body: 21 levels of the reference's own float descent.
-//   2. radix sort  8 stable LSD passes of 8 bits over (key, body index) pairs.
-//   3. k_gather    bodies in key order + per-body FP64 moments; exclusive scan -> prefix
-//                  sums, so a node's mass and centre over bodies [a, b] are two lookups.
-//   4. k_count /   with d(k) = common prefix (in octal digits) of keys k and k+1, the
-//      k_emit      nodes that start at sorted position a are found by walking the prefix
-//                  minima of d to the right of a (each found with a galloping search);
-//                  pre-order is "for a = 0..n-1: nodes starting at a, shallow to deep,
-//                  then leaf a", so a scan of per-position counts gives every node's
-//                  index, and a node covering bodies [a, b] has next = base[b + 1].
+//   2. radix sort  8 stable LSD passes of 8 bits over (key, body index) pairs: block
+//                  histograms (and digit totals by atomics), offsets, a ranked scatter.
+//   3. k_gather    bodies in key order + per-body FP64 moments and their tile totals;
+//                  the scan's apply phase -> prefix sums, so a node's mass and centre
+//                  over bodies [a, b] are two lookups.
+//   4. k_levels    d(k) = common prefix (in octal digits) of keys k and k+1, one byte per
+//                  position, with its minima over 16, 256 and 4096 positions.
+//   5. k_count /   the nodes that start at sorted position a are the prefix minima of d
+//      k_emit      to the right of a; each node's end is where d first drops below its
+//                  level, a nearest-smaller-value query on the levels.  Pre-order is
+//                  "for a = 0..n-1: nodes starting at a, shallow to deep, then leaf a",
+//                  so a scan of per-position counts gives every node's index, and a node
+//                  covering bodies [a, b] has next = base[b + 1].
 // Cells of 21 levels that still hold several bodies, and (leaf_size > 1) cells of at
 // most leaf_size bodies, become buckets: tested like an internal node, applied body by
 // body when opened.
@@ -37,6 +41,9 @@
 // minimum pointer over its lanes, loads that node once with a wave-uniform address, and
 // only the lanes sitting on it advance.  Every lane therefore sees exactly its own
 // reference traversal while the node fetches stay uniform.
+//
+// Export (k_tree_count / k_tree_emit): the same sorted keys in the reference's full
+// 48-byte node layout, for Solver::tree().
 #include <algorithm>
 #include <cstdlib>
 #include <vector>
@@ -415,23 +422,6 @@ struct KeyView {
         }
         return lo;
     }
-    // largest b >= lo with cpl(a, b) >= m, given cpl(a, lo) >= m
-    __device__ int last_from(int a, int lo, int m) const
-    {
-        int step = 1;
-        for (;;) {
-            const int t = lo + step;
-            if (t >= n || cpl(a, t) < m) break;
-            lo = t;
-            step <<= 1;
-        }
-        while (step > 1) {
-            step >>= 1;
-            const int t = lo + step;
-            if (t < n && cpl(a, t) >= m) lo = t;
-        }
-        return lo;
-    }
     // smallest a' <= a with cpl(a', a) >= m
     __device__ int first(int a, int m) const
     {
@@ -776,22 +766,9 @@ struct WalkArgs {
     int wrap;
 };
 
-template <bool CUT>
-__device__ __forceinline__ void interact(const float4 x, const float r2, const float4 q, const float eps2,
-                                         float& ax, float& ay, float& az)
-{
-    const float dx = q.x - x.x, dy = q.y - x.y, dz = q.z - x.z;
-    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
-    float w = q.w * (inv * inv * inv);
-    if (CUT) w = d2 > r2 ? w : 0.f;
-    ax = fmaf(w, dx, ax);
-    ay = fmaf(w, dy, ay);
-    az = fmaf(w, dz, az);
-}
-
-// The same for every lane of the wave, adding exactly zero where `on` is false (the weight
-// is selected, not multiplied, so an infinite 1/|d|^3 never reaches the sum).
+// Adds body-or-node q's pull on x to (ax, ay, az), for every lane of the wave, adding
+// exactly zero where `on` is false (the weight is selected, not multiplied, so an infinite
+// 1/|d|^3 never reaches the sum).
 template <bool CUT>
 __device__ __forceinline__ void interact_if(bool on, const float4 x, const float r2, const float4 q,
                                             const float eps2, float& ax, float& ay, float& az)
