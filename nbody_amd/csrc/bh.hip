@@ -459,16 +459,28 @@ struct DView {
 
     // Bytes of a 16-byte chunk with value < M, as bit 7 of each byte of two words (the
     // values are below 0x80 or 0xFF, so the subtraction never borrows across bytes).
-    __device__ static void lt_bits(const unsigned char* L, int base, unsigned M, unsigned long long& w0,
-                                   unsigned long long& w1)
+    __device__ static void lt_words(const uint4 v, unsigned M, unsigned long long& w0, unsigned long long& w1)
     {
         constexpr unsigned long long HI = 0x8080808080808080ull, ONES = 0x0101010101010101ull;
-        const uint4 v = *reinterpret_cast<const uint4*>(L + base);
         const unsigned long long a0 = (unsigned long long)v.x | ((unsigned long long)v.y << 32);
         const unsigned long long a1 = (unsigned long long)v.z | ((unsigned long long)v.w << 32);
         const unsigned long long rep = ONES * M;
         w0 = ~(((a0 | HI) - rep) & HI) & HI;
         w1 = ~(((a1 | HI) - rep) & HI) & HI;
+    }
+    __device__ static void lt_bits(const unsigned char* L, int base, unsigned M, unsigned long long& w0,
+                                   unsigned long long& w1)
+    {
+        lt_words(*reinterpret_cast<const uint4*>(L + base), M, w0, w1);
+    }
+    // The same test as a 16-bit mask, bit i for byte i: multiplying the bytes' 0/1 values
+    // by 0x0102040810204080 gathers byte i's bit into bit 56 + i without carries.
+    __device__ static unsigned lt_mask16(const uint4 v, unsigned M)
+    {
+        constexpr unsigned long long GATHER = 0x0102040810204080ull;
+        unsigned long long w0, w1;
+        lt_words(v, M, w0, w1);
+        return unsigned(((w0 >> 7) * GATHER) >> 56) | (unsigned(((w1 >> 7) * GATHER) >> 56) << 8);
     }
     // first i in [idx, end of idx's chunk] with L[i] < M, or -1
     __device__ static int scan_from(const unsigned char* L, int idx, unsigned M)
@@ -554,6 +566,51 @@ struct DView {
         }
         return idx;
     }
+
+    // Window forms of the two searches for nodes of at most NEAR_MAX bodies: both chunks a
+    // window can touch are loaded at once, so the answer costs one memory round trip.
+    static constexpr int NEAR_MAX = 16;
+    __device__ uint4 chunk(int c) const { return *reinterpret_cast<const uint4*>(L[0] + 16 * c); }
+
+    // first j in [from, last] with d(j) < m, or -1 (0 <= from <= n - 1, last - from < 16)
+    __device__ int next_lt_near(int from, int m, int last) const
+    {
+        if (last < from) return -1;
+        const unsigned M = unsigned(m + 1);
+        const int c = from >> 4, c1 = min(c + 1, (len[0] >> 4) - 1);
+        const uint4 v0 = chunk(c), v1 = chunk(c1);
+        unsigned mk = lt_mask16(v0, M) | (c1 > c ? lt_mask16(v1, M) << 16 : 0u);
+        mk >>= (from & 15);
+        if (mk == 0u) return -1;
+        const int j = from + __ffs(int(mk)) - 1;
+        return j <= last ? j : -1;
+    }
+    // last j in [first, from] with d(j) < m, counting d(-1) = -1; -2 if none
+    // (-1 <= from <= n - 1, from - first < 16)
+    __device__ int prev_lt_near(int from, int m, int first) const
+    {
+        if (from < 0) return -1;
+        if (from < first) return -2;
+        const unsigned M = unsigned(m + 1);
+        const int c = from >> 4, c0 = max(c - 1, 0);
+        const uint4 v1 = chunk(c), v0 = chunk(c0);
+        // bit 16 + i: position 16c + i; bit i: position 16(c - 1) + i
+        unsigned mk = (lt_mask16(v1, M) << 16) | (c > 0 ? lt_mask16(v0, M) : 0u);
+        mk &= (2u << (16 + (from & 15))) - 1u;  // positions <= from
+        int j;
+        if (mk != 0u) j = 16 * (c - 1) + (31 - __clz(int(mk)));
+        else j = c <= 1 ? -1 : -2;  // both chunks clear: the start of the array, or too far
+        return j >= first ? j : -2;
+    }
+    // Whether the level-m node holding position a has more than cap bodies:
+    // next_lt(a, m) - prev_lt(a - 1, m) > cap, for 1 <= cap <= NEAR_MAX.
+    __device__ bool run_exceeds(int a, int m, int cap) const
+    {
+        const int f = next_lt_near(a, m, a + cap - 1);
+        const int p = prev_lt_near(a - 1, m, a - cap);
+        if (f < 0 || p == -2) return true;
+        return f - p > cap;
+    }
 };
 
 // One block per 4096 positions: e for its positions (16 per thread), then the minima of its
@@ -632,19 +689,27 @@ template <typename Visit>
 __device__ bool walk_position(const KeyView& kv, const DView& dv, int a, int leaf_size, Visit&& visit)
 {
     if (kv.n == 1) return true;
+    // buckets fit the window searches when leaf_size <= NEAR_MAX; larger nodes, or a larger
+    // leaf_size, take the full searches
+    const bool near = leaf_size <= DView::NEAR_MAX;
+    const auto exceeds = [&](int m) {
+        return near ? dv.run_exceeds(a, m, leaf_size) : dv.next_lt(a, m) - dv.prev_lt(a - 1, m) > leaf_size;
+    };
     const int dl = dv.d(a - 1);
     int m = dv.d(a);
     if (m <= dl) {
         // no node starts at a: the deepest node holding a starts before it, at level d(a-1)
-        const int size = dv.next_lt(a, dl) - dv.prev_lt(a - 1, dl);
-        return size > leaf_size && dl < LEVELS;
+        return dl < LEVELS && exceeds(dl);
     }
     int lo = a;  // cpl(a, lo) >= m
     bool deepest_is_bucket = false, first = true;
-    bool held = false;  // a bucket seen but not yet emitted (its parent decides)
+    bool held = false;      // a bucket seen but not yet emitted (its parent decides)
+    bool internal = false;  // an internal node seen: every shallower node is internal too
     int held_l = 0, held_b = 0;
     while (m > dl) {
-        const int b = dv.next_lt(lo, m);  // the node's last position: d drops below m there
+        // the node's last position, where d drops below m: a bucket ends within a + leaf_size
+        int b = near && !internal ? dv.next_lt_near(lo, m, a + leaf_size - 1) : -1;
+        if (b < 0) b = dv.next_lt(near && !internal ? max(lo, a + leaf_size) : lo, m);
         const bool bucket = (b - a + 1) <= leaf_size || m == LEVELS;
         if (first) {
             deepest_is_bucket = bucket;
@@ -655,6 +720,7 @@ __device__ bool walk_position(const KeyView& kv, const DView& dv, int a, int lea
             held_l = m;
             held_b = b;
         } else {
+            internal = true;
             if (held) {   // its parent is this internal node: keep it
                 visit(held_l, held_b, true);
                 held = false;
@@ -666,23 +732,44 @@ __device__ bool walk_position(const KeyView& kv, const DView& dv, int a, int lea
     }
     if (held) {
         // the shallowest node at a is a bucket; its parent starts before a, at level dl
-        bool keep = true;
-        if (dl >= 0) keep = dv.next_lt(a, dl) - dv.prev_lt(a - 1, dl) > leaf_size;
-        if (keep) visit(held_l, held_b, true);
+        if (dl < 0 || exceeds(dl)) visit(held_l, held_b, true);
     }
     return !deepest_is_bucket;
 }
 
+// The nodes a position keeps, as its walk met them (deepest first), for the emit pass: up to
+// VC_MAX entries of {last position b, level | bucket << 8} per position, in memory the FP64
+// moments held until their prefix sums were taken (32 B per position).
+constexpr unsigned VC_MAX = 4;
+struct VisitCache {
+    uint4* v;  // 2 per position
+
+    __device__ static unsigned meta(int lvl, bool bucket) { return unsigned(lvl) | (bucket ? 256u : 0u); }
+};
+
 // cnt[a]: entries position a adds to the pre-order (kept nodes + leaf); kept[a]: nodes only
 __global__ __launch_bounds__(256) void k_count(const unsigned long long* keys, int n, int leaf_size,
-                                               unsigned* cnt, unsigned char* kept, DView dv)
+                                               unsigned* cnt, unsigned char* kept, DView dv, VisitCache vc)
 {
     const int a = blockIdx.x * 256 + threadIdx.x;
     if (a >= n) return;
     unsigned nodes = 0;
-    const bool leaf = walk_position(KeyView{ keys, n }, dv, a, leaf_size, [&](int, int, bool) { ++nodes; });
+    uint4 lo = make_uint4(0u, 0u, 0u, 0u), hi = lo;
+    const bool leaf = walk_position(KeyView{ keys, n }, dv, a, leaf_size, [&](int lvl, int b, bool bucket) {
+        const unsigned m = VisitCache::meta(lvl, bucket);
+        switch (nodes) {
+        case 0: lo.x = unsigned(b); lo.y = m; break;
+        case 1: lo.z = unsigned(b); lo.w = m; break;
+        case 2: hi.x = unsigned(b); hi.y = m; break;
+        case 3: hi.z = unsigned(b); hi.w = m; break;
+        default: break;
+        }
+        ++nodes;
+    });
     cnt[a] = nodes + (leaf ? 1u : 0u);
     kept[a] = (unsigned char)nodes;
+    if (nodes > 0u) vc.v[2 * size_t(a)] = lo;
+    if (nodes > 2u) vc.v[2 * size_t(a) + 1] = hi;
 }
 
 struct EmitArgs {
@@ -699,6 +786,7 @@ struct EmitArgs {
     BhNode* nodes;
     unsigned capacity;     // nodes allocated; entries past it are never written
     DView dv;
+    VisitCache vc;
 };
 
 __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
@@ -710,11 +798,7 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
     const unsigned kept = e.kept[a];
     const bool leaf_kept = e.base[a + 1] - base > kept;
     const float theta_sq = e.theta * e.theta;
-    // nodes kept at a occupy [base, base + kept) shallow to deep; the walk sees them deep
-    // to shallow, so they are placed from the top of that range down
-    unsigned next_slot = base + kept;
-    walk_position(KeyView{ e.keys, e.n }, e.dv, a, e.leaf_size, [&](int lvl, int b, bool bucket) {
-        const unsigned slot = --next_slot;
+    const auto emit = [&](unsigned slot, int lvl, int b, bool bucket) {
         if (slot >= e.capacity) return;
         const Dbl4 hi = e.pref[b + 1], lo = e.pref[a];
         const double M = hi.m - lo.m;
@@ -732,7 +816,21 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
         nd.first = unsigned(a);
         nd.count = bucket ? unsigned(b - a + 1) : 0u;
         e.nodes[slot] = nd;
-    });
+    };
+    // nodes kept at a occupy [base, base + kept) shallow to deep; the walk met them deep to
+    // shallow, so the j-th one met goes to base + kept - 1 - j
+    if (kept > VC_MAX) {
+        unsigned next_slot = base + kept;
+        walk_position(KeyView{ e.keys, e.n }, e.dv, a, e.leaf_size,
+                      [&](int lvl, int b, bool bucket) { emit(--next_slot, lvl, b, bucket); });
+    } else if (kept > 0u) {
+        const uint4 lo = e.vc.v[2 * size_t(a)];
+        const uint4 hi = kept > 2u ? e.vc.v[2 * size_t(a) + 1] : make_uint4(0u, 0u, 0u, 0u);
+        const unsigned bs[VC_MAX] = { lo.x, lo.z, hi.x, hi.z }, ms[VC_MAX] = { lo.y, lo.w, hi.y, hi.w };
+#pragma unroll
+        for (unsigned j = 0; j < VC_MAX; ++j)
+            if (j < kept) emit(base + kept - 1u - j, int(ms[j] & 255u), int(bs[j]), (ms[j] & 256u) != 0u);
+    }
     if (leaf_kept) {
         const unsigned slot = base + kept;
         if (slot < e.capacity) {
@@ -1112,13 +1210,16 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     NBHIP_CHECK(hipGetLastError());
 
     const int leaf = std::max(1, p.leaf_size);
+    // the moments are spent once their prefix sums exist: their memory holds the visits
+    static_assert(sizeof(Dbl4) == 2 * sizeof(uint4), "two cache words per position");
+    const VisitCache vc{ reinterpret_cast<uint4*>(c->bh_moments.ptr) };
     hipLaunchKernelGGL(k_count, dim3(grid(n)), dim3(256), 0, s, keys, n, leaf, c->bh_base.ptr,
-                       c->bh_kept.ptr, dv);
+                       c->bh_kept.ptr, dv, vc);
     NBHIP_CHECK(hipGetLastError());
     exclusive_scan<unsigned>(c, c->bh_base.ptr, n, c->bh_base.ptr, c->bh_uscan);
 
     EmitArgs e{ keys, n, leaf, c->bh_base.ptr, c->bh_kept.ptr, c->bh_pref.ptr, c->bh_spm.ptr, p.size,
-                p.theta, p.opening, c->bh_nodes.ptr, unsigned(c->bh_nodes.cap), dv };
+                p.theta, p.opening, c->bh_nodes.ptr, unsigned(c->bh_nodes.cap), dv, vc };
     hipLaunchKernelGGL(k_emit, dim3(grid(n)), dim3(256), 0, s, e);
     NBHIP_CHECK(hipGetLastError());
     c->bh_built = true;

@@ -98,6 +98,97 @@ def test_tree_structure(ctx, n, leaf):
         assert nodes["count"].max() <= leaf or n <= leaf
 
 
+def bucket_tree(keys, spm, size, theta, leaf, textbook=False):
+    """The leaf-bucket octree written from its definition, recursively over key prefixes:
+    a cell of two or more bodies is one node at the deepest level its bodies share (chains
+    of one-child cells collapse); it is a bucket when it holds at most `leaf` bodies or sits
+    at the 21st level, else an internal node whose children follow in octant order, a
+    one-body child being a leaf entry.  Nodes in pre-order with skip pointers; `spm` holds
+    the bodies in key order (x, y, z, mass)."""
+    K = [int(k) for k in keys]
+    n = len(K)
+    th2 = np.float32(theta) * np.float32(theta)
+    rows = []  # first, count, next, open_sq, com xyz, mass
+
+    def cpl(i, j):
+        x = K[i] ^ K[j]
+        return 21 if x == 0 else (64 - x.bit_length() - 1) // 3
+
+    def open_sq(level):
+        s = np.float32(size)
+        for _ in range(level):
+            s = np.float32(s * np.float32(0.5))
+        s2 = np.float32(s * s)
+        return np.float32(s2 / th2) if textbook else np.float32(s2 * th2)
+
+    def leaf_entry(a):
+        rows.append([a, 1, len(rows) + 1, np.float32(0.0), *spm[a, :3], spm[a, 3]])
+
+    def cell(s, e):
+        m = cpl(s, e - 1)
+        me = len(rows)
+        w = spm[s:e, 3].astype(np.float64)
+        M = w.sum()
+        com = (spm[s:e, :3].astype(np.float64) * w[:, None]).sum(0) / M if M != 0 else np.zeros(3)
+        bucket = e - s <= leaf or m == 21
+        rows.append([s, e - s if bucket else 0, 0, open_sq(m), *com, M])
+        if not bucket:
+            shift = 60 - 3 * m
+            a = s
+            while a < e:
+                q = (K[a] >> shift) & 7
+                b = a
+                while b < e and (K[b] >> shift) & 7 == q:
+                    b += 1
+                if b - a == 1:
+                    leaf_entry(a)
+                else:
+                    cell(a, b)
+                a = b
+        rows[me][2] = len(rows)
+
+    if n == 1:
+        leaf_entry(0)
+    else:
+        cell(0, n)
+    return rows
+
+
+@pytest.mark.parametrize("leaf", [1, 2, 5, 15, 16, 17, 40])
+@pytest.mark.parametrize("kind", ["plummer", "clumps"])
+def test_bucket_tree_matches_definition(ctx, leaf, kind):
+    # the node array against the definition, at leaf sizes either side of the window
+    # searches' 16-body limit; clumps put many bodies in a few deep cells and duplicates
+    # in 21-level cells
+    n = 6000
+    if kind == "plummer":
+        b = ics.plummer(n, seed=11 + leaf)
+    else:
+        rng = np.random.default_rng(leaf)
+        b = ics.plummer(n, seed=leaf)
+        centres = rng.normal(size=(12, 3)).astype(np.float32)
+        pick = rng.integers(0, 12, n)
+        b[:, 0:3] = centres[pick] + (rng.normal(size=(n, 3)) * 10.0 ** rng.integers(-6, -1, (n, 1))).astype(np.float32)
+        b[::37, 0:3] = b[1::37, 0:3][: b[::37].shape[0]]  # exact duplicates
+    ctx.upload_bodies(b)
+    p = bh_params(size=64.0, theta=0.7, leaf_size=leaf)
+    nodes = ctx.bh_nodes(p)
+    keys, perm = ctx.bh_keys(p)
+    spm = b[perm][:, [0, 1, 2, 7]]
+    ref = bucket_tree(np.sort(keys), spm, 64.0, 0.7, leaf)
+    assert nodes.size == len(ref)
+    r = np.array([row[:3] for row in ref], np.int64)
+    assert np.array_equal(nodes["first"], r[:, 0])
+    assert np.array_equal(nodes["count"], r[:, 1])
+    assert np.array_equal(nodes["next"], r[:, 2])
+    thr = np.array([row[3] for row in ref], np.float32)
+    assert np.array_equal(nodes["open_sq"].view(np.uint32), thr.view(np.uint32))
+    com = np.array([row[4:7] for row in ref], np.float64)
+    mass = np.array([row[7] for row in ref], np.float64)
+    assert np.allclose(nodes["mass"], mass, rtol=1e-6, atol=0)
+    assert np.allclose(nodes["com"], com, rtol=1e-6, atol=1e-6)
+
+
 def test_golden_reference_barnes_hut(ctx):
     # util::disk(512) with its radius cutoffs and central mass, root 10000 (the reference
     # default), theta 0.5 and 8: forces and per-body applied-node counts from the reference
