@@ -19,8 +19,9 @@
 //
 // Build (all on the device, no host synchronisation):
 //   1. k_keys      63-bit key per body: 21 levels of the reference's own float descent.
-//   2. radix sort  8 stable LSD passes of 8 bits over (key, body index) pairs: block
-//                  histograms (and digit totals by atomics), offsets, a ranked scatter.
+//   2. radix sort  8 stable LSD passes of 8 bits over (key, body index) pairs: the digit
+//                  counts of all passes in one read, then one launch per pass that ranks
+//                  a tile, looks back over earlier tiles for its digit starts and scatters.
 //   3. k_gather    bodies in key order + per-body FP64 moments and their tile totals;
 //                  the scan's apply phase -> prefix sums, so a node's mass and centre
 //                  over bodies [a, b] are two lookups.
@@ -28,7 +29,8 @@
 //                  position, with its minima over 16, 256 and 4096 positions.
 //   5. k_count /   the nodes that start at sorted position a are the prefix minima of d
 //      k_emit      to the right of a; each node's end is where d first drops below its
-//                  level, a nearest-smaller-value query on the levels.  Pre-order is
+//                  level, a nearest-smaller-value query on the levels (for a bucket, on
+//                  the two 16-position chunks it can span).  Pre-order is
 //                  "for a = 0..n-1: nodes starting at a, shallow to deep, then leaf a",
 //                  so a scan of per-position counts gives every node's index, and a node
 //                  covering bodies [a, b] has next = base[b + 1].
@@ -202,125 +204,196 @@ void exclusive_scan_apply(nbhip_ctx* c, const T* in, int n, T* out, const T* par
     NBHIP_CHECK(hipGetLastError());
 }
 
-// ---- 2. LSD radix sort, 8-bit digits, stable ----------------------------------------------
-// A block of RB threads sorts tiles of RB * IPT items: a histogram pass, a scan of the
-// digit-major block histograms, then a scatter that ranks items stably within the tile.
-template <int RB, int IPT>
-__global__ __launch_bounds__(RB) void k_rs_hist(const unsigned long long* keys, int n, int shift, unsigned* hist,
-                                                unsigned* totals)
+// ---- 2. LSD radix sort, 8-bit digits, stable, one launch per pass ---------------------------
+// k_rs_count takes the digit counts of all 8 passes in one read of the keys (they do not depend
+// on the order), k_rs_bases turns them into each pass's digit starts, and each pass is then a
+// single k_rs_pass launch: a block ranks one tile of RB * IPT keys, finds where each of its
+// digits starts by looking back over the tiles before it (decoupled look-back: every tile
+// publishes its digit counts as soon as it has them, then its running totals), and scatters.
+//
+// Sort state (bh_rs_state, unsigned words): the 8 x 256 digit counts, zero between sorts
+// (k_rs_bases clears them once read; zeroed once at allocation); the 8 x 256 digit starts;
+// 8 tile tickets, cleared by k_rs_bases; and a fault word.  bh_rs_status holds 8 x tiles x 256
+// look-back words, cleared by k_rs_count: bit 30 = tile count published, bit 31 = running
+// total (this and every earlier tile) published, low 30 bits = the value.
+constexpr int RS_PASSES = 8;
+constexpr int RS_COUNTS = 0, RS_BASES = RS_PASSES * 256, RS_TICKETS = 2 * RS_PASSES * 256,
+              RS_FAULT = RS_TICKETS + RS_PASSES, RS_WORDS = RS_FAULT + 8;
+constexpr unsigned LB_COUNT = 1u << 30, LB_TOTAL = 1u << 31, LB_VALUE = LB_COUNT - 1u;
+// Look-back waits end here, about a second in: the tile then takes zero for the unpublished
+// value and sets the fault word, which the next synchronisation reports; the sort is then
+// wrong but every block still finishes.
+constexpr unsigned LB_SPIN_LIMIT = 1u << 22;
+
+template <int RB>
+__global__ __launch_bounds__(RB) void k_rs_count(const unsigned long long* keys, int n, unsigned* counts,
+                                                 unsigned* status, size_t status_words)
 {
-    __shared__ unsigned h[256];
-    for (int i = threadIdx.x; i < 256; i += RB) h[i] = 0;
+    __shared__ unsigned h[RS_PASSES][256];
+    for (int i = threadIdx.x; i < RS_PASSES * 256; i += RB) (&h[0][0])[i] = 0u;
     __syncthreads();
-    const int base = blockIdx.x * (RB * IPT);
-    for (int r = 0; r < IPT; ++r) {
-        const int k = base + r * RB + threadIdx.x;
-        if (k < n) atomicAdd(&h[unsigned(keys[k] >> shift) & 255u], 1u);
+    const int stride = gridDim.x * RB;
+    for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += stride) {
+        const unsigned long long key = keys[k];
+#pragma unroll
+        for (int p = 0; p < RS_PASSES; ++p) atomicAdd(&h[p][unsigned(key >> (8 * p)) & 255u], 1u);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < 256; i += RB) {
-        hist[i * gridDim.x + blockIdx.x] = h[i];  // digit-major
-        if (h[i]) atomicAdd(&totals[i], h[i]);    // the digit's count over all blocks
+    for (int i = threadIdx.x; i < RS_PASSES * 256; i += RB) {
+        const unsigned v = (&h[0][0])[i];
+        if (v) atomicAdd(&counts[i], v);
     }
+    for (size_t i = size_t(blockIdx.x) * RB + threadIdx.x; i < status_words; i += size_t(stride)) status[i] = 0u;
 }
 
-// Scatter offsets of one pass, in place: block d turns row d of the digit-major block
-// histograms into (keys of smaller digits) + (keys of digit d in earlier blocks).  It also
-// clears the other totals buffer, which the next pass's histogram fills.
-__global__ __launch_bounds__(256) void k_rs_offsets(unsigned* hist, int nb, const unsigned* totals,
-                                                    unsigned* next_totals)
+// Block p of 8: digit starts of pass p from its counts, which it clears; block 0 also clears
+// the tile tickets.
+__global__ __launch_bounds__(256) void k_rs_bases(unsigned* state)
 {
     __shared__ unsigned sh[256];
-    const int d = blockIdx.x;
-    unsigned below = 0;
-    for (int i = threadIdx.x; i < d; i += 256) below += totals[i];
-    unsigned sum_below;
-    (void)block_exclusive_scan(below, sh, sum_below);
-    unsigned carry = sum_below;
-    unsigned* row = hist + size_t(d) * nb;
-    for (int b0 = 0; b0 < nb; b0 += 256) {
-        const int b = b0 + threadIdx.x;
-        const unsigned v = b < nb ? row[b] : 0u;
-        unsigned tile;
-        const unsigned ex = block_exclusive_scan(v, sh, tile);
-        if (b < nb) row[b] = carry + ex;
-        carry += tile;
-    }
-    if (d == 0) next_totals[threadIdx.x] = 0u;
+    const int t = threadIdx.x, p = blockIdx.x;
+    const unsigned v = state[RS_COUNTS + p * 256 + t];
+    state[RS_COUNTS + p * 256 + t] = 0u;
+    unsigned total;
+    state[RS_BASES + p * 256 + t] = block_exclusive_scan(v, sh, total);
+    if (p == 0 && t < RS_PASSES) state[RS_TICKETS + t] = 0u;
 }
 
-// Scatter with stable ranks: items are taken in index order, RB per round; within a wave,
-// lanes holding the same digit are found with 8 ballots; wave counts are combined in wave
-// order through LDS.
-template <int RB, int IPT>
-__global__ __launch_bounds__(RB) void k_rs_scatter(const unsigned long long* kin, const unsigned* vin,
-                                                   unsigned long long* kout, unsigned* vout, int n, int shift,
-                                                   const unsigned* offs)
+__device__ __forceinline__ unsigned lb_load(unsigned* word)
 {
-    constexpr int WAVES = RB / 64;
-    __shared__ unsigned running[256];
-    __shared__ unsigned wcnt[WAVES][256];
+    return __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned lb_wait(unsigned* word, unsigned* fault)
+{
+    for (unsigned spins = 0;; ++spins) {
+        const unsigned v = lb_load(word);
+        if (v & (LB_COUNT | LB_TOTAL)) return v;
+        if (spins == LB_SPIN_LIMIT) {
+            atomicOr(fault, 1u);
+            return LB_TOTAL;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// One pass.  Tiles are numbered by a ticket taken when the block starts, so every tile a
+// block waits for belongs to a block that is already running, and that block publishes its
+// counts before it waits for anything.  Wave w of the block owns items [64 IPT w, 64 IPT
+// (w + 1)) of the tile, 64 per round in index order: lanes holding the same digit are found
+// with 8 ballots, and the wave's own LDS counters give each item its rank among the wave's
+// earlier items of that digit.  The waves' counts then give every wave its start within each
+// digit, and the tile's counts go to the look-back, 8 earlier tiles per read.
+template <int RB, int IPT>
+__global__ __launch_bounds__(RB) void k_rs_pass(const unsigned long long* kin, const unsigned* vin,
+                                                unsigned long long* kout, unsigned* vout, int n, int shift,
+                                                const unsigned* bases, unsigned* ticket, unsigned* status,
+                                                unsigned* fault)
+{
+    static_assert(RB >= 256, "one thread per digit for the look-back");
+    constexpr int WAVES = RB / 64, LB_WINDOW = 8;
+    __shared__ unsigned s_tile;
+    __shared__ unsigned wh[WAVES][256];  // a wave's count of each digit, then its start in the tile's run
+    __shared__ unsigned start[256];      // where the tile's run of each digit begins in the output
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wave = t >> 6;
-    for (int i = t; i < 256; i += RB) running[i] = offs[i * gridDim.x + blockIdx.x];
-    const unsigned long long below = (1ull << lane) - 1ull;
-    const int base = blockIdx.x * (RB * IPT);
+    if (t == 0) s_tile = atomicAdd(ticket, 1u);
+    for (int i = t; i < WAVES * 256; i += RB) (&wh[0][0])[i] = 0u;
+    __syncthreads();
+    const unsigned tile = s_tile;
+    const int first = int(tile) * (RB * IPT) + wave * (64 * IPT) + lane;
+    unsigned long long key[IPT];
+    unsigned val[IPT], rank[IPT];
+#pragma unroll
     for (int r = 0; r < IPT; ++r) {
-        const int k = base + r * RB + t;
-        const bool valid = k < n;
-        const unsigned long long key = valid ? kin[k] : 0ull;
-        const unsigned digit = unsigned(key >> shift) & 255u;
-        for (int i = t; i < WAVES * 256; i += RB) (&wcnt[0][0])[i] = 0;
-        __syncthreads();
+        const int k = first + 64 * r;
+        key[r] = k < n ? kin[k] : 0ull;
+        val[r] = k < n ? vin[k] : 0u;
+    }
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+        const bool valid = first + 64 * r < n;
+        const unsigned digit = unsigned(key[r] >> shift) & 255u;
         unsigned long long peers = __ballot(valid);
         for (int b = 0; b < 8; ++b) {
             const bool bit = (digit >> b) & 1u;
             const unsigned long long m = __ballot(bit);
             peers &= bit ? m : ~m;
         }
-        const unsigned rank = unsigned(__popcll(peers & below));
-        if (valid && rank == 0) wcnt[wave][digit] = unsigned(__popcll(peers));
-        __syncthreads();
-        if (valid) {
-            unsigned pos = running[digit] + rank;
-            for (int w = 0; w < wave; ++w) pos += wcnt[w][digit];
-            if (pos < unsigned(n)) {
-                kout[pos] = key;
-                vout[pos] = vin[k];
+        // every lane reads the wave's count before the first lane of its digit adds to it;
+        // the wave's LDS accesses complete in program order
+        const unsigned before = wh[wave][digit];
+        const unsigned rk = unsigned(__popcll(peers & below));
+        if (valid && rk == 0u) wh[wave][digit] = before + unsigned(__popcll(peers));
+        rank[r] = before + rk;
+    }
+    __syncthreads();
+    if (t < 256) {
+        unsigned cnt = 0u;
+        for (int w = 0; w < WAVES; ++w) {
+            const unsigned c = wh[w][t];
+            wh[w][t] = cnt;
+            cnt += c;
+        }
+        unsigned* mine = status + size_t(tile) * 256 + t;
+        unsigned before = 0u;
+        if (tile == 0u) {
+            __hip_atomic_store(mine, LB_TOTAL | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(mine, LB_COUNT | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // tiles before 0 read as a zero total, so the walk ends at tile 0 at the latest
+            bool done = false;
+            for (int q = int(tile) - 1; !done; q -= LB_WINDOW) {
+                unsigned v[LB_WINDOW];
+#pragma unroll
+                for (int j = 0; j < LB_WINDOW; ++j) v[j] = q - j >= 0 ? lb_load(status + size_t(q - j) * 256 + t) : LB_TOTAL;
+#pragma unroll
+                for (int j = 0; j < LB_WINDOW; ++j) {
+                    if (done) continue;
+                    if (!(v[j] & (LB_COUNT | LB_TOTAL))) v[j] = lb_wait(status + size_t(q - j) * 256 + t, fault);
+                    before += v[j] & LB_VALUE;
+                    done = (v[j] & LB_TOTAL) != 0u;
+                }
             }
+            __hip_atomic_store(mine, LB_TOTAL | ((before + cnt) & LB_VALUE), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         }
-        __syncthreads();
-        for (int i = t; i < 256; i += RB) {
-            unsigned add = 0;
-            for (int w = 0; w < WAVES; ++w) add += wcnt[w][i];
-            running[i] += add;
+        start[t] = bases[t] + before;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+        if (first + 64 * r >= n) continue;
+        const unsigned digit = unsigned(key[r] >> shift) & 255u;
+        const unsigned pos = start[digit] + wh[wave][digit] + rank[r];
+        if (pos < unsigned(n)) {
+            kout[pos] = key[r];
+            vout[pos] = val[r];
         }
-        __syncthreads();
     }
 }
 
 struct SortShape {
     int rb, ipt;
-    void (*hist)(const unsigned long long*, int, int, unsigned*, unsigned*);
-    void (*scatter)(const unsigned long long*, const unsigned*, unsigned long long*, unsigned*, int, int,
-                    const unsigned*);
+    void (*pass)(const unsigned long long*, const unsigned*, unsigned long long*, unsigned*, int, int,
+                 const unsigned*, unsigned*, unsigned*, unsigned*);
 };
 
 template <int RB, int IPT>
 constexpr SortShape sort_shape()
 {
-    return SortShape{ RB, IPT, k_rs_hist<RB, IPT>, k_rs_scatter<RB, IPT> };
+    return SortShape{ RB, IPT, k_rs_pass<RB, IPT> };
 }
 
-// Block shape of the radix sort (NBHIP_RS_SHAPE = index into this table overrides).  At
-// 2^20 keys 1024 x 4 measured fastest (16 waves per block hide the scatter's latency):
-// 0.883 ms per Barnes-Hut step against 0.894 (512 x 8), 0.931 (256 x 16), 0.943 (1024 x 8)
-// and 0.990 (512 x 16).
+// Block shape of a pass (NBHIP_RS_SHAPE = index into this table overrides).  At 2^20 keys
+// 1024 x 4 measured fastest: 0.684 ms per Barnes-Hut step against 0.691 (512 x 8), 0.696
+// (256 x 16), 0.722 (512 x 4) and 0.733 (256 x 8).
 const SortShape& sort_config()
 {
     static const SortShape shapes[] = { sort_shape<1024, 4>(), sort_shape<512, 8>(), sort_shape<256, 16>(),
-                                        sort_shape<512, 16>(), sort_shape<1024, 8>() };
+                                        sort_shape<256, 8>(), sort_shape<512, 4>(), sort_shape<256, 4>() };
     static const int pick = [] {
         const char* e = std::getenv("NBHIP_RS_SHAPE");
         const int k = e ? std::atoi(e) : 0;
@@ -335,32 +408,40 @@ int sort_blocks(int n)
     return std::max(1, (n + sh.rb * sh.ipt - 1) / (sh.rb * sh.ipt));
 }
 
+constexpr int RS_COUNT_BLOCK = 1024;
+int rs_count_blocks(int n) { return std::max(1, std::min(64, (n + 16383) / 16384)); }
+
+// The sort's buffers for n keys; the state words are zeroed when first allocated (never
+// under graph capture: bh_reserve allocates before a capture starts).
+void rs_reserve(nbhip_ctx* c, int n)
+{
+    if (!c->bh_rs_state.ptr) {
+        c->bh_rs_state.grow(RS_WORDS, 0, c->stream);
+        NBHIP_CHECK(hipMemsetAsync(c->bh_rs_state.ptr, 0, RS_WORDS * sizeof(unsigned), c->stream));
+    }
+    c->bh_rs_status.grow(size_t(RS_PASSES) * 256 * sort_blocks(n), 0, c->stream);
+}
+
 // Sorts keys[src]/vals[src] over 63 key bits; returns the buffer index of the result.
-// bh_rs_totals holds two 256-entry digit-count buffers: pass p fills buffer p & 1, which
-// is zero when it starts (each pass's offsets kernel clears the other one; eight passes
-// per sort leave buffer 0 clear for the next sort).
 int radix_sort(nbhip_ctx* c, int n, int src)
 {
+    // look-back words carry 30-bit positions; 2^30 bodies would need over 288 GB here anyway
+    if (n >= int(LB_COUNT)) fail(NBHIP_ERR_INVALID, "Barnes-Hut sorts at most 2^30 - 1 bodies, not %d", n);
     const SortShape& sh = sort_config();
     const int nb = sort_blocks(n);
-    c->bh_hist.grow(size_t(256) * nb + 1, 0, c->stream);
-    if (!c->bh_rs_totals.ptr) {
-        c->bh_rs_totals.grow(512, 0, c->stream);
-        NBHIP_CHECK(hipMemsetAsync(c->bh_rs_totals.ptr, 0, 512 * sizeof(unsigned), c->stream));
-    }
-    for (int pass = 0; pass < 8; ++pass) {
-        const int shift = 8 * pass;
+    rs_reserve(c, n);
+    unsigned* st = c->bh_rs_state.ptr;
+    const size_t words = size_t(RS_PASSES) * 256 * nb;
+    hipLaunchKernelGGL(k_rs_count<RS_COUNT_BLOCK>, dim3(rs_count_blocks(n)), dim3(RS_COUNT_BLOCK), 0, c->stream,
+                       c->bh_keys[src].ptr, n, st + RS_COUNTS, c->bh_rs_status.ptr, words);
+    NBHIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_rs_bases, dim3(RS_PASSES), dim3(256), 0, c->stream, st);
+    NBHIP_CHECK(hipGetLastError());
+    for (int pass = 0; pass < RS_PASSES; ++pass) {
         const int dst = 1 - src;
-        unsigned* totals = c->bh_rs_totals.ptr + 256 * (pass & 1);
-        unsigned* next_totals = c->bh_rs_totals.ptr + 256 * ((pass + 1) & 1);
-        hipLaunchKernelGGL(sh.hist, dim3(nb), dim3(sh.rb), 0, c->stream, c->bh_keys[src].ptr, n, shift,
-                           c->bh_hist.ptr, totals);
-        NBHIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_rs_offsets, dim3(256), dim3(256), 0, c->stream, c->bh_hist.ptr, nb, totals,
-                           next_totals);
-        NBHIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(sh.scatter, dim3(nb), dim3(sh.rb), 0, c->stream, c->bh_keys[src].ptr,
-                           c->bh_vals[src].ptr, c->bh_keys[dst].ptr, c->bh_vals[dst].ptr, n, shift, c->bh_hist.ptr);
+        hipLaunchKernelGGL(sh.pass, dim3(nb), dim3(sh.rb), 0, c->stream, c->bh_keys[src].ptr, c->bh_vals[src].ptr,
+                           c->bh_keys[dst].ptr, c->bh_vals[dst].ptr, n, 8 * pass, st + RS_BASES + 256 * pass,
+                           st + RS_TICKETS + pass, c->bh_rs_status.ptr + size_t(256) * nb * pass, st + RS_FAULT);
         NBHIP_CHECK(hipGetLastError());
         src = dst;
     }
@@ -872,9 +953,29 @@ __device__ __forceinline__ void interact_if(bool on, const float4 x, const float
                                             const float eps2, float& ax, float& ay, float& az)
 {
     const float dx = q.x - x.x, dy = q.y - x.y, dz = q.z - x.z;
-    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
+    float inv;
+    if (CUT) {
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        inv = __builtin_amdgcn_rsqf(d2 + eps2);
+        on = on && d2 > r2;
+    } else {
+        inv = __builtin_amdgcn_rsqf(fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2))));
+    }
     float w = q.w * (inv * inv * inv);
+    w = on ? w : 0.f;
+    ax = fmaf(w, dx, ax);
+    ay = fmaf(w, dy, ay);
+    az = fmaf(w, dz, az);
+}
+
+// The same for a node whose offset (dx, dy, dz) and squared distance d2 are already known.
+template <bool CUT>
+__device__ __forceinline__ void interact_d2(bool on, const float dx, const float dy, const float dz, const float d2,
+                                            const float m, const float r2, const float eps2, float& ax, float& ay,
+                                            float& az)
+{
+    const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
+    float w = m * (inv * inv * inv);
     if (CUT) on = on && d2 > r2;
     w = on ? w : 0.f;
     ax = fmaf(w, dx, ax);
@@ -913,6 +1014,20 @@ struct Accum {
     }
 };
 
+// Minimum of v over the 64 lanes, for a wave with every lane active: row shifts leave each
+// row's minimum in its lane 15, two row broadcasts carry those into lane 63.  Lanes a DPP
+// move does not write read ~0u, which a minimum ignores.
+__device__ __forceinline__ unsigned wave_min(unsigned v)
+{
+    v = min(v, unsigned(__builtin_amdgcn_update_dpp(-1, int(v), 0x111, 0xf, 0xf, false)));  // row_shr:1
+    v = min(v, unsigned(__builtin_amdgcn_update_dpp(-1, int(v), 0x112, 0xf, 0xf, false)));  // row_shr:2
+    v = min(v, unsigned(__builtin_amdgcn_update_dpp(-1, int(v), 0x114, 0xf, 0xf, false)));  // row_shr:4
+    v = min(v, unsigned(__builtin_amdgcn_update_dpp(-1, int(v), 0x118, 0xf, 0xf, false)));  // row_shr:8
+    v = min(v, unsigned(__builtin_amdgcn_update_dpp(-1, int(v), 0x142, 0xa, 0xf, false)));  // row_bcast:15
+    v = min(v, unsigned(__builtin_amdgcn_update_dpp(-1, int(v), 0x143, 0xc, 0xf, false)));  // row_bcast:31
+    return unsigned(__builtin_amdgcn_readlane(int(v), 63));
+}
+
 // Workgroups are handed to the 8 XCDs round-robin (block b to XCD b mod 8).  Renumbering
 // them so that XCD x takes one contiguous run of the key order keeps the nodes and bucket
 // bodies a region walks in that XCD's own L2, instead of every L2 caching every region.
@@ -944,51 +1059,55 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a)
     // opened node's first child, next in pre-order) that is the new minimum and the
     // wave-wide reduction is skipped.  The node is wave-uniform (scalar registers); the
     // per-lane choices -- apply it, open it, apply a bucket's bodies -- are predicates on
-    // straight-line code rather than branches, so the accumulators are never copied
-    // between paths.  Partial sums fold into the compensated totals every 8 steps of the
-    // wave (at most 8 nodes or buckets per lane in between).
+    // straight-line code rather than branches.  Partial sums fold into the compensated
+    // totals every 8 steps of the wave (at most 8 nodes or buckets per lane in between).
     unsigned cur = __ockl_wfred_min_u32(idx);
     unsigned steps = 0;
     while (cur < total) {
         const BhNode nd = a.nodes[cur];
         const bool at = idx == cur;
         // empty nodes are skipped (reference bhtree.cpp:108-113); a leaf is applied
-        // directly; otherwise the reference's inverted criterion decides (bhtree.cpp:125-128)
-        const bool on = at && nd.cm.w != 0.f;
-        bool agg;
+        // directly; otherwise the reference's inverted criterion decides
+        // (bhtree.cpp:125-128), on the distance the force then uses
+        unsigned mbits = __float_as_uint(nd.cm.w);
+        asm("" : "+s"(mbits));  // a scalar test (the node is wave-uniform), not a per-lane compare
+        const bool on = at && (mbits << 1) != 0u;
+        float dx, dy, dz, d2;
         {
 #pragma clang fp contract(off)
-            const float dx = nd.cm.x - x.x, dy = nd.cm.y - x.y, dz = nd.cm.z - x.z;
-            agg = (nd.count == 1) | (((dx * dx) + (dy * dy)) + (dz * dz) > nd.thr);
+            dx = nd.cm.x - x.x;
+            dy = nd.cm.y - x.y;
+            dz = nd.cm.z - x.z;
+            d2 = ((dx * dx) + (dy * dy)) + (dz * dz);
         }
-        interact_if<CUT>(on && agg, x, r2, nd.cm, a.eps2, acc.tx, acc.ty, acc.tz);
+        const bool agg = (nd.count == 1) | (d2 > nd.thr);
+        interact_d2<CUT>(on && agg, dx, dy, dz, d2, nd.cm.w, r2, a.eps2, acc.tx, acc.ty, acc.tz);
         if (COUNT) applied += (on && agg) ? 1u : 0u;
         const bool opened = on && !agg;
-        unsigned nxt = nd.next;
-        if (nd.count == 0) {
-            if (opened) nxt = cur + 1;  // open: the first child follows in pre-order
-        } else if (nd.count > 1 && __ballot(opened) != 0ull) {
-            // an opened bucket, body by body; four bodies' loads in flight at a time
-            const unsigned stop = min(nd.first + nd.count, unsigned(a.n));
-            unsigned b = nd.first;
-            for (; b + 4 <= stop; b += 4) {
-                const float4 q0 = a.spm[b], q1 = a.spm[b + 1], q2 = a.spm[b + 2], q3 = a.spm[b + 3];
-                interact_if<CUT>(opened, x, r2, q0, a.eps2, acc.tx, acc.ty, acc.tz);
-                interact_if<CUT>(opened, x, r2, q1, a.eps2, acc.tx, acc.ty, acc.tz);
-                interact_if<CUT>(opened, x, r2, q2, a.eps2, acc.tx, acc.ty, acc.tz);
-                interact_if<CUT>(opened, x, r2, q3, a.eps2, acc.tx, acc.ty, acc.tz);
-            }
-            for (; b < stop; ++b) interact_if<CUT>(opened, x, r2, a.spm[b], a.eps2, acc.tx, acc.ty, acc.tz);
-            if (COUNT) applied += opened ? nd.count : 0u;
+        // open an internal node: its first child follows in pre-order
+        const unsigned nxt = (nd.count == 0 && opened) ? cur + 1 : nd.next;
+        // an opened bucket, body by body, four bodies' loads in flight at a time; the
+        // trip count is wave-uniform and zero unless some lane opened a bucket
+        const unsigned nb = (nd.count > 1 && __ballot(opened) != 0ull) ? min(nd.count, unsigned(a.n) - nd.first) : 0u;
+        unsigned j = 0;
+        for (; j + 4 <= nb; j += 4) {
+            const unsigned b = nd.first + j;
+            const float4 q0 = a.spm[b], q1 = a.spm[b + 1], q2 = a.spm[b + 2], q3 = a.spm[b + 3];
+            interact_if<CUT>(opened, x, r2, q0, a.eps2, acc.tx, acc.ty, acc.tz);
+            interact_if<CUT>(opened, x, r2, q1, a.eps2, acc.tx, acc.ty, acc.tz);
+            interact_if<CUT>(opened, x, r2, q2, a.eps2, acc.tx, acc.ty, acc.tz);
+            interact_if<CUT>(opened, x, r2, q3, a.eps2, acc.tx, acc.ty, acc.tz);
         }
+        for (; j < nb; ++j) interact_if<CUT>(opened, x, r2, a.spm[nd.first + j], a.eps2, acc.tx, acc.ty, acc.tz);
+        if (COUNT) applied += (opened && nd.count > 1) ? nd.count : 0u;
         if (at) idx = max(nxt, cur + 1);  // pre-order pointers only move forward
-        if ((++steps & 7u) == 0u) acc.fold();
         // every pointer is now > cur: if a lane is on cur + 1, that is the minimum
         const unsigned step = cur + 1;
         if (__ballot(idx == step) != 0ull)
             cur = step;
         else
-            cur = __ockl_wfred_min_u32(idx);
+            cur = max(wave_min(idx), step);  // the minimum is >= step; max keeps cur moving
+        if ((++steps & 7u) == 0u) acc.fold();
     }
     if (!valid) return;
     const float3 t = acc.total();
@@ -1157,14 +1276,22 @@ void bh_reserve(nbhip_ctx* c)
     c->bh_nodes.grow(2 * size_t(n) + 1, 0, s);
     c->bh_kept.grow(n, 0, s);
     c->bh_levels.grow(LevelSizes(n).total, 0, s);
-    if (!c->bh_rs_totals.ptr) {
-        c->bh_rs_totals.grow(512, 0, s);
-        NBHIP_CHECK(hipMemsetAsync(c->bh_rs_totals.ptr, 0, 512 * sizeof(unsigned), s));
-    }
-    const int nb = sort_blocks(n);
-    c->bh_hist.grow(size_t(256) * nb + 1, 0, s);
-    c->bh_uscan.grow(size_t(std::max(1, (std::max(256 * nb, n) + SC_TILE - 1) / SC_TILE)), 0, s);
+    rs_reserve(c, n);
+    c->bh_uscan.grow(size_t(std::max(1, (n + SC_TILE - 1) / SC_TILE)), 0, s);
     c->bh_dscan.grow(size_t(std::max(1, (n + SC_TILE - 1) / SC_TILE)), 0, s);
+}
+
+void bh_check_fault(nbhip_ctx* c)
+{
+    if (!c->bh_rs_state.ptr) return;
+    unsigned f = 0;
+    NBHIP_CHECK(hipMemcpyAsync(&f, c->bh_rs_state.ptr + RS_FAULT, sizeof f, hipMemcpyDeviceToHost, c->stream));
+    NBHIP_CHECK(hipStreamSynchronize(c->stream));
+    if (f == 0u) return;
+    NBHIP_CHECK(hipMemsetAsync(c->bh_rs_state.ptr + RS_FAULT, 0, sizeof f, c->stream));
+    NBHIP_CHECK(hipStreamSynchronize(c->stream));
+    fail(NBHIP_ERR_HIP, "a radix-sort pass gave up waiting for an earlier tile: Barnes-Hut results since the "
+                        "last synchronisation are invalid");
 }
 
 void bh_build(nbhip_ctx* c, const nbhip_params& p)
@@ -1329,7 +1456,7 @@ void bh_release(nbhip_ctx* c)
     c->bh_pref.release();
     c->bh_base.release();
     c->bh_nodes.release();
-    c->bh_hist.release();
+    c->bh_rs_status.release();
     c->bh_uscan.release();
     c->bh_dscan.release();
     c->bh_visits.release();
@@ -1337,7 +1464,7 @@ void bh_release(nbhip_ctx* c)
     c->bh_icount.release();
     c->bh_tree.release();
     c->bh_levels.release();
-    c->bh_rs_totals.release();
+    c->bh_rs_state.release();
     c->bh_built = false;
     c->bh_tree_build = ~0ull;
 }

@@ -152,8 +152,8 @@ struct nbhip_ctx {
     nbhip::DevBuf<unsigned char> bh_kept;
     nbhip::DevBuf<unsigned char> bh_levels;  // d + 1 per sorted position and its chunk minima
     nbhip::DevBuf<nbhip::BhNode> bh_nodes;
-    nbhip::DevBuf<unsigned> bh_hist;
-    nbhip::DevBuf<unsigned> bh_rs_totals;  // radix sort: two 256-entry digit-count buffers
+    nbhip::DevBuf<unsigned> bh_rs_state;   // radix sort: digit counts and starts, tickets, fault
+    nbhip::DevBuf<unsigned> bh_rs_status;  // radix sort: look-back words, 8 passes x tiles x 256
     nbhip::DevBuf<unsigned> bh_uscan;
     nbhip::DevBuf<nbhip::Dbl4> bh_dscan;
     nbhip::DevBuf<unsigned> bh_visits;
@@ -217,6 +217,8 @@ void bh_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse_integrate, boo
 void bh_release(nbhip_ctx* c);
 void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm);
 void bh_reserve(nbhip_ctx* c);
+// Reports (and clears) a radix-sort look-back that gave up waiting; call with the stream idle.
+void bh_check_fault(nbhip_ctx* c);
 size_t bh_tree(nbhip_ctx* c, const nbhip_params* p);
 void check_ics(const nbhip_ics* s, size_t n);
 bool ics_have_radius(const nbhip_ics& s);

@@ -487,6 +487,7 @@ int nbhip_synchronize(nbhip_ctx* c)
         check_ctx(c);
         NBHIP_CHECK(hipSetDevice(c->device));
         NBHIP_CHECK(hipStreamSynchronize(c->stream));
+        nbhip::bh_check_fault(c);
     });
 }
 

@@ -51,15 +51,32 @@ def test_every_job_same_alone_and_in_the_mix(device_ics):
 
 
 def test_budget_is_respected_and_queue_drains():
-    jobs = small_jobs(count=10, seed=9)
+    # every job is there at the start, so the unbudgeted server admits at once every job
+    # before the third Barnes-Hut one (two Barnes-Hut contexts, and the batch is idle); a
+    # budget between the largest job and those jobs' bytes then has to hold jobs back
+    def jobs():
+        js = small_jobs(count=10, seed=5)
+        for j in js:
+            j.arrival = 0.0
+        return js
+    js = jobs()
+    first, bh = 0, 0
+    for j in js:
+        bh += j.mode == _lib.MODE_NLOGN
+        if bh > 2:
+            break
+        first += j.device_bytes()
+    big = max(j.device_bytes() for j in js)
+    assert first > big
+    budget = (first + big) / 2
     with serve.Server(chunk=8, bh_contexts=2) as srv:
-        free = srv.run(small_jobs(count=10, seed=9))
-    budget = max(j.device_bytes() for j in jobs) * 1.2
+        free = srv.run(jobs())
     with serve.Server(hbm_budget=budget, chunk=8, bh_contexts=2) as srv:
-        rep = srv.run(jobs)
-    assert rep.jobs == len(jobs) and free.jobs == len(jobs)
-    assert rep.peak_device_bytes <= budget < free.peak_device_bytes
-    assert rep.max_concurrent <= free.max_concurrent
+        rep = srv.run(js)
+    assert rep.jobs == len(js) and free.jobs == len(js)
+    assert free.peak_device_bytes >= first > budget >= rep.peak_device_bytes
+    # jobs held at once never exceed what the budget holds of the smallest job
+    assert rep.max_concurrent <= int(budget // min(j.device_bytes() for j in js))
 
 
 def test_rejects_impossible_jobs():
