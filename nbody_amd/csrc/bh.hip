@@ -127,8 +127,26 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_reduce(const T* in, int n, T*
 // out[i] = sum of in[0..i-1]; out[n] = total.  `out` may alias `in`.  Each block first
 // sums the totals of the tiles before it (part[0 .. blockIdx.x), a few KB read from L2 by
 // every block) in a fixed order, so no separate pass scans the tile totals.
+// Item sources for the apply phase: an array, or a value computed from another array.
 template <typename T>
-__global__ __launch_bounds__(SC_BLOCK) void k_scan_apply(const T* in, int n, const T* part, T* out)
+struct ArrayOf {
+    const T* in;
+    __device__ T operator()(int k) const { return in[k]; }
+};
+
+// FP64 moments {m, m x, m y, m z} of a body in key order.
+struct MomentOf {
+    const float4* spm;
+    __device__ Dbl4 operator()(int k) const
+    {
+        const float4 p = spm[k];
+        const double m = p.w;
+        return Dbl4{ m, m * double(p.x), m * double(p.y), m * double(p.z) };
+    }
+};
+
+template <typename T, typename Src>
+__global__ __launch_bounds__(SC_BLOCK) void k_scan_apply(Src in, int n, const T* part, T* out)
 {
     __shared__ T sh[SC_BLOCK];
     T pre{};
@@ -139,7 +157,7 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_apply(const T* in, int n, con
     T v[SC_IPT];
     T sum{};
     for (int k = 0; k < SC_IPT; ++k) {
-        v[k] = base + k < n ? in[base + k] : T{};
+        v[k] = base + k < n ? in(base + k) : T{};
         sum = sum + v[k];
     }
     T total;
@@ -191,16 +209,17 @@ void exclusive_scan(nbhip_ctx* c, const T* in, int n, T* out, DevBuf<T>& part)
     part.grow(nb, 0, c->stream);
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(SC_BLOCK), 0, c->stream, in, n, part.ptr);
     NBHIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_scan_apply<T>, dim3(nb), dim3(SC_BLOCK), 0, c->stream, in, n, part.ptr, out);
+    hipLaunchKernelGGL((k_scan_apply<T, ArrayOf<T>>), dim3(nb), dim3(SC_BLOCK), 0, c->stream, ArrayOf<T>{ in }, n,
+                       part.ptr, out);
     NBHIP_CHECK(hipGetLastError());
 }
 
 // The apply phase alone, for a producer that already wrote the tile totals.
-template <typename T>
-void exclusive_scan_apply(nbhip_ctx* c, const T* in, int n, T* out, const T* part)
+template <typename T, typename Src>
+void exclusive_scan_apply(nbhip_ctx* c, Src in, int n, T* out, const T* part)
 {
     const int nb = std::max(1, (n + SC_TILE - 1) / SC_TILE);
-    hipLaunchKernelGGL(k_scan_apply<T>, dim3(nb), dim3(SC_BLOCK), 0, c->stream, in, n, part, out);
+    hipLaunchKernelGGL((k_scan_apply<T, Src>), dim3(nb), dim3(SC_BLOCK), 0, c->stream, in, n, part, out);
     NBHIP_CHECK(hipGetLastError());
 }
 
@@ -284,7 +303,7 @@ __device__ __forceinline__ unsigned lb_wait(unsigned* word, unsigned* fault)
 // with 8 ballots, and the wave's own LDS counters give each item its rank among the wave's
 // earlier items of that digit.  The waves' counts then give every wave its start within each
 // digit, and the tile's counts go to the look-back, 8 earlier tiles per read.
-template <int RB, int IPT>
+template <int RB, int IPT, bool STAGE>
 __global__ __launch_bounds__(RB) void k_rs_pass(const unsigned long long* kin, const unsigned* vin,
                                                 unsigned long long* kout, unsigned* vout, int n, int shift,
                                                 const unsigned* bases, unsigned* ticket, unsigned* status,
@@ -295,6 +314,10 @@ __global__ __launch_bounds__(RB) void k_rs_pass(const unsigned long long* kin, c
     __shared__ unsigned s_tile;
     __shared__ unsigned wh[WAVES][256];  // a wave's count of each digit, then its start in the tile's run
     __shared__ unsigned start[256];      // where the tile's run of each digit begins in the output
+    __shared__ unsigned lstart[STAGE ? 256 : 1];  // ... and in the tile's own digit order
+    __shared__ unsigned wtot[4];
+    __shared__ unsigned long long skey[STAGE ? RB * IPT : 1];
+    __shared__ unsigned sval[STAGE ? RB * IPT : 1];
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wave = t >> 6;
@@ -361,8 +384,51 @@ __global__ __launch_bounds__(RB) void k_rs_pass(const unsigned long long* kin, c
                                __HIP_MEMORY_SCOPE_AGENT);
         }
         start[t] = bases[t] + before;
+        if (STAGE) {
+            // the tile's digit runs in its own order: an exclusive scan of cnt over digits
+            unsigned inc = cnt;
+            for (int off = 1; off < 64; off <<= 1) {
+                const unsigned y = __shfl_up(inc, off);
+                if (lane >= off) inc += y;
+            }
+            if (lane == 63) wtot[wave] = inc;
+            lstart[t] = inc - cnt;
+        }
     }
     __syncthreads();
+    if (STAGE) {
+        // stage the tile in digit order, then write it out in that order: each digit's run
+        // of the tile lands on consecutive output positions from consecutive threads
+        if (t < 256) {
+            unsigned pre = 0u;
+            for (int w = 0; w < wave; ++w) pre += wtot[w];
+            lstart[t] += pre;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) {
+            if (first + 64 * r >= n) continue;
+            const unsigned digit = unsigned(key[r] >> shift) & 255u;
+            const unsigned lp = lstart[digit] + wh[wave][digit] + rank[r];
+            skey[lp] = key[r];
+            sval[lp] = val[r];
+        }
+        __syncthreads();
+        const int items = min(RB * IPT, n - int(tile) * (RB * IPT));
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) {
+            const int i = r * RB + t;
+            if (i >= items) continue;
+            const unsigned long long kk = skey[i];
+            const unsigned digit = unsigned(kk >> shift) & 255u;
+            const unsigned pos = start[digit] + (unsigned(i) - lstart[digit]);
+            if (pos < unsigned(n)) {
+                kout[pos] = kk;
+                vout[pos] = sval[i];
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < IPT; ++r) {
         if (first + 64 * r >= n) continue;
@@ -381,10 +447,10 @@ struct SortShape {
                  const unsigned*, unsigned*, unsigned*, unsigned*);
 };
 
-template <int RB, int IPT>
+template <int RB, int IPT, bool STAGE>
 constexpr SortShape sort_shape()
 {
-    return SortShape{ RB, IPT, k_rs_pass<RB, IPT> };
+    return SortShape{ RB, IPT, k_rs_pass<RB, IPT, STAGE> };
 }
 
 // Block shape of a pass (NBHIP_RS_SHAPE = index into this table overrides).  At 2^20 keys
@@ -392,8 +458,9 @@ constexpr SortShape sort_shape()
 // (256 x 16), 0.722 (512 x 4) and 0.733 (256 x 8).
 const SortShape& sort_config()
 {
-    static const SortShape shapes[] = { sort_shape<1024, 4>(), sort_shape<512, 8>(), sort_shape<256, 16>(),
-                                        sort_shape<256, 8>(), sort_shape<512, 4>(), sort_shape<256, 4>() };
+    static const SortShape shapes[] = { sort_shape<1024, 4, true>(), sort_shape<512, 8, true>(),
+                                        sort_shape<1024, 4, false>(), sort_shape<256, 16, true>(),
+                                        sort_shape<512, 4, true>(),  sort_shape<512, 8, false>() };
     static const int pick = [] {
         const char* e = std::getenv("NBHIP_RS_SHAPE");
         const int k = e ? std::atoi(e) : 0;
@@ -409,7 +476,9 @@ int sort_blocks(int n)
 }
 
 constexpr int RS_COUNT_BLOCK = 1024;
-int rs_count_blocks(int n) { return std::max(1, std::min(64, (n + 16383) / 16384)); }
+// Blocks of k_rs_count: at 2^20 keys 64 measured fastest (0.617 ms per Barnes-Hut step,
+// against 0.620 with 32, 0.634 with 128 and 0.662 with 256: more blocks, more global adds).
+int rs_count_blocks(int n) { return std::max(1, std::min(64, (n + 4095) / 4096)); }
 
 // The sort's buffers for n keys; the state words are zeroed when first allocated (never
 // under graph capture: bh_reserve allocates before a capture starts).
@@ -450,9 +519,10 @@ int radix_sort(nbhip_ctx* c, int n, int src)
 
 // ---- 3. bodies in key order + FP64 moments -------------------------------------------------
 // Each block gathers one scan tile (coalesced: position tile + i * SC_BLOCK + thread) and
-// writes the tile's moment total, so the prefix scan that follows needs only its apply phase.
+// writes the tile's moment total, so the prefix scan that follows needs only its apply
+// phase, which computes the moments again from the gathered bodies (the same doubles).
 __global__ __launch_bounds__(SC_BLOCK) void k_gather(const unsigned* perm, const float4* pm, int n, float4* spm,
-                                                     Dbl4* mom, Dbl4* part)
+                                                     Dbl4* part)
 {
     __shared__ Dbl4 sh[SC_BLOCK];
     const int base = blockIdx.x * SC_TILE + threadIdx.x;
@@ -464,9 +534,7 @@ __global__ __launch_bounds__(SC_BLOCK) void k_gather(const unsigned* perm, const
         const float4 p = j < unsigned(n) ? pm[j] : make_float4(0.f, 0.f, 0.f, 0.f);
         spm[k] = p;
         const double m = p.w;
-        const Dbl4 d{ m, m * double(p.x), m * double(p.y), m * double(p.z) };
-        mom[k] = d;
-        sum = sum + d;
+        sum = sum + Dbl4{ m, m * double(p.x), m * double(p.y), m * double(p.z) };
     }
     Dbl4 total;
     (void)block_exclusive_scan(sum, sh, total);
@@ -527,13 +595,16 @@ struct KeyView {
 //   last(a, m)  = next_lt(a, m),   first(a, m) = prev_lt(a - 1, m) + 1,
 // with next_lt(f, m) = first j >= f with d(j) < m and prev_lt(f, m) = last j <= f with
 // d(j) < m (-1 if none).  k_levels stores e = d + 1 as one byte per position (e = 0 at the
-// last position, 255 as padding past it) and the minima of 16, 256 and 4096 positions, so
-// a search reads one 16-byte chunk per level, up and then down, instead of chasing keys.
-constexpr int LV_LEVELS = 4;
+// last position, 255 as padding past it) and the minima of 16, 256 and 4096 positions;
+// beyond 2^20 positions k_levels_top adds minima of 16 entries of the level below until
+// the top level is at most 16 chunks of 16 bytes.  A search reads one chunk per level, up and then down, instead of chasing
+// keys.
+constexpr int LV_MAX = 8;  // enough for 2^31 positions
 
 struct DView {
-    const unsigned char* L[LV_LEVELS];
-    int len[LV_LEVELS];  // entries per level, multiples of 16
+    const unsigned char* L[LV_MAX];
+    int len[LV_MAX];  // entries per level, multiples of 16
+    int levels;
 
     // d(k) for -1 <= k <= n-1 (e(n-1) = 0 stores d = -1 past the last pair)
     __device__ int d(int k) const { return k < 0 ? -1 : int(L[0][k]) - 1; }
@@ -604,7 +675,7 @@ struct DView {
                 idx = f;
                 break;
             }
-            if (lvl == LV_LEVELS - 1) {
+            if (lvl == levels - 1) {
                 idx = ((idx >> 4) + 1) << 4;  // the top level is scanned chunk by chunk
                 if (idx >= len[lvl]) return len[0] - 1;  // unreachable: e(n-1) = 0
                 continue;
@@ -633,7 +704,7 @@ struct DView {
             }
             const int c = idx >> 4;
             if (c == 0) return -1;
-            if (lvl == LV_LEVELS - 1) {
+            if (lvl == levels - 1) {
                 idx = (c << 4) - 1;
                 continue;
             }
@@ -740,10 +811,11 @@ __global__ __launch_bounds__(256) void k_levels(const unsigned long long* K, int
     if (blockIdx.x == gridDim.x - 1 && t > 0 && int(gridDim.x) + t - 1 < l3len) L3[gridDim.x + t - 1] = 255u;
 }
 
-// Layout of the level arrays for n positions: one block of k_levels per 4096.
+// Layout of the level arrays for n positions: one block of k_levels per 4096, then levels
+// of 16-entry minima until the top level, scanned chunk by chunk, has at most 16 chunks.
 struct LevelSizes {
-    int nb, len[LV_LEVELS];
-    size_t off[LV_LEVELS], total;
+    int nb, levels, len[LV_MAX];
+    size_t off[LV_MAX], total;
     explicit LevelSizes(int n)
     {
         nb = std::max(1, (n + 4095) / 4096);
@@ -751,8 +823,13 @@ struct LevelSizes {
         len[1] = nb * 256;
         len[2] = nb * 16;
         len[3] = (nb + 15) / 16 * 16;
+        levels = 4;
+        while (levels < LV_MAX && len[levels - 1] > 256) {
+            len[levels] = (len[levels - 1] / 16 + 15) / 16 * 16;
+            ++levels;
+        }
         size_t o = 0;
-        for (int l = 0; l < LV_LEVELS; ++l) {
+        for (int l = 0; l < levels; ++l) {
             off[l] = o;
             o += size_t(len[l]);
         }
@@ -760,18 +837,52 @@ struct LevelSizes {
     }
 };
 
+struct LevelsTop {
+    unsigned char* L[LV_MAX];
+    int len[LV_MAX];
+    int levels;
+};
+
+// One block: levels 4 and up, each entry the minimum of 16 entries of the level below
+// (255 past the end).
+__global__ __launch_bounds__(256) void k_levels_top(LevelsTop lt)
+{
+    for (int l = 4; l < lt.levels; ++l) {
+        const unsigned char* src = lt.L[l - 1];
+        for (int i = threadIdx.x; i < lt.len[l]; i += 256) {
+            unsigned v = 255u;
+            if (16 * i < lt.len[l - 1])
+                for (int k = 0; k < 16; ++k) v = min(v, unsigned(src[16 * i + k]));
+            lt.L[l][i] = (unsigned char)v;
+        }
+        __syncthreads();  // the next level reads this one
+    }
+}
+
+// ns[j] = the first position k > j with d(k) < d(j), for 0 <= j < n - 1 (every search ends
+// by n - 1, where d = -1): one independent search per position.  The nodes starting at a
+// position then chain through it, one load per node (below).
+__global__ __launch_bounds__(256) void k_next_smaller(int n, DView dv, unsigned* ns)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n - 1) return;
+    ns[j] = unsigned(dv.next_lt(j + 1, dv.d(j)));
+}
+
 // Walks the nodes that start at sorted position a, deepest first, and calls
 // visit(level, b, bucket) for each node the tree keeps: internal nodes, and at most one
 // bucket (the shallowest cell at a holding at most leaf_size bodies, or a 21-level cell,
 // provided its parent is not itself such a cell).  Returns whether leaf a (the body at
-// position a on its own) is kept, i.e. is not inside a bucket.  Each node's end is found
-// by galloping from the previous node's end: cpl(a, b + 1) = d(b) is the next level.
+// position a on its own) is kept, i.e. is not inside a bucket.  The deepest node at a has
+// level d(a) and ends where d first drops below d(a), at ns[a]; the next one has level
+// cpl(a, b + 1) = d(b) and ends where d first drops below that after b, at ns[b].
 template <typename Visit>
-__device__ bool walk_position(const KeyView& kv, const DView& dv, int a, int leaf_size, Visit&& visit)
+__device__ bool walk_position(const KeyView& kv, const DView& dv, const unsigned* ns, int a, int leaf_size,
+                              Visit&& visit)
 {
     if (kv.n == 1) return true;
-    // buckets fit the window searches when leaf_size <= NEAR_MAX; larger nodes, or a larger
-    // leaf_size, take the full searches
+    // the no-node case asks whether a's cell holds more than leaf_size bodies: the window
+    // searches answer that for leaf_size <= NEAR_MAX, the full searches otherwise
     const bool near = leaf_size <= DView::NEAR_MAX;
     const auto exceeds = [&](int m) {
         return near ? dv.run_exceeds(a, m, leaf_size) : dv.next_lt(a, m) - dv.prev_lt(a - 1, m) > leaf_size;
@@ -782,15 +893,11 @@ __device__ bool walk_position(const KeyView& kv, const DView& dv, int a, int lea
         // no node starts at a: the deepest node holding a starts before it, at level d(a-1)
         return dl < LEVELS && exceeds(dl);
     }
-    int lo = a;  // cpl(a, lo) >= m
     bool deepest_is_bucket = false, first = true;
-    bool held = false;      // a bucket seen but not yet emitted (its parent decides)
-    bool internal = false;  // an internal node seen: every shallower node is internal too
+    bool held = false;  // a bucket seen but not yet emitted (its parent decides)
     int held_l = 0, held_b = 0;
+    int b = int(ns[a]);  // m > dl >= -1 puts a below n - 1
     while (m > dl) {
-        // the node's last position, where d drops below m: a bucket ends within a + leaf_size
-        int b = near && !internal ? dv.next_lt_near(lo, m, a + leaf_size - 1) : -1;
-        if (b < 0) b = dv.next_lt(near && !internal ? max(lo, a + leaf_size) : lo, m);
         const bool bucket = (b - a + 1) <= leaf_size || m == LEVELS;
         if (first) {
             deepest_is_bucket = bucket;
@@ -801,15 +908,16 @@ __device__ bool walk_position(const KeyView& kv, const DView& dv, int a, int lea
             held_l = m;
             held_b = b;
         } else {
-            internal = true;
             if (held) {   // its parent is this internal node: keep it
                 visit(held_l, held_b, true);
                 held = false;
             }
             visit(m, b, false);
         }
-        lo = b + 1;
+        // both loads at once; ns[b] counts only when d(b) > dl, which puts b below n - 1
+        const int nb = int(ns[min(b, kv.n - 2)]);
         m = dv.d(b);
+        b = nb;
     }
     if (held) {
         // the shallowest node at a is a bucket; its parent starts before a, at level dl
@@ -819,8 +927,7 @@ __device__ bool walk_position(const KeyView& kv, const DView& dv, int a, int lea
 }
 
 // The nodes a position keeps, as its walk met them (deepest first), for the emit pass: up to
-// VC_MAX entries of {last position b, level | bucket << 8} per position, in memory the FP64
-// moments held until their prefix sums were taken (32 B per position).
+// VC_MAX entries of {last position b, level | bucket << 8} per position (32 B).
 constexpr unsigned VC_MAX = 4;
 struct VisitCache {
     uint4* v;  // 2 per position
@@ -830,13 +937,14 @@ struct VisitCache {
 
 // cnt[a]: entries position a adds to the pre-order (kept nodes + leaf); kept[a]: nodes only
 __global__ __launch_bounds__(256) void k_count(const unsigned long long* keys, int n, int leaf_size,
-                                               unsigned* cnt, unsigned char* kept, DView dv, VisitCache vc)
+                                               unsigned* cnt, unsigned char* kept, DView dv, const unsigned* ns,
+                                               VisitCache vc)
 {
     const int a = blockIdx.x * 256 + threadIdx.x;
     if (a >= n) return;
     unsigned nodes = 0;
     uint4 lo = make_uint4(0u, 0u, 0u, 0u), hi = lo;
-    const bool leaf = walk_position(KeyView{ keys, n }, dv, a, leaf_size, [&](int lvl, int b, bool bucket) {
+    const bool leaf = walk_position(KeyView{ keys, n }, dv, ns, a, leaf_size, [&](int lvl, int b, bool bucket) {
         const unsigned m = VisitCache::meta(lvl, bucket);
         switch (nodes) {
         case 0: lo.x = unsigned(b); lo.y = m; break;
@@ -867,6 +975,7 @@ struct EmitArgs {
     BhNode* nodes;
     unsigned capacity;     // nodes allocated; entries past it are never written
     DView dv;
+    const unsigned* ns;
     VisitCache vc;
 };
 
@@ -879,9 +988,8 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
     const unsigned kept = e.kept[a];
     const bool leaf_kept = e.base[a + 1] - base > kept;
     const float theta_sq = e.theta * e.theta;
-    const auto emit = [&](unsigned slot, int lvl, int b, bool bucket) {
+    const auto make = [&](unsigned slot, int lvl, int b, bool bucket, const Dbl4& lo, const Dbl4& hi, unsigned next) {
         if (slot >= e.capacity) return;
-        const Dbl4 hi = e.pref[b + 1], lo = e.pref[a];
         const double M = hi.m - lo.m;
         BhNode nd;
         if (M != 0.0)
@@ -893,7 +1001,7 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
         for (int l = 0; l < lvl; ++l) s = s * .5f;
         const float s2 = s * s;
         nd.thr = e.opening == NBHIP_OPEN_TEXTBOOK ? s2 / theta_sq : s2 * theta_sq;
-        nd.next = e.base[b + 1];
+        nd.next = next;
         nd.first = unsigned(a);
         nd.count = bucket ? unsigned(b - a + 1) : 0u;
         e.nodes[slot] = nd;
@@ -902,15 +1010,29 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
     // shallow, so the j-th one met goes to base + kept - 1 - j
     if (kept > VC_MAX) {
         unsigned next_slot = base + kept;
-        walk_position(KeyView{ e.keys, e.n }, e.dv, a, e.leaf_size,
-                      [&](int lvl, int b, bool bucket) { emit(--next_slot, lvl, b, bucket); });
+        const Dbl4 lo = e.pref[a];
+        walk_position(KeyView{ e.keys, e.n }, e.dv, e.ns, a, e.leaf_size, [&](int lvl, int b, bool bucket) {
+            make(--next_slot, lvl, b, bucket, lo, e.pref[b + 1], e.base[b + 1]);
+        });
     } else if (kept > 0u) {
-        const uint4 lo = e.vc.v[2 * size_t(a)];
-        const uint4 hi = kept > 2u ? e.vc.v[2 * size_t(a) + 1] : make_uint4(0u, 0u, 0u, 0u);
-        const unsigned bs[VC_MAX] = { lo.x, lo.z, hi.x, hi.z }, ms[VC_MAX] = { lo.y, lo.w, hi.y, hi.w };
+        // every load first, then the nodes
+        const uint4 v0 = e.vc.v[2 * size_t(a)];
+        const uint4 v1 = kept > 2u ? e.vc.v[2 * size_t(a) + 1] : make_uint4(0u, 0u, 0u, 0u);
+        const unsigned bs[VC_MAX] = { v0.x, v0.z, v1.x, v1.z }, ms[VC_MAX] = { v0.y, v0.w, v1.y, v1.w };
+        const Dbl4 lo = e.pref[a];
+        Dbl4 hi[VC_MAX];
+        unsigned next[VC_MAX];
+#pragma unroll
+        for (unsigned j = 0; j < VC_MAX; ++j) {
+            if (j < kept) {
+                hi[j] = e.pref[bs[j] + 1];
+                next[j] = e.base[bs[j] + 1];
+            }
+        }
 #pragma unroll
         for (unsigned j = 0; j < VC_MAX; ++j)
-            if (j < kept) emit(base + kept - 1u - j, int(ms[j] & 255u), int(bs[j]), (ms[j] & 256u) != 0u);
+            if (j < kept)
+                make(base + kept - 1u - j, int(ms[j] & 255u), int(bs[j]), (ms[j] & 256u) != 0u, lo, hi[j], next[j]);
     }
     if (leaf_kept) {
         const unsigned slot = base + kept;
@@ -1270,7 +1392,7 @@ void bh_reserve(nbhip_ctx* c)
     c->bh_vals[0].grow(n, 0, s);
     c->bh_vals[1].grow(n, 0, s);
     c->bh_spm.grow(n, 0, s);
-    c->bh_moments.grow(n, 0, s);
+    c->bh_vcache.grow(2 * size_t(n), 0, s);
     c->bh_pref.grow(size_t(n) + 1, 0, s);
     c->bh_base.grow(size_t(n) + 1, 0, s);
     c->bh_nodes.grow(2 * size_t(n) + 1, 0, s);
@@ -1303,7 +1425,7 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     c->bh_vals[0].grow(n, 0, s);
     c->bh_vals[1].grow(n, 0, s);
     c->bh_spm.grow(n, 0, s);
-    c->bh_moments.grow(n, 0, s);
+    c->bh_vcache.grow(2 * size_t(n), 0, s);
     c->bh_pref.grow(size_t(n) + 1, 0, s);
     c->bh_base.grow(size_t(n) + 1, 0, s);
     c->bh_nodes.grow(2 * size_t(n) + 1, 0, s);
@@ -1320,33 +1442,44 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     const int tiles = std::max(1, (n + SC_TILE - 1) / SC_TILE);
     c->bh_dscan.grow(size_t(tiles), 0, s);
     hipLaunchKernelGGL(k_gather, dim3(tiles), dim3(SC_BLOCK), 0, s, perm, c->pm[c->cur].ptr, n, c->bh_spm.ptr,
-                       c->bh_moments.ptr, c->bh_dscan.ptr);
+                       c->bh_dscan.ptr);
     NBHIP_CHECK(hipGetLastError());
-    exclusive_scan_apply<Dbl4>(c, c->bh_moments.ptr, n, c->bh_pref.ptr, c->bh_dscan.ptr);
+    exclusive_scan_apply<Dbl4>(c, MomentOf{ c->bh_spm.ptr }, n, c->bh_pref.ptr, c->bh_dscan.ptr);
 
     const LevelSizes ls(n);
     c->bh_levels.grow(ls.total, 0, s);
     DView dv{};
-    for (int l = 0; l < LV_LEVELS; ++l) {
-        dv.L[l] = c->bh_levels.ptr + ls.off[l];
-        dv.len[l] = ls.len[l];
+    LevelsTop lt{};
+    dv.levels = lt.levels = ls.levels;
+    for (int l = 0; l < ls.levels; ++l) {
+        dv.L[l] = lt.L[l] = c->bh_levels.ptr + ls.off[l];
+        dv.len[l] = lt.len[l] = ls.len[l];
     }
     hipLaunchKernelGGL(k_levels, dim3(ls.nb), dim3(256), 0, s, keys, n, c->bh_levels.ptr + ls.off[0],
                        c->bh_levels.ptr + ls.off[1], c->bh_levels.ptr + ls.off[2], c->bh_levels.ptr + ls.off[3],
                        ls.len[3]);
     NBHIP_CHECK(hipGetLastError());
+    if (ls.levels > 4) {
+        hipLaunchKernelGGL(k_levels_top, dim3(1), dim3(256), 0, s, lt);
+        NBHIP_CHECK(hipGetLastError());
+    }
 
     const int leaf = std::max(1, p.leaf_size);
-    // the moments are spent once their prefix sums exist: their memory holds the visits
-    static_assert(sizeof(Dbl4) == 2 * sizeof(uint4), "two cache words per position");
-    const VisitCache vc{ reinterpret_cast<uint4*>(c->bh_moments.ptr) };
+    const VisitCache vc{ c->bh_vcache.ptr };
+    // the sort's other value buffer is free once the sort is done: it holds the next-smaller
+    // positions
+    unsigned* ns = c->bh_vals[1 - sorted].ptr;
+    if (n > 1) {
+        hipLaunchKernelGGL(k_next_smaller, dim3(grid(n - 1)), dim3(256), 0, s, n, dv, ns);
+        NBHIP_CHECK(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_count, dim3(grid(n)), dim3(256), 0, s, keys, n, leaf, c->bh_base.ptr,
-                       c->bh_kept.ptr, dv, vc);
+                       c->bh_kept.ptr, dv, ns, vc);
     NBHIP_CHECK(hipGetLastError());
     exclusive_scan<unsigned>(c, c->bh_base.ptr, n, c->bh_base.ptr, c->bh_uscan);
 
     EmitArgs e{ keys, n, leaf, c->bh_base.ptr, c->bh_kept.ptr, c->bh_pref.ptr, c->bh_spm.ptr, p.size,
-                p.theta, p.opening, c->bh_nodes.ptr, unsigned(c->bh_nodes.cap), dv, vc };
+                p.theta, p.opening, c->bh_nodes.ptr, unsigned(c->bh_nodes.cap), dv, ns, vc };
     hipLaunchKernelGGL(k_emit, dim3(grid(n)), dim3(256), 0, s, e);
     NBHIP_CHECK(hipGetLastError());
     c->bh_built = true;
@@ -1452,7 +1585,7 @@ void bh_release(nbhip_ctx* c)
         c->bh_vals[i].release();
     }
     c->bh_spm.release();
-    c->bh_moments.release();
+    c->bh_vcache.release();
     c->bh_pref.release();
     c->bh_base.release();
     c->bh_nodes.release();

@@ -146,7 +146,7 @@ struct nbhip_ctx {
     nbhip::DevBuf<unsigned long long> bh_keys[2];
     nbhip::DevBuf<unsigned> bh_vals[2];
     nbhip::DevBuf<float4> bh_spm;
-    nbhip::DevBuf<nbhip::Dbl4> bh_moments;
+    nbhip::DevBuf<uint4> bh_vcache;        // the nodes k_count met per sorted position, for k_emit
     nbhip::DevBuf<nbhip::Dbl4> bh_pref;
     nbhip::DevBuf<unsigned> bh_base;
     nbhip::DevBuf<unsigned char> bh_kept;

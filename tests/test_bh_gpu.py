@@ -367,3 +367,22 @@ def test_full_size_c3(ctx, orc):
     assert eq.mean() >= 0.999
     own = rel_err(x64[:, 8:11], ref[:, 8:11])   # the reference's own rounding
     assert np.all(e[eq] <= own[eq] + 1e-4)
+
+
+def test_more_than_2p20_bodies(ctx, orc):
+    # past 2^20 sorted positions the nearest-smaller searches gain levels above the
+    # 4096-position minima: forces of leaf size 1 against the FP64-centre oracle
+    n = 1_250_000
+    size = 128.0
+    b = ics.plummer(n, seed=77)
+    ctx.upload_bodies(b)
+    p = bh_params(size=size, theta=0.5, eps=0.01, leaf_size=1)
+    visits, total = ctx.bh_interactions(p)
+    out = ctx.download_bodies()
+    x64 = b.copy()
+    _, xv = orc.bh_accelerate_f64com(x64, size, theta=0.5, eps=0.01, visits=True)
+    e64 = rel_err(out[:, 8:11], x64[:, 8:11])
+    eq64 = visits == xv
+    print(f"n={n}: same traversal {eq64.mean():.5f}, max {e64[eq64].max():.2e}")
+    assert eq64.mean() >= 0.9999 and e64[eq64].max() <= 1e-4
+    assert total == visits.sum()
