@@ -162,11 +162,27 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_apply(Src in, int n, const T*
     }
     T total;
     T run = before + block_exclusive_scan(sum, sh, total);
-    for (int k = 0; k < SC_IPT; ++k) {
-        if (base + k < n) out[base + k] = run;
-        run = run + v[k];
+    if constexpr (sizeof(T) >= 16) {
+        // wide items leave through LDS, so that each store instruction covers consecutive
+        // items across the wave instead of one thread's run of SC_IPT
+        __shared__ T stage[SC_TILE];
+        for (int k = 0; k < SC_IPT; ++k) {
+            stage[threadIdx.x * SC_IPT + k] = run;
+            run = run + v[k];
+        }
+        if (base <= n - 1 && n - 1 < base + SC_IPT) out[n] = run;  // items past n - 1 are zero
+        __syncthreads();
+        for (int k = 0; k < SC_IPT; ++k) {
+            const int i = k * SC_BLOCK + threadIdx.x;
+            if (blockIdx.x * SC_TILE + i < n) out[blockIdx.x * SC_TILE + i] = stage[i];
+        }
+    } else {
+        for (int k = 0; k < SC_IPT; ++k) {
+            if (base + k < n) out[base + k] = run;
+            run = run + v[k];
+        }
+        if (base <= n - 1 && n - 1 < base + SC_IPT) out[n] = run;  // the thread holding the last item
     }
-    if (base <= n - 1 && n - 1 < base + SC_IPT) out[n] = run;  // the thread holding the last item
 }
 
 // The whole scan in one block, for short arrays where launches cost more than bytes:
