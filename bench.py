@@ -310,13 +310,15 @@ def run_served(args) -> dict:
         rate = 0.7 * capacity
         serve.poisson_arrivals(jobs, rate=rate, seed=SEED)
         rep = srv.run(jobs)
+        leaves = {f"2^{c}": leaf for c, leaf in sorted(srv.leaf_by_class.items())}
     s = rep.summary()
     n_direct = sum(1 for j in jobs if j.mode == 0)
     return {
         "workload": f"{count} jobs: Plummer or disk+bulge, N log-uniform in [1024, 262144], steps "
                     "uniform in [10, 1000]; direct sum below 16384 bodies (batched), Barnes-Hut "
-                    "theta 0.5 leaf 16 above (a context each, 8 concurrent, chunks of 16 steps "
-                    "replayed from captured graphs); bodies drawn on the device at admission; "
+                    "theta 0.5 above, leaf size autotuned per power-of-two size class (a context "
+                    "each, 8 concurrent, chunks of 16 steps replayed from captured graphs); bodies "
+                    "drawn on the device at admission; "
                     "200 GB budget; "
                     f"Poisson arrivals at 0.7x the measured capacity ({rate:.1f} jobs/s)",
         "jobs": s["jobs"], "direct_jobs": n_direct, "barnes_hut_jobs": count - n_direct,
@@ -329,6 +331,7 @@ def run_served(args) -> dict:
         "saturated_steps_per_s": sat.job_steps / sat.wall_s,
         "max_concurrent_jobs": s["max_concurrent_jobs"],
         "peak_device_bytes": s["peak_device_bytes"],
+        "bh_leaf_by_size_class": leaves,
     }
 
 

@@ -194,6 +194,21 @@ typedef struct nbhip_tune_entry {
 int nbhip_tune(nbhip_ctx* ctx, const nbhip_params* p, nbhip_tune_entry* out, size_t cap,
                size_t* count, nbhip_tune_entry* best);
 
+/* ---- autotuner for the Barnes-Hut leaf size -------------------------------------------
+ * No counterpart in the reference, whose octree holds one body per leaf
+ * (source/bhtree.cpp:14-95); leaf_size > 1 is this solver's bucket extension. */
+
+/* Time a Barnes-Hut force pass (tree build and traversal, no integration) at each leaf
+ * size in leaf_sizes[0..count) on the current bodies, with the other fields of *p: a
+ * warm pass, then the mean of three.  leaf_sizes NULL takes 1, 2, 4, 8, 12, 16, 24, 32
+ * (count is then the capacity of ms, at least 8 when ms is not NULL).  ms (may be NULL)
+ * receives the time of each, *best (may be NULL) the fastest leaf size.  Positions and
+ * velocities do not move; accelerations are those of the last pass.  Forces differ
+ * between leaf sizes within the Barnes-Hut error, so a caller fixes the leaf size it
+ * picks for a whole run.  Synchronises. */
+int nbhip_bh_tune(nbhip_ctx* ctx, const nbhip_params* p, const int* leaf_sizes, size_t count, float* ms,
+                  int* best);
+
 /* ---- diagnostics (FP64 reductions on the device) --------------------------------- */
 
 /* out[0] kinetic energy, out[1] potential energy (softened, with the radius cutoff),

@@ -114,6 +114,7 @@ SIGNATURES = {
     "nbhip_readback": (C.c_int, [_vp]),
     "nbhip_collect_bodies": (C.c_int, [_vp, _vp, _sz, C.POINTER(C.c_int)]),
     "nbhip_tune": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, C.POINTER(_sz), C.POINTER(TuneEntry)]),
+    "nbhip_bh_tune": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, _vp, C.POINTER(C.c_int)]),
     "nbhip_diagnostics": (C.c_int, [_vp, C.POINTER(Params), _vp]),
     "nbhip_force_error_f64": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, _vp, _vp]),
     "nbhip_bh_keys": (C.c_int, [_vp, C.POINTER(Params), _vp, _vp]),
@@ -327,6 +328,21 @@ class Context:
         self._check(self.L.nbhip_tune(self.h, C.byref(p), arr, cap, C.byref(cnt), C.byref(best)))
         rows = [(e.P, e.W, e.jsplit, e.ms) for e in arr[:cnt.value]]
         return rows, (best.P, best.W, best.jsplit, best.ms)
+
+    BH_TUNE_DEFAULT = (1, 2, 4, 8, 12, 16, 24, 32)
+
+    def bh_tune(self, p: Params, leaf_sizes=None) -> tuple[list[int], np.ndarray, int]:
+        """Time a Barnes-Hut force pass at each leaf size (None: the library's defaults);
+        returns (leaf sizes, ms per pass, fastest leaf size)."""
+        best = C.c_int()
+        if leaf_sizes is None:
+            ms = np.zeros(len(self.BH_TUNE_DEFAULT), np.float32)
+            self._check(self.L.nbhip_bh_tune(self.h, C.byref(p), None, ms.size, _ptr(ms), C.byref(best)))
+            return list(self.BH_TUNE_DEFAULT), ms, best.value
+        ls = np.ascontiguousarray(leaf_sizes, np.int32)
+        ms = np.zeros(ls.size, np.float32)
+        self._check(self.L.nbhip_bh_tune(self.h, C.byref(p), _ptr(ls), ls.size, _ptr(ms), C.byref(best)))
+        return [int(v) for v in ls], ms, best.value
 
     # ---- diagnostics ----
     def diagnostics(self, p: Params) -> np.ndarray:

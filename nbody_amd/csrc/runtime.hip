@@ -648,6 +648,55 @@ int nbhip_bh_interactions(nbhip_ctx* c, const nbhip_params* p, uint32_t* per_bod
     });
 }
 
+int nbhip_bh_tune(nbhip_ctx* c, const nbhip_params* p, const int* leaf_sizes, size_t count, float* ms, int* best)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        check_params(p);
+        static const int defaults[] = { 1, 2, 4, 8, 12, 16, 24, 32 };
+        const int* sizes = leaf_sizes ? leaf_sizes : defaults;
+        const size_t n_sizes = leaf_sizes ? count : sizeof defaults / sizeof defaults[0];
+        if (!leaf_sizes && ms && count < n_sizes)
+            fail(NBHIP_ERR_INVALID, "ms holds %zu timings, the default leaf sizes need %zu", count, n_sizes);
+        for (size_t k = 0; k < n_sizes; ++k)
+            if (sizes[k] < 1) fail(NBHIP_ERR_INVALID, "leaf size %d must be >= 1", sizes[k]);
+        if (best) *best = p->leaf_size;
+        if (c->count == 0 || n_sizes == 0) return;
+        ++c->body_version;  // accelerations change
+        NBHIP_CHECK(hipSetDevice(c->device));
+        hipEvent_t e0, e1;
+        NBHIP_CHECK(hipEventCreate(&e0));
+        NBHIP_CHECK(hipEventCreate(&e1));
+        float best_ms = 0.f;
+        try {
+            for (size_t k = 0; k < n_sizes; ++k) {
+                nbhip_params q = *p;
+                q.mode = NBHIP_MODE_NLOGN;
+                q.leaf_size = sizes[k];
+                nbhip::bh_accelerate(c, q, false, false);  // warm: buffers, caches
+                NBHIP_CHECK(hipEventRecord(e0, c->stream));
+                for (int r = 0; r < 3; ++r) nbhip::bh_accelerate(c, q, false, false);
+                NBHIP_CHECK(hipEventRecord(e1, c->stream));
+                NBHIP_CHECK(hipEventSynchronize(e1));
+                float t = 0.f;
+                NBHIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+                t /= 3.f;
+                if (ms) ms[k] = t;
+                if (k == 0 || t < best_ms) {
+                    best_ms = t;
+                    if (best) *best = sizes[k];
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        NBHIP_CHECK(hipEventDestroy(e0));
+        NBHIP_CHECK(hipEventDestroy(e1));
+    });
+}
+
 int nbhip_timer_start(nbhip_ctx* c)
 {
     return guarded(c, [&] {

@@ -104,8 +104,13 @@ class _BhSlot:
 class Server:
     def __init__(self, hbm_budget: float = 200e9, bh_contexts: int = 8, chunk: int = 16,
                  direct_below: int = 16384, keep_results: bool = False, batch_shape=None,
-                 bh_graph: bool = True):
+                 bh_graph: bool = True, bh_leaf="auto"):
         self.budget = hbm_budget
+        # Barnes-Hut leaf size: a number for every job, or "auto": the autotuner
+        # (nbhip_bh_tune) times the candidates on the first job of each power-of-two size
+        # class, on that job's own bodies, and the class keeps its pick for the run.
+        self.bh_leaf = bh_leaf
+        self.leaf_by_class: dict[int, int] = {}
         self.chunk = chunk
         # Barnes-Hut chunks replayed from a captured graph (nbhip_step_graph): one launch per
         # chunk instead of ~20 per step; captured again when a chunk's length changes.
@@ -129,6 +134,16 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+    def _leaf_for(self, job: Job, ctx: _lib.Context) -> int:
+        if self.bh_leaf != "auto":
+            return int(self.bh_leaf)
+        cls = max(0, int(job.n).bit_length() - 1)
+        if cls not in self.leaf_by_class:
+            # the accelerations the tuner leaves behind are recomputed by the job's first step
+            _, _, best = ctx.bh_tune(job.params)
+            self.leaf_by_class[cls] = best
+        return self.leaf_by_class[cls]
 
     # ---- the loop ------------------------------------------------------------------------
     def run(self, jobs: list[Job], clock=time.perf_counter) -> Report:
@@ -204,6 +219,7 @@ class Server:
                         slot.ctx.upload_bodies(job.bodies)
                     else:
                         slot.ctx.generate(job.ics, job.n)
+                    job.params.leaf_size = self._leaf_for(job, slot.ctx)
                     _, inter = slot.ctx.bh_interactions(job.params)
                     job.interactions_per_step = float(inter)
                     slot.job, slot.remaining, slot.inflight = job, job.steps, 0

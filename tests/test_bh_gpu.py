@@ -386,3 +386,22 @@ def test_more_than_2p20_bodies(ctx, orc):
     print(f"n={n}: same traversal {eq64.mean():.5f}, max {e64[eq64].max():.2e}")
     assert eq64.mean() >= 0.9999 and e64[eq64].max() <= 1e-4
     assert total == visits.sum()
+
+
+def test_leaf_tuner(ctx):
+    b = ics.disk_bulge(50000, seed=12)
+    ctx.upload_bodies(b)
+    p = bh_params(size=128.0, theta=0.5, eps=0.01, leaf_size=16)
+    sizes, ms, best = ctx.bh_tune(p)
+    print("leaf sizes", sizes, "ms", np.round(ms, 4), "best", best)
+    assert sizes == list(ctx.BH_TUNE_DEFAULT) and np.all(ms > 0)
+    assert best == sizes[int(np.argmin(ms))]
+    # positions do not move, and the accelerations are the last pass's
+    tuned = ctx.download_bodies()
+    assert np.array_equal(tuned[:, 0:8].view(np.uint32), b[:, 0:8].view(np.uint32))
+    ctx.accelerate(bh_params(size=128.0, theta=0.5, eps=0.01, leaf_size=sizes[-1]))
+    assert np.array_equal(ctx.download_bodies().view(np.uint32), tuned.view(np.uint32))
+    sizes2, ms2, best2 = ctx.bh_tune(p, [8, 3])
+    assert sizes2 == [8, 3] and ms2.size == 2 and best2 in (8, 3)
+    with pytest.raises(_lib.HipError):
+        ctx.bh_tune(p, [4, 0])
