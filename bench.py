@@ -46,7 +46,7 @@ def cpu_baseline(bodies: np.ndarray, eps: float) -> dict:
     """Oracle restatement of the reference brute-force loop (source/solvers/
     cpu_brute_force.h:26-40) on a seeded subset of rows x all N columns."""
     from oracle import oracle as O  # the CPU baseline leg is the only oracle user here
-    if not os.path.exists(O.LIB_PATH):
+    if not (os.path.exists(O.LIB_PATH) and os.path.exists(O.LIB_O3_PATH)):
         O.build(ref=False)
     threads = max(1, min(CPU_THREADS_MAX, os.cpu_count() or 1))
     n = bodies.shape[0]
@@ -69,9 +69,18 @@ def cpu_baseline(bodies: np.ndarray, eps: float) -> dict:
     O.brute_force_rows(work, sel, G=1.0, eps=eps, threads=threads)
     t = time.perf_counter() - t0
     rate = rows * n / t
+    # the same rows with the -O3 -march=x86-64-v3 build
+    O.variant("o3")
+    try:
+        t0 = time.perf_counter()
+        O.brute_force_rows(work, sel, G=1.0, eps=eps, threads=threads)
+        t3 = time.perf_counter() - t0
+    finally:
+        O.variant("o2")
     return {"value": rate / 1e9, "unit": "G-interactions/s", "cores": threads, "kind": "port",
             "sample": f"{rows} seeded rows x all {n} bodies, 1 force pass, {t:.1f} s, "
-                      f"oracle/nbody_oracle.c brute force (-O2, {threads} threads)"}
+                      f"oracle/nbody_oracle.c brute force (-O2, {threads} threads)",
+            "o3_march_x86_64_v3": {"value": rows * n / t3 / 1e9, "seconds": t3}}
 
 
 def load_traffic(name: str):
@@ -227,17 +236,25 @@ def run_batch(args) -> dict:
         from oracle import oracle as O
         threads = max(1, min(CPU_THREADS_MAX, os.cpu_count() or 1))
         pick = np.random.default_rng(SEED).choice(len(systems), 64, replace=False)
-        t0 = time.perf_counter()
-        work = 0.0
-        for k in pick:
-            b = systems[k].copy()
-            O.brute_force(b, eps=0.01, threads=threads)
-            work += float(b.shape[0]) ** 2
-        t = time.perf_counter() - t0
+        def timed(variant: str) -> tuple[float, float]:
+            O.variant(variant)
+            try:
+                t0 = time.perf_counter()
+                work = 0.0
+                for k in pick:
+                    b = systems[k].copy()
+                    O.brute_force(b, eps=0.01, threads=threads)
+                    work += float(b.shape[0]) ** 2
+                return work, time.perf_counter() - t0
+            finally:
+                O.variant("o2")
+        work, t = timed("o2")
+        _, t3 = timed("o3")
         out["cpu_baseline"] = {"value": work / t / 1e9, "unit": "G-interactions/s", "cores": threads,
                                "kind": "port",
                                "sample": f"one full force pass of 64 of the 512 systems (seeded pick), "
-                                         f"{t:.1f} s"}
+                                         f"{t:.1f} s, -O2",
+                               "o3_march_x86_64_v3": {"value": work / t3 / 1e9, "seconds": t3}}
     return out
 
 
@@ -279,18 +296,29 @@ def run_bh(args) -> dict:
     if not args.no_cpu:
         from oracle import oracle as O
         threads = max(1, min(CPU_THREADS_MAX, os.cpu_count() or 1))
-        b = bodies.copy()
-        t0 = time.perf_counter()
-        nodes = O.bh_build(b, size)
-        t_build = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        _, v = O.bh_accelerate(b, size, theta=0.5, eps=0.01, threads=threads, nodes=nodes, visits=True)
-        t_walk = time.perf_counter() - t0
+        def timed(variant: str):
+            O.variant(variant)
+            try:
+                b = bodies.copy()
+                t0 = time.perf_counter()
+                nodes = O.bh_build(b, size)
+                t_build = time.perf_counter() - t0
+                t0 = time.perf_counter()
+                _, v = O.bh_accelerate(b, size, theta=0.5, eps=0.01, threads=threads, nodes=nodes,
+                                       visits=True)
+                return t_build, time.perf_counter() - t0, v
+            finally:
+                O.variant("o2")
+        t_build, t_walk, v = timed("o2")
+        t3_build, t3_walk, _ = timed("o3")
         out["cpu_baseline"] = {"value": 1.0 / (t_build + t_walk), "unit": "steps/s", "cores": threads,
                                "kind": "port", "build_ms": t_build * 1e3, "walk_ms": t_walk * 1e3,
                                "interactions_per_step": int(v.sum()),
                                "sample": "1 force pass of the reference octree restatement "
-                                         "(serial build, leaf 1, threaded walk)"}
+                                         "(serial build, leaf 1, threaded walk), -O2",
+                               "o3_march_x86_64_v3": {"value": 1.0 / (t3_build + t3_walk),
+                                                      "build_ms": t3_build * 1e3,
+                                                      "walk_ms": t3_walk * 1e3}}
     return out
 
 

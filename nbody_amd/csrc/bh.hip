@@ -48,6 +48,7 @@
 // 48-byte node layout, for Solver::tree().
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -145,12 +146,20 @@ struct MomentOf {
     }
 };
 
+// part holds the totals of consecutive runs of SC_TILE / parts_per_tile items.
 template <typename T, typename Src>
-__global__ __launch_bounds__(SC_BLOCK) void k_scan_apply(Src in, int n, const T* part, T* out)
+__global__ __launch_bounds__(SC_BLOCK) void k_scan_apply(Src in, int n, const T* part, T* out, int parts_per_tile)
 {
     __shared__ T sh[SC_BLOCK];
     T pre{};
-    for (int b = threadIdx.x; b < int(blockIdx.x); b += SC_BLOCK) pre = pre + part[b];
+    const int before_parts = int(blockIdx.x) * parts_per_tile;
+    int b = threadIdx.x;
+    if constexpr (std::is_integral_v<T>) {
+        // four loads in flight per round (integer sums do not depend on the order)
+        for (; b + 3 * SC_BLOCK < before_parts; b += 4 * SC_BLOCK)
+            pre += (part[b] + part[b + SC_BLOCK]) + (part[b + 2 * SC_BLOCK] + part[b + 3 * SC_BLOCK]);
+    }
+    for (; b < before_parts; b += SC_BLOCK) pre = pre + part[b];
     T before;
     (void)block_exclusive_scan(pre, sh, before);
     const int base = blockIdx.x * SC_TILE + threadIdx.x * SC_IPT;
@@ -226,16 +235,18 @@ void exclusive_scan(nbhip_ctx* c, const T* in, int n, T* out, DevBuf<T>& part)
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(SC_BLOCK), 0, c->stream, in, n, part.ptr);
     NBHIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL((k_scan_apply<T, ArrayOf<T>>), dim3(nb), dim3(SC_BLOCK), 0, c->stream, ArrayOf<T>{ in }, n,
-                       part.ptr, out);
+                       part.ptr, out, 1);
     NBHIP_CHECK(hipGetLastError());
 }
 
-// The apply phase alone, for a producer that already wrote the tile totals.
+// The apply phase alone, for a producer that already wrote the totals of runs of
+// SC_TILE / parts_per_tile items.
 template <typename T, typename Src>
-void exclusive_scan_apply(nbhip_ctx* c, Src in, int n, T* out, const T* part)
+void exclusive_scan_apply(nbhip_ctx* c, Src in, int n, T* out, const T* part, int parts_per_tile = 1)
 {
     const int nb = std::max(1, (n + SC_TILE - 1) / SC_TILE);
-    hipLaunchKernelGGL((k_scan_apply<T, Src>), dim3(nb), dim3(SC_BLOCK), 0, c->stream, in, n, part, out);
+    hipLaunchKernelGGL((k_scan_apply<T, Src>), dim3(nb), dim3(SC_BLOCK), 0, c->stream, in, n, part, out,
+                       parts_per_tile);
     NBHIP_CHECK(hipGetLastError());
 }
 
@@ -954,27 +965,36 @@ struct VisitCache {
 // cnt[a]: entries position a adds to the pre-order (kept nodes + leaf); kept[a]: nodes only
 __global__ __launch_bounds__(256) void k_count(const unsigned long long* keys, int n, int leaf_size,
                                                unsigned* cnt, unsigned char* kept, DView dv, const unsigned* ns,
-                                               VisitCache vc)
+                                               VisitCache vc, unsigned* block_total)
 {
+    __shared__ unsigned wsum[4];
     const int a = blockIdx.x * 256 + threadIdx.x;
-    if (a >= n) return;
-    unsigned nodes = 0;
-    uint4 lo = make_uint4(0u, 0u, 0u, 0u), hi = lo;
-    const bool leaf = walk_position(KeyView{ keys, n }, dv, ns, a, leaf_size, [&](int lvl, int b, bool bucket) {
-        const unsigned m = VisitCache::meta(lvl, bucket);
-        switch (nodes) {
-        case 0: lo.x = unsigned(b); lo.y = m; break;
-        case 1: lo.z = unsigned(b); lo.w = m; break;
-        case 2: hi.x = unsigned(b); hi.y = m; break;
-        case 3: hi.z = unsigned(b); hi.w = m; break;
-        default: break;
-        }
-        ++nodes;
-    });
-    cnt[a] = nodes + (leaf ? 1u : 0u);
-    kept[a] = (unsigned char)nodes;
-    if (nodes > 0u) vc.v[2 * size_t(a)] = lo;
-    if (nodes > 2u) vc.v[2 * size_t(a) + 1] = hi;
+    unsigned entries = 0u;
+    if (a < n) {
+        unsigned nodes = 0;
+        uint4 lo = make_uint4(0u, 0u, 0u, 0u), hi = lo;
+        const bool leaf = walk_position(KeyView{ keys, n }, dv, ns, a, leaf_size, [&](int lvl, int b, bool bucket) {
+            const unsigned m = VisitCache::meta(lvl, bucket);
+            switch (nodes) {
+            case 0: lo.x = unsigned(b); lo.y = m; break;
+            case 1: lo.z = unsigned(b); lo.w = m; break;
+            case 2: hi.x = unsigned(b); hi.y = m; break;
+            case 3: hi.z = unsigned(b); hi.w = m; break;
+            default: break;
+            }
+            ++nodes;
+        });
+        entries = nodes + (leaf ? 1u : 0u);
+        cnt[a] = entries;
+        kept[a] = (unsigned char)nodes;
+        if (nodes > 0u) vc.v[2 * size_t(a)] = lo;
+        if (nodes > 2u) vc.v[2 * size_t(a) + 1] = hi;
+    }
+    // the block's total, so the scan of the counts needs only its apply phase
+    for (int off = 32; off > 0; off >>= 1) entries += __shfl_xor(entries, off);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = entries;
+    __syncthreads();
+    if (threadIdx.x == 0) block_total[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
 struct EmitArgs {
@@ -1415,7 +1435,7 @@ void bh_reserve(nbhip_ctx* c)
     c->bh_kept.grow(n, 0, s);
     c->bh_levels.grow(LevelSizes(n).total, 0, s);
     rs_reserve(c, n);
-    c->bh_uscan.grow(size_t(std::max(1, (n + SC_TILE - 1) / SC_TILE)), 0, s);
+    c->bh_uscan.grow(std::max(size_t(grid(n)), size_t(std::max(1, (n + SC_TILE - 1) / SC_TILE))), 0, s);
     c->bh_dscan.grow(size_t(std::max(1, (n + SC_TILE - 1) / SC_TILE)), 0, s);
 }
 
@@ -1489,10 +1509,17 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
         hipLaunchKernelGGL(k_next_smaller, dim3(grid(n - 1)), dim3(256), 0, s, n, dv, ns);
         NBHIP_CHECK(hipGetLastError());
     }
+    static_assert(SC_TILE % 256 == 0, "k_count blocks tile the scan tiles");
+    c->bh_uscan.grow(grid(n), 0, s);
     hipLaunchKernelGGL(k_count, dim3(grid(n)), dim3(256), 0, s, keys, n, leaf, c->bh_base.ptr,
-                       c->bh_kept.ptr, dv, ns, vc);
+                       c->bh_kept.ptr, dv, ns, vc, c->bh_uscan.ptr);
     NBHIP_CHECK(hipGetLastError());
-    exclusive_scan<unsigned>(c, c->bh_base.ptr, n, c->bh_base.ptr, c->bh_uscan);
+    if (n <= SC_SINGLE_MAX) {
+        exclusive_scan<unsigned>(c, c->bh_base.ptr, n, c->bh_base.ptr, c->bh_uscan);
+    } else {
+        exclusive_scan_apply<unsigned>(c, ArrayOf<unsigned>{ c->bh_base.ptr }, n, c->bh_base.ptr, c->bh_uscan.ptr,
+                                       SC_TILE / 256);
+    }
 
     EmitArgs e{ keys, n, leaf, c->bh_base.ptr, c->bh_kept.ptr, c->bh_pref.ptr, c->bh_spm.ptr, p.size,
                 p.theta, p.opening, c->bh_nodes.ptr, unsigned(c->bh_nodes.cap), dv, ns, vc };

@@ -18,6 +18,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+# The same source at -O3 -march=x86-64-v3 (identical results; the CPU baseline's second
+# column).  variant("o3") makes the functions below use it.
+LIB_O3_PATH = os.path.join(HERE, "liboracle_o3.so")
 REF_PATH = os.path.join(HERE, "_ref", "libnbref.so")
 
 NODE_DTYPE = np.dtype([
@@ -34,6 +37,8 @@ _u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
 _nodep = np.ctypeslib.ndpointer(NODE_DTYPE, flags="C_CONTIGUOUS")
 
 _lib = None
+_libs: dict = {}
+_path = LIB_PATH
 _ref = None
 
 # Worker threads when a caller passes threads=0: the host's cores, at most 16 (the CPU
@@ -46,19 +51,27 @@ def _t(threads: int) -> int:
 
 
 def build(ref: bool = True) -> None:
-    """Compile liboracle.so (and _ref/libnbref.so when the reference tree is present)."""
-    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    """Compile liboracle.so and liboracle_o3.so (and _ref/libnbref.so when the reference
+    tree is present)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so", "liboracle_o3.so"], check=True)
     if ref and os.path.isdir("/root/reference/source"):
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def variant(name: str = "o2") -> None:
+    """Select the build the functions below call: "o2" (the default) or "o3"."""
+    global _lib, _path
+    _path = {"o2": LIB_PATH, "o3": LIB_O3_PATH}[name]
+    _lib = _libs.get(_path)
 
 
 def lib():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError(f"oracle library not built: {LIB_PATH} (run make -C oracle)")
-    L = C.CDLL(LIB_PATH)
+    if not os.path.exists(_path):
+        raise RuntimeError(f"oracle library not built: {_path} (run make -C oracle)")
+    L = C.CDLL(_path)
     sz = C.c_size_t
     L.orc_hardware_threads.restype = C.c_int
     L.orc_gravity_pair.argtypes = [_f32p, C.c_float, _f32p, C.c_float, C.c_float, C.c_float, _f32p]
@@ -82,7 +95,7 @@ def lib():
     L.orc_generate.argtypes = [C.c_void_p, sz, _f32p, C.c_int]
     L.orc_generate.restype = C.c_int
     assert L.orc_node_size() == 48
-    _lib = L
+    _lib = _libs[_path] = L
     return L
 
 

@@ -16,7 +16,7 @@ def pytest_configure(config):
 def orc():
     """The CPU oracle (built on demand; test infrastructure only)."""
     from oracle import oracle as O
-    if not os.path.exists(O.LIB_PATH):
+    if not (os.path.exists(O.LIB_PATH) and os.path.exists(O.LIB_O3_PATH)):
         O.build(ref=False)
     O.lib()
     return O

@@ -9,6 +9,8 @@ import os
 import numpy as np
 import pytest
 
+from nbody_amd import ics
+
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
@@ -216,3 +218,22 @@ def test_bh_f64com_oracle(orc):
     orc.bh_accelerate_f64com(x, 64.0, theta=1e6, eps=0.01)
     a64 = orc.direct_f64(p, eps=0.01)
     assert (np.linalg.norm(x[:, 8:11] - a64, axis=1) / np.linalg.norm(a64, axis=1)).max() < 1e-6
+
+
+def test_o3_build_is_bit_identical(orc):
+    # the CPU baseline's second column (-O3 -march=x86-64-v3, contraction still off) must
+    # compute exactly what the -O2 oracle does
+    b = ics.disk_bulge(3000, seed=4)
+    runs = []
+    for v in ("o2", "o3"):
+        orc.variant(v)
+        try:
+            x = b.copy()
+            orc.brute_force(x, eps=0.01)
+            y = b.copy()
+            _, vis = orc.bh_accelerate(y, 128.0, theta=0.5, eps=0.01, visits=True)
+            runs.append((x, y, vis, orc.morton_keys(b, 128.0, 21)))
+        finally:
+            orc.variant("o2")
+    for a, c in zip(runs[0], runs[1]):
+        assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(c).view(np.uint8))
