@@ -1259,12 +1259,19 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a)
         for (; j < nb; ++j) interact_if<CUT>(opened, x, r2, a.spm[nd.first + j], a.eps2, acc.tx, acc.ty, acc.tz);
         if (COUNT) applied += (opened && nd.count > 1) ? nd.count : 0u;
         if (at) idx = max(nxt, cur + 1);  // pre-order pointers only move forward
-        // every pointer is now > cur: if a lane is on cur + 1, that is the minimum
+        // every pointer is now > cur: if a lane is on cur + 1, that is the minimum.  If not,
+        // every lane that sat on cur went to the same place, past cur's subtree; when no
+        // other lane waits before that point, it is the minimum too.
         const unsigned step = cur + 1;
-        if (__ballot(idx == step) != 0ull)
+        if (__ballot(idx == step) != 0ull) {
             cur = step;
-        else
-            cur = max(wave_min(idx), step);  // the minimum is >= step; max keeps cur moving
+        } else {
+            const unsigned skip = max(nd.next, step);
+            if (__ballot(idx < skip) == 0ull)
+                cur = skip;
+            else
+                cur = max(wave_min(idx), step);  // the minimum is >= step; max keeps cur moving
+        }
         if ((++steps & 7u) == 0u) acc.fold();
     }
     if (!valid) return;
