@@ -155,7 +155,11 @@ int nbhip_step(nbhip_ctx* ctx, const nbhip_params* p, int steps);
  * (small systems, Barnes-Hut's ~20 kernels per step). */
 int nbhip_step_graph(nbhip_ctx* ctx, const nbhip_params* p, int steps);
 
-/* Block until all work enqueued on the context has finished. */
+/* Block until all work enqueued on the context has finished.  Also reports, once, a
+ * Barnes-Hut radix-sort pass that gave up waiting for an earlier tile (NBHIP_ERR_HIP): the
+ * wait is bounded so that no kernel can hang, and the builds since the last call are then
+ * invalid.  (The wait only ends this way if the device stops running a block that has
+ * started, which a working device does not.) */
 int nbhip_synchronize(nbhip_ctx* ctx);
 
 /* Non-blocking: *idle = 1 when all work enqueued on the context has finished, else 0
