@@ -259,13 +259,15 @@ void exclusive_scan_apply(nbhip_ctx* c, Src in, int n, T* out, const T* part, in
 //
 // Sort state (bh_rs_state, unsigned words): the 8 x 256 digit counts, zero between sorts
 // (k_rs_bases clears them once read; zeroed once at allocation); the 8 x 256 digit starts;
-// 8 tile tickets, cleared by k_rs_bases; and a fault word.  bh_rs_status holds 8 x tiles x 256
-// look-back words, cleared by k_rs_count: bit 30 = tile count published, bit 31 = running
-// total (this and every earlier tile) published, low 30 bits = the value.
+// 8 tile tickets, cleared by k_rs_bases; and a fault word.  bh_rs_status holds, per pass,
+// 256 look-back words per tile and then per super-tile, cleared by k_rs_count: bit 30 =
+// count published, bit 31 = running total (this and every earlier super-tile) published,
+// low 30 bits = the value.
 constexpr int RS_PASSES = 8;
 constexpr int RS_COUNTS = 0, RS_BASES = RS_PASSES * 256, RS_TICKETS = 2 * RS_PASSES * 256,
               RS_FAULT = RS_TICKETS + RS_PASSES, RS_WORDS = RS_FAULT + 8;
 constexpr unsigned LB_COUNT = 1u << 30, LB_TOTAL = 1u << 31, LB_VALUE = LB_COUNT - 1u;
+constexpr unsigned RS_SUPER = 16;  // tiles per super-tile of the two-level look-back
 // Look-back waits end here, about a second in: the tile then takes zero for the unpublished
 // value and sets the fault word, which the next synchronisation reports; the sort is then
 // wrong but every block still finishes.
@@ -329,7 +331,7 @@ __device__ __forceinline__ unsigned lb_wait(unsigned* word, unsigned* fault)
 // (w + 1)) of the tile, 64 per round in index order: lanes holding the same digit are found
 // with 8 ballots, and the wave's own LDS counters give each item its rank among the wave's
 // earlier items of that digit.  The waves' counts then give every wave its start within each
-// digit, and the tile's counts go to the look-back, 8 earlier tiles per read.
+// digit, and the tile's counts go to the two-level look-back below.
 template <int RB, int IPT, bool STAGE>
 __global__ __launch_bounds__(RB) void k_rs_pass(const unsigned long long* kin, const unsigned* vin,
                                                 unsigned long long* kout, unsigned* vout, int n, int shift,
@@ -338,6 +340,7 @@ __global__ __launch_bounds__(RB) void k_rs_pass(const unsigned long long* kin, c
 {
     static_assert(RB >= 256, "one thread per digit for the look-back");
     constexpr int WAVES = RB / 64, LB_WINDOW = 8;
+    constexpr unsigned LB_SUPER = RS_SUPER;
     __shared__ unsigned s_tile;
     __shared__ unsigned wh[WAVES][256];  // a wave's count of each digit, then its start in the tile's run
     __shared__ unsigned start[256];      // where the tile's run of each digit begins in the output
@@ -387,29 +390,56 @@ __global__ __launch_bounds__(RB) void k_rs_pass(const unsigned long long* kin, c
             wh[w][t] = cnt;
             cnt += c;
         }
-        unsigned* mine = status + size_t(tile) * 256 + t;
+        // Two levels: the tiles before this one in its super-tile of LB_SUPER tiles, whose
+        // counts are all published at about the same time (read at once), then the
+        // super-tiles before it by decoupled look-back.  The last tile of each super-tile
+        // publishes the super-tile's count and then its running total.  Every wait is for
+        // a tile with a smaller ticket.
+        const unsigned nb = gridDim.x;
+        unsigned* super_status = status + size_t(nb) * 256;
+        const unsigned sup = tile / LB_SUPER, first = sup * LB_SUPER;
+        const bool last = tile == min(first + LB_SUPER, nb) - 1u;
+        __hip_atomic_store(status + size_t(tile) * 256 + t, LB_COUNT | cnt, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        unsigned intra = 0u;
+        {
+            unsigned v[LB_SUPER - 1];
+#pragma unroll
+            for (unsigned j = 0; j < LB_SUPER - 1; ++j)
+                v[j] = first + j < tile ? lb_load(status + size_t(first + j) * 256 + t) : LB_COUNT;
+#pragma unroll
+            for (unsigned j = 0; j < LB_SUPER - 1; ++j) {
+                if (first + j >= tile) continue;
+                if (!(v[j] & (LB_COUNT | LB_TOTAL))) v[j] = lb_wait(status + size_t(first + j) * 256 + t, fault);
+                intra += v[j] & LB_VALUE;
+            }
+        }
+        unsigned* mine = super_status + size_t(sup) * 256 + t;
+        if (last)
+            __hip_atomic_store(mine, (sup == 0u ? LB_TOTAL : LB_COUNT) | ((intra + cnt) & LB_VALUE),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         unsigned before = 0u;
-        if (tile == 0u) {
-            __hip_atomic_store(mine, LB_TOTAL | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(mine, LB_COUNT | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            // tiles before 0 read as a zero total, so the walk ends at tile 0 at the latest
+        if (sup > 0u) {
+            // super-tiles before 0 read as a zero total, so the walk ends at 0 at the latest
             bool done = false;
-            for (int q = int(tile) - 1; !done; q -= LB_WINDOW) {
+            for (int q = int(sup) - 1; !done; q -= LB_WINDOW) {
                 unsigned v[LB_WINDOW];
 #pragma unroll
-                for (int j = 0; j < LB_WINDOW; ++j) v[j] = q - j >= 0 ? lb_load(status + size_t(q - j) * 256 + t) : LB_TOTAL;
+                for (int j = 0; j < LB_WINDOW; ++j)
+                    v[j] = q - j >= 0 ? lb_load(super_status + size_t(q - j) * 256 + t) : LB_TOTAL;
 #pragma unroll
                 for (int j = 0; j < LB_WINDOW; ++j) {
                     if (done) continue;
-                    if (!(v[j] & (LB_COUNT | LB_TOTAL))) v[j] = lb_wait(status + size_t(q - j) * 256 + t, fault);
+                    if (!(v[j] & (LB_COUNT | LB_TOTAL))) v[j] = lb_wait(super_status + size_t(q - j) * 256 + t, fault);
                     before += v[j] & LB_VALUE;
                     done = (v[j] & LB_TOTAL) != 0u;
                 }
             }
-            __hip_atomic_store(mine, LB_TOTAL | ((before + cnt) & LB_VALUE), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            if (last)
+                __hip_atomic_store(mine, LB_TOTAL | ((before + intra + cnt) & LB_VALUE), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
+        before += intra;
         start[t] = bases[t] + before;
         if (STAGE) {
             // the tile's digit runs in its own order: an exclusive scan of cnt over digits
@@ -502,6 +532,13 @@ int sort_blocks(int n)
     return std::max(1, (n + sh.rb * sh.ipt - 1) / (sh.rb * sh.ipt));
 }
 
+// Look-back words per pass: one row of 256 per tile, then one per super-tile.
+size_t rs_status_tiles(int n)
+{
+    const size_t nb = size_t(sort_blocks(n));
+    return nb + (nb + RS_SUPER - 1) / RS_SUPER;
+}
+
 constexpr int RS_COUNT_BLOCK = 1024;
 // Blocks of k_rs_count: at 2^20 keys 64 measured fastest (0.617 ms per Barnes-Hut step,
 // against 0.620 with 32, 0.634 with 128 and 0.662 with 256: more blocks, more global adds).
@@ -515,7 +552,7 @@ void rs_reserve(nbhip_ctx* c, int n)
         c->bh_rs_state.grow(RS_WORDS, 0, c->stream);
         NBHIP_CHECK(hipMemsetAsync(c->bh_rs_state.ptr, 0, RS_WORDS * sizeof(unsigned), c->stream));
     }
-    c->bh_rs_status.grow(size_t(RS_PASSES) * 256 * sort_blocks(n), 0, c->stream);
+    c->bh_rs_status.grow(size_t(RS_PASSES) * 256 * rs_status_tiles(n), 0, c->stream);
 }
 
 // Sorts keys[src]/vals[src] over 63 key bits; returns the buffer index of the result.
@@ -527,7 +564,8 @@ int radix_sort(nbhip_ctx* c, int n, int src)
     const int nb = sort_blocks(n);
     rs_reserve(c, n);
     unsigned* st = c->bh_rs_state.ptr;
-    const size_t words = size_t(RS_PASSES) * 256 * nb;
+    const size_t per_pass = size_t(256) * rs_status_tiles(n);
+    const size_t words = RS_PASSES * per_pass;
     hipLaunchKernelGGL(k_rs_count<RS_COUNT_BLOCK>, dim3(rs_count_blocks(n)), dim3(RS_COUNT_BLOCK), 0, c->stream,
                        c->bh_keys[src].ptr, n, st + RS_COUNTS, c->bh_rs_status.ptr, words);
     NBHIP_CHECK(hipGetLastError());
@@ -537,7 +575,7 @@ int radix_sort(nbhip_ctx* c, int n, int src)
         const int dst = 1 - src;
         hipLaunchKernelGGL(sh.pass, dim3(nb), dim3(sh.rb), 0, c->stream, c->bh_keys[src].ptr, c->bh_vals[src].ptr,
                            c->bh_keys[dst].ptr, c->bh_vals[dst].ptr, n, 8 * pass, st + RS_BASES + 256 * pass,
-                           st + RS_TICKETS + pass, c->bh_rs_status.ptr + size_t(256) * nb * pass, st + RS_FAULT);
+                           st + RS_TICKETS + pass, c->bh_rs_status.ptr + per_pass * pass, st + RS_FAULT);
         NBHIP_CHECK(hipGetLastError());
         src = dst;
     }
