@@ -21,16 +21,20 @@
 //   1. k_keys      63-bit key per body: 21 levels of the reference's own float descent.
 //   2. radix sort  8 stable LSD passes of 8 bits over (key, body index) pairs: the digit
 //                  counts of all passes in one read, then one launch per pass that ranks
-//                  a tile, looks back over earlier tiles for its digit starts and scatters.
-//   3. k_gather    bodies in key order + per-body FP64 moments and their tile totals;
-//                  the scan's apply phase -> prefix sums, so a node's mass and centre
-//                  over bodies [a, b] are two lookups.
+//                  a tile, looks back over earlier tiles (two levels) for its digit
+//                  starts, and scatters through LDS.
+//   3. k_gather    bodies in key order and the tile totals of their FP64 moments; the
+//                  scan's apply phase -> prefix sums, so a node's mass and centre over
+//                  bodies [a, b] are two lookups.
 //   4. k_levels    d(k) = common prefix (in octal digits) of keys k and k+1, one byte per
-//                  position, with its minima over 16, 256 and 4096 positions.
-//   5. k_count /   the nodes that start at sorted position a are the prefix minima of d
-//      k_emit      to the right of a; each node's end is where d first drops below its
-//                  level, a nearest-smaller-value query on the levels (for a bucket, on
-//                  the two 16-position chunks it can span).  Pre-order is
+//                  position, with its minima over 16, 256 and 4096 positions (and more
+//                  levels past 2^20 positions, k_levels_top).
+//   5. k_next_smaller, k_count, k_emit
+//                  the nodes that start at sorted position a are the prefix minima of d
+//                  to the right of a; each node's end is where d first drops below its
+//                  level: ns[a] for the deepest, ns[b] after an end b for the next one,
+//                  ns being each position's nearest smaller value to the right (one
+//                  search over the levels per position).  Pre-order is
 //                  "for a = 0..n-1: nodes starting at a, shallow to deep, then leaf a",
 //                  so a scan of per-position counts gives every node's index, and a node
 //                  covering bodies [a, b] has next = base[b + 1].
