@@ -172,6 +172,34 @@ def run_direct(args) -> dict:
     return out
 
 
+def run_integrate(args) -> dict:
+    """The standalone semi-implicit Euler kick-drift (nbhip_integrate, k_integrate) on
+    N = 2^24 device-drawn Plummer bodies, with and without the torus wrap: 80 B of HBM
+    traffic per body (read position, velocity, acceleration; write position, velocity)
+    against the 8 TB/s peak.  Inside a step it is fused into the force pass; this is the
+    Solver::integrate path on its own."""
+    from nbody_amd import _lib
+    n = 1 << 24
+    reps = 20
+    ctx = _lib.Context(0)
+    ctx.generate(_lib.ics(_lib.IC_PLUMMER, seed=SEED), n)
+    out = {"workload": f"k_integrate alone, N = 2^24 Plummer bodies drawn on the device, {reps} launches",
+           "bodies": n, "bytes_per_body": 80}
+    for name, wrap in (("wrap", 1), ("no_wrap", 0)):
+        p = _lib.params(dt=1e-6, eps=0.01, size=10000.0, wrap=wrap)
+        ctx.integrate(p)  # warm
+        ctx.synchronize()
+        ctx.timer_start()
+        for _ in range(reps):
+            ctx.integrate(p)
+        ms, _ = ctx.timer_stop()
+        per = ms / reps
+        gbs = 80.0 * n / (per * 1e-3) / 1e9
+        out[name] = {"ms_per_launch": per, "GB/s": gbs, "frac_of_hbm_peak": gbs / 8000.0}
+    ctx.close()
+    return out
+
+
 def run_batch(args) -> dict:
     """512 independent Plummer systems (N log-uniform in [256, 32768], own seeds), eps 0.01,
     dt 1e-3: autotuned batched launch, 50 steps captured in a hipGraph and replayed."""
@@ -369,7 +397,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--only", choices=["direct", "batch", "bh", "served"], default=None,
+    ap.add_argument("--only", choices=["direct", "batch", "integrate", "bh", "served"], default=None,
                     help="run one workload (the headline fields then describe it)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -383,6 +411,8 @@ def main() -> int:
         out = {"metric": METRIC, "n_gpus": 1}
     if args.only in (None, "batch"):
         out["batched"] = run_batch(args)
+    if args.only in (None, "integrate"):
+        out["integrate"] = run_integrate(args)
     if args.only in (None, "bh"):
         out["barnes_hut"] = run_bh(args)
     if args.only in (None, "served"):

@@ -84,16 +84,37 @@ __global__ __launch_bounds__(256) void k_reduce(ForceArgs a, int jsplit)
     finish_body(a, i, a.pm[i], a.G * tot[0], a.G * tot[1], a.G * tot[2]);
 }
 
+// IPT bodies per thread, 256 apart, every load issued before the arithmetic; NT streams
+// the stores past the caches (the arrays are touched once per launch).
+template <int IPT, bool NT>
 __global__ __launch_bounds__(256) void k_integrate(float4* pm, float4* vr, const float4* acc, int n,
                                                    float dt, float size, int wrap)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    float4 x = pm[i], v = vr[i];
-    const float4 g = acc[i];
-    kick_drift(x, v, g.x, g.y, g.z, dt, size, wrap);
-    pm[i] = x;
-    vr[i] = v;
+    const int i0 = blockIdx.x * (256 * IPT) + threadIdx.x;
+    float4 x[IPT], v[IPT], g[IPT];
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+        const int i = i0 + 256 * r;
+        if (i < n) {
+            x[r] = pm[i];
+            v[r] = vr[i];
+            g[r] = acc[i];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+        const int i = i0 + 256 * r;
+        if (i >= n) continue;
+        kick_drift(x[r], v[r], g[r].x, g[r].y, g[r].z, dt, size, wrap);
+        if (NT) {
+            using F4 = float __attribute__((ext_vector_type(4)));
+            __builtin_nontemporal_store(F4{ x[r].x, x[r].y, x[r].z, x[r].w }, reinterpret_cast<F4*>(pm + i));
+            __builtin_nontemporal_store(F4{ v[r].x, v[r].y, v[r].z, v[r].w }, reinterpret_cast<F4*>(vr + i));
+        } else {
+            pm[i] = x[r];
+            vr[i] = v[r];
+        }
+    }
 }
 
 // Per-block maximum of |radius| -> out[blockIdx.x].
@@ -435,9 +456,27 @@ void tune_direct(nbhip_ctx* c, const nbhip_params& p, std::vector<nbhip_tune_ent
 
 void integrate(nbhip_ctx* c, const nbhip_params& p)
 {
+    using K = void (*)(float4*, float4*, const float4*, int, float, float, int);
+    struct Shape {
+        int ipt;
+        K k;
+    };
+    // bodies per thread and streaming stores (NBHIP_INTEGRATE_SHAPE = index overrides).  At
+    // 2^24 bodies all six lie within 4% (5.32-5.55 TB/s); one body per thread with
+    // streaming stores was the fastest.
+    static const Shape shapes[] = { { 1, k_integrate<1, true> },  { 2, k_integrate<2, true> },
+                                    { 4, k_integrate<4, true> },  { 1, k_integrate<1, false> },
+                                    { 2, k_integrate<2, false> }, { 4, k_integrate<4, false> } };
+    static const int pick = [] {
+        const char* e = std::getenv("NBHIP_INTEGRATE_SHAPE");
+        const int k = e ? std::atoi(e) : 0;
+        return (k >= 0 && k < int(sizeof shapes / sizeof shapes[0])) ? k : 0;
+    }();
+    const Shape& sh = shapes[pick];
     const size_t n = c->count;
-    hipLaunchKernelGGL(k_integrate, dim3(grid1(n)), dim3(256), 0, c->stream, c->pm[c->cur].ptr,
-                       c->vr.ptr, c->acc.ptr, int(n), p.dt, p.size, p.wrap);
+    const unsigned blocks = unsigned((n + 256 * sh.ipt - 1) / (256 * sh.ipt));
+    hipLaunchKernelGGL(sh.k, dim3(std::max(1u, blocks)), dim3(256), 0, c->stream, c->pm[c->cur].ptr, c->vr.ptr,
+                       c->acc.ptr, int(n), p.dt, p.size, p.wrap);
     NBHIP_CHECK(hipGetLastError());
 }
 
