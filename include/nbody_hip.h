@@ -213,6 +213,14 @@ int nbhip_tune(nbhip_ctx* ctx, const nbhip_params* p, nbhip_tune_entry* out, siz
 int nbhip_bh_tune(nbhip_ctx* ctx, const nbhip_params* p, const int* leaf_sizes, size_t count, float* ms,
                   int* best);
 
+/* No counterpart in the reference, which walks each body's traversal on its own
+ * (source/bhtree.cpp:97-135).
+ * How many waves share each group of 64 bodies in the Barnes-Hut traversal: 1, 2, 4 or 8
+ * (each takes a contiguous share of the root's octants and their sums are added in a
+ * fixed order), or 0 to choose by body count (more waves for fewer bodies).  Forces
+ * differ between choices by rounding only.  NBHIP_ERR_INVALID for any other value. */
+int nbhip_bh_configure(nbhip_ctx* ctx, int walk_split);
+
 /* ---- diagnostics (FP64 reductions on the device) --------------------------------- */
 
 /* out[0] kinetic energy, out[1] potential energy (softened, with the radius cutoff),

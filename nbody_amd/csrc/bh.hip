@@ -46,7 +46,9 @@
 // bodies.  Each lane keeps its own monotone pre-order pointer; the wave takes the
 // minimum pointer over its lanes, loads that node once with a wave-uniform address, and
 // only the lanes sitting on it advance.  Every lane therefore sees exactly its own
-// reference traversal while the node fetches stay uniform.
+// reference traversal while the node fetches stay uniform.  Up to 8 waves may share one
+// group of 64 bodies, each walking the subtrees of a share of the root's octants, their
+// sums added in a fixed order.
 //
 // Export (k_tree_count / k_tree_emit): the same sorted keys in the reference's full
 // 48-byte node layout, for Solver::tree().
@@ -63,6 +65,12 @@ namespace nbhip {
 namespace {
 
 constexpr int LEVELS = 21;
+
+// the octant digit of a 63-bit key that picks its cell at `level` (level 1 is the root's child)
+__device__ __forceinline__ unsigned key_digit(unsigned long long k, int level)
+{
+    return unsigned(k >> (63 - 3 * level)) & 7u;
+}
 
 __device__ __forceinline__ Dbl4 operator+(const Dbl4& a, const Dbl4& b)
 {
@@ -1004,13 +1012,15 @@ struct VisitCache {
     __device__ static unsigned meta(int lvl, bool bucket) { return unsigned(lvl) | (bucket ? 256u : 0u); }
 };
 
-// cnt[a]: entries position a adds to the pre-order (kept nodes + leaf); kept[a]: nodes only
+// cnt[a]: entries position a adds to the pre-order (kept nodes + leaf); kept[a]: nodes only.
+// Block 0 also empties the root's child table, which k_emit fills.
 __global__ __launch_bounds__(256) void k_count(const unsigned long long* keys, int n, int leaf_size,
                                                unsigned* cnt, unsigned char* kept, DView dv, const unsigned* ns,
-                                               VisitCache vc, unsigned* block_total)
+                                               VisitCache vc, unsigned* block_total, unsigned* top)
 {
     __shared__ unsigned wsum[4];
     const int a = blockIdx.x * 256 + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x < 8) top[threadIdx.x] = ~0u;
     unsigned entries = 0u;
     if (a < n) {
         unsigned nodes = 0;
@@ -1055,6 +1065,7 @@ struct EmitArgs {
     DView dv;
     const unsigned* ns;
     VisitCache vc;
+    unsigned* top;         // the root's children, by their octant in the root
 };
 
 __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
@@ -1066,6 +1077,12 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
     const unsigned kept = e.kept[a];
     const bool leaf_kept = e.base[a + 1] - base > kept;
     const float theta_sq = e.theta * e.theta;
+    // The root (node 0) covers every key, which share its r leading digits; its children
+    // are the nodes spanning exactly the keys of one value of digit r + 1, i.e. those that
+    // start where d drops to r or below and end before the next such drop.  At most 8
+    // positions start such a run.
+    const int r = e.n > 1 ? KeyView{ e.keys, e.n }.cpl(0, e.n - 1) : LEVELS;
+    const bool run_start = e.n > 1 && e.dv.d(a - 1) <= r;
     const auto make = [&](unsigned slot, int lvl, int b, bool bucket, const Dbl4& lo, const Dbl4& hi, unsigned next) {
         if (slot >= e.capacity) return;
         const double M = hi.m - lo.m;
@@ -1083,6 +1100,7 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
         nd.first = unsigned(a);
         nd.count = bucket ? unsigned(b - a + 1) : 0u;
         e.nodes[slot] = nd;
+        if (run_start && !(a == 0 && b == e.n - 1) && e.dv.d(b) <= r) e.top[key_digit(e.keys[a], r + 1)] = slot;
     };
     // nodes kept at a occupy [base, base + kept) shallow to deep; the walk met them deep to
     // shallow, so the j-th one met goes to base + kept - 1 - j
@@ -1122,6 +1140,8 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
             nd.first = unsigned(a);
             nd.count = 1;
             e.nodes[slot] = nd;
+            // a body alone in its octant of the root is a child of the root
+            if (run_start && e.dv.d(a) <= r) e.top[key_digit(e.keys[a], r + 1)] = slot;
         }
     }
 }
@@ -1137,6 +1157,7 @@ struct WalkArgs {
     float4* vr;
     float4* acc;
     unsigned* visits;       // per-body applied-node counts (COUNT only)
+    const unsigned* top;    // node index of the root's child in each octant, ~0u if none
     int n;
     float G;
     float eps2;
@@ -1239,10 +1260,28 @@ __device__ __forceinline__ unsigned xcd_block()
     return x * (nb / XCDS) + min(x, nb % XCDS) + q;
 }
 
-template <bool CUT, bool STEP, bool COUNT>
-__global__ __launch_bounds__(256) void k_walk(WalkArgs a)
+// K waves share each 64-body group: wave part p of the group walks the root's children in
+// its octants [8p/K, 8(p+1)/K), which are one contiguous run of the pre-order, and part 0
+// also the root itself.  An octant without an entry in the child table only moves its
+// children into the part before; the parts' runs always tile the pre-order after the root.  Every lane still meets exactly its own reference traversal, cut into K
+// pieces; the pieces' compensated sums meet in LDS, in part order, so the result does not
+// depend on timing.  Splitting shortens the dependent chain of node loads a wave waits on
+// when there are too few waves to hide it (small systems).
+template <int K>
+constexpr int walk_threads()
 {
-    const int k = int(xcd_block()) * 256 + threadIdx.x;
+    return K == 8 ? 512 : 256;
+}
+
+template <bool CUT, bool STEP, bool COUNT, int K>
+__global__ __launch_bounds__(walk_threads<K>()) void k_walk(WalkArgs a)
+{
+    constexpr int GROUPS = walk_threads<K>() / 64 / K;  // 64-body groups per block
+    static_assert(GROUPS >= 1 && (K & (K - 1)) == 0 && K <= 8, "K is 1, 2, 4 or 8");
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int lane = int(threadIdx.x & 63);
+    const int part = wave % K, group = wave / K;
+    const int k = (int(xcd_block()) * GROUPS + group) * 64 + lane;
     const bool valid = k < a.n;
     const unsigned total = min(*a.total, a.capacity);
     const float4 x = valid ? a.spm[k] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1252,7 +1291,36 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a)
         const float r = a.vr[body].w;
         r2 = r * r;
     }
-    unsigned idx = valid ? 0u : total;
+    // this part's run of the pre-order: [begin, end)
+    unsigned begin = 0u, end = total;
+    if (K > 1) {
+        const int q0 = 8 * part / K, q1 = 8 * (part + 1) / K;
+        unsigned lo = total, hi = total;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const unsigned t = min(a.top[q], total);
+            if (q >= q0) lo = min(lo, t);
+            if (q >= q1) hi = min(hi, t);
+        }
+        if (part > 0) begin = lo;
+        end = hi;
+    }
+    unsigned idx = valid ? 0u : end;
+    if (K > 1 && part > 0 && valid) {
+        // the root, tested as the walk below tests it: only the lanes that open it walk
+        // this part's children (the ones that apply it leave that to part 0)
+        const BhNode nd = a.nodes[0];
+        float dx, dy, dz, d2;
+        {
+#pragma clang fp contract(off)
+            dx = nd.cm.x - x.x;
+            dy = nd.cm.y - x.y;
+            dz = nd.cm.z - x.z;
+            d2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+        }
+        const bool opened = nd.cm.w != 0.f && nd.count == 0 && !(d2 > nd.thr);
+        idx = opened ? begin : end;
+    }
     Accum acc;
     unsigned applied = 0;
     // After a step every lane's pointer is past cur, so when any lane sits on cur + 1 (an
@@ -1263,7 +1331,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a)
     // totals every 8 steps of the wave (at most 8 nodes or buckets per lane in between).
     unsigned cur = __ockl_wfred_min_u32(idx);
     unsigned steps = 0;
-    while (cur < total) {
+    while (cur < end) {
         const BhNode nd = a.nodes[cur];
         const bool at = idx == cur;
         // empty nodes are skipped (reference bhtree.cpp:108-113); a leaf is applied
@@ -1316,6 +1384,34 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a)
         }
         if ((++steps & 7u) == 0u) acc.fold();
     }
+    if constexpr (K > 1) {
+        // parts 1..K-1 hand their sums to part 0, which adds them in part order
+        __shared__ float red[GROUPS][K - 1][7][64];
+        acc.fold();
+        if (part > 0) {
+            float* r = &red[group][part - 1][0][lane];
+            r[0 * 64] = acc.sx;
+            r[1 * 64] = acc.sy;
+            r[2 * 64] = acc.sz;
+            r[3 * 64] = acc.cx;
+            r[4 * 64] = acc.cy;
+            r[5 * 64] = acc.cz;
+            r[6 * 64] = __uint_as_float(applied);
+        }
+        __syncthreads();
+        if (part > 0) return;
+#pragma unroll
+        for (int p = 0; p < K - 1; ++p) {
+            const float* r = &red[group][p][0][lane];
+            two_sum(acc.sx, acc.cx, r[0 * 64]);
+            two_sum(acc.sy, acc.cy, r[1 * 64]);
+            two_sum(acc.sz, acc.cz, r[2 * 64]);
+            acc.cx += r[3 * 64];
+            acc.cy += r[4 * 64];
+            acc.cz += r[5 * 64];
+            applied += __float_as_uint(r[6 * 64]);
+        }
+    }
     if (!valid) return;
     const float3 t = acc.total();
     const float ax = a.G * t.x, ay = a.G * t.y, az = a.G * t.z;
@@ -1350,10 +1446,6 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a)
 // d(a-1) < L.  Numbering these internal cells in depth-first order (by start position,
 // then depth) by a scan of per-position counts gives each one's children block directly:
 // internal cell g has its child q at node 1 + 8 g + q, and the root is node 0.
-__device__ __forceinline__ unsigned key_digit(unsigned long long k, int level)
-{
-    return unsigned(k >> (63 - 3 * level)) & 7u;  // level 1 is the root's child
-}
 
 __global__ __launch_bounds__(256) void k_tree_count(const unsigned long long* keys, int n, unsigned* cnt)
 {
@@ -1482,6 +1574,7 @@ void bh_reserve(nbhip_ctx* c)
     c->bh_base.grow(size_t(n) + 1, 0, s);
     c->bh_nodes.grow(2 * size_t(n) + 1, 0, s);
     c->bh_kept.grow(n, 0, s);
+    c->bh_top.grow(8, 0, s);
     c->bh_levels.grow(LevelSizes(n).total, 0, s);
     rs_reserve(c, n);
     c->bh_uscan.grow(std::max(size_t(grid(n)), size_t(std::max(1, (n + SC_TILE - 1) / SC_TILE))), 0, s);
@@ -1515,6 +1608,7 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     c->bh_base.grow(size_t(n) + 1, 0, s);
     c->bh_nodes.grow(2 * size_t(n) + 1, 0, s);
     c->bh_kept.grow(n, 0, s);
+    c->bh_top.grow(8, 0, s);
 
     hipLaunchKernelGGL(k_keys, dim3(grid(n)), dim3(256), 0, s, c->pm[c->cur].ptr, n, p.size,
                        c->bh_keys[0].ptr, c->bh_vals[0].ptr);
@@ -1561,7 +1655,7 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     static_assert(SC_TILE % 256 == 0, "k_count blocks tile the scan tiles");
     c->bh_uscan.grow(grid(n), 0, s);
     hipLaunchKernelGGL(k_count, dim3(grid(n)), dim3(256), 0, s, keys, n, leaf, c->bh_base.ptr,
-                       c->bh_kept.ptr, dv, ns, vc, c->bh_uscan.ptr);
+                       c->bh_kept.ptr, dv, ns, vc, c->bh_uscan.ptr, c->bh_top.ptr);
     NBHIP_CHECK(hipGetLastError());
     if (n <= SC_SINGLE_MAX) {
         exclusive_scan<unsigned>(c, c->bh_base.ptr, n, c->bh_base.ptr, c->bh_uscan);
@@ -1571,12 +1665,34 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     }
 
     EmitArgs e{ keys, n, leaf, c->bh_base.ptr, c->bh_kept.ptr, c->bh_pref.ptr, c->bh_spm.ptr, p.size,
-                p.theta, p.opening, c->bh_nodes.ptr, unsigned(c->bh_nodes.cap), dv, ns, vc };
+                p.theta, p.opening, c->bh_nodes.ptr, unsigned(c->bh_nodes.cap), dv, ns, vc, c->bh_top.ptr };
     hipLaunchKernelGGL(k_emit, dim3(grid(n)), dim3(256), 0, s, e);
     NBHIP_CHECK(hipGetLastError());
     c->bh_built = true;
     c->bh_size = p.size;
     ++c->bh_builds;
+}
+
+template <int K>
+void (*walk_kernel(bool cut, bool fuse, bool count))(WalkArgs)
+{
+    using Walk = void (*)(WalkArgs);
+    if (count)
+        return cut ? (fuse ? (Walk)k_walk<true, true, true, K> : (Walk)k_walk<true, false, true, K>)
+                   : (fuse ? (Walk)k_walk<false, true, true, K> : (Walk)k_walk<false, false, true, K>);
+    return cut ? (fuse ? (Walk)k_walk<true, true, false, K> : (Walk)k_walk<true, false, false, K>)
+               : (fuse ? (Walk)k_walk<false, true, false, K> : (Walk)k_walk<false, false, false, K>);
+}
+
+// Waves per 64-body group in the walk: the context's fixed choice, else by body count as
+// measured on disk + bulge systems at leaf size 16 (2^14 .. 2^22 bodies): shorter walks
+// hide the node loads' latency better until the extra waves' root tests and sums cost more.
+int walk_split(const nbhip_ctx* c, int n)
+{
+    if (c->bh_split > 0) return c->bh_split;
+    if (n < (1 << 17)) return 8;
+    if (n < (1 << 20)) return 4;
+    return 2;
 }
 
 void bh_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse, bool count)
@@ -1602,15 +1718,16 @@ void bh_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse, bool count)
     a.dt = p.dt;
     a.size = p.size;
     a.wrap = p.wrap;
-    using K = void (*)(WalkArgs);
-    K k;
-    if (count)
-        k = cut ? (fuse ? (K)k_walk<true, true, true> : (K)k_walk<true, false, true>)
-                : (fuse ? (K)k_walk<false, true, true> : (K)k_walk<false, false, true>);
-    else
-        k = cut ? (fuse ? (K)k_walk<true, true, false> : (K)k_walk<true, false, false>)
-                : (fuse ? (K)k_walk<false, true, false> : (K)k_walk<false, false, false>);
-    hipLaunchKernelGGL(k, dim3(grid(n)), dim3(256), 0, c->stream, a);
+    a.top = c->bh_top.ptr;
+    const int split = walk_split(c, int(n));
+    using Walk = void (*)(WalkArgs);
+    const Walk k = split == 8   ? walk_kernel<8>(cut, fuse, count)
+                   : split == 4 ? walk_kernel<4>(cut, fuse, count)
+                   : split == 2 ? walk_kernel<2>(cut, fuse, count)
+                                : walk_kernel<1>(cut, fuse, count);
+    const int threads = split == 8 ? walk_threads<8>() : 256;
+    const int bodies = threads / split;  // bodies per block
+    hipLaunchKernelGGL(k, dim3(unsigned((n + size_t(bodies) - 1) / size_t(bodies))), dim3(threads), 0, c->stream, a);
     NBHIP_CHECK(hipGetLastError());
     c->launches += 1;
 }
@@ -1681,6 +1798,7 @@ void bh_release(nbhip_ctx* c)
     c->bh_pref.release();
     c->bh_base.release();
     c->bh_nodes.release();
+    c->bh_top.release();
     c->bh_rs_status.release();
     c->bh_uscan.release();
     c->bh_dscan.release();

@@ -152,12 +152,14 @@ struct nbhip_ctx {
     nbhip::DevBuf<unsigned char> bh_kept;
     nbhip::DevBuf<unsigned char> bh_levels;  // d + 1 per sorted position and its chunk minima
     nbhip::DevBuf<nbhip::BhNode> bh_nodes;
+    nbhip::DevBuf<unsigned> bh_top;        // node index of the root's child in each octant, ~0u if none
     nbhip::DevBuf<unsigned> bh_rs_state;   // radix sort: digit counts and starts, tickets, fault
     nbhip::DevBuf<unsigned> bh_rs_status;  // radix sort: look-back words, 8 passes x tiles x 256
     nbhip::DevBuf<unsigned> bh_uscan;
     nbhip::DevBuf<nbhip::Dbl4> bh_dscan;
     nbhip::DevBuf<unsigned> bh_visits;
     int bh_sorted = 0;          // which keys/vals buffer holds the sorted result
+    int bh_split = 0;           // waves per 64-body group in the walk (0: by body count)
     bool bh_built = false;      // a tree of the current count exists on the device
     float bh_size = 0.f;        // root edge of that tree
     unsigned long long bh_builds = 0;  // builds enqueued so far (graph replays included)

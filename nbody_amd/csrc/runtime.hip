@@ -407,7 +407,7 @@ void enqueue_steps(nbhip_ctx* c, const nbhip_params& p, int steps)
 // Barnes-Hut step is ~20 launches, so for small systems the graph removes most of the
 // launch cost.  Every
 // allocation and host query a step makes is done first, outside the capture; the graph
-// is reused while mode, body count, parameters, parity, launch shape and every buffer
+// is reused while mode, body count, parameters, parity, launch shapes and every buffer
 // address (the epoch) are unchanged.
 void step_graph(nbhip_ctx* c, const nbhip_params& p, int steps)
 {
@@ -419,8 +419,8 @@ void step_graph(nbhip_ctx* c, const nbhip_params& p, int steps)
         nbhip::bh_reserve(c);
     }
     char key[256];
-    std::snprintf(key, sizeof key, "%d/%d/%zu/%d/%d/%d/%d/%d/%llu/", p.mode, steps, c->count, c->cur,
-                  cfg.P, cfg.W, cfg.jsplit, int(c->any_radius), nbhip::g_buffer_epoch.load());
+    std::snprintf(key, sizeof key, "%d/%d/%zu/%d/%d/%d/%d/%d/%d/%llu/", p.mode, steps, c->count, c->cur,
+                  cfg.P, cfg.W, cfg.jsplit, c->bh_split, int(c->any_radius), nbhip::g_buffer_epoch.load());
     std::string k(key);
     k.append(reinterpret_cast<const char*>(&p), sizeof p);
     const int cur_after = p.mode == NBHIP_MODE_N2 ? (c->cur ^ (steps & 1)) : c->cur;
@@ -645,6 +645,16 @@ int nbhip_bh_interactions(nbhip_ctx* c, const nbhip_params* p, uint32_t* per_bod
         for (uint32_t x : v) sum += x;
         if (total) *total = sum;
         if (per_body) std::memcpy(per_body, v.data(), v.size() * sizeof(uint32_t));
+    });
+}
+
+int nbhip_bh_configure(nbhip_ctx* c, int walk_split)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        if (walk_split != 0 && walk_split != 1 && walk_split != 2 && walk_split != 4 && walk_split != 8)
+            fail(NBHIP_ERR_INVALID, "walk split %d must be 0, 1, 2, 4 or 8", walk_split);
+        c->bh_split = walk_split;
     });
 }
 

@@ -322,6 +322,83 @@ def test_textbook_criterion_converges(ctx, orc):
     assert errs[0] > errs[1] > errs[2] and errs[2] < 1e-3
 
 
+def _one_octant(n, seed, sign):
+    b = ics.plummer(n, seed=seed)
+    # every body in one octant of the root cube: the root is then that octant's cell, whose
+    # children split on the next digit
+    b[:, 0:3] = sign * (np.abs(b[:, 0:3]) * 0.1 + 1.0)
+    return b
+
+
+@pytest.mark.parametrize("case", ["plummer", "root_bucket", "theta0", "octant_first", "octant_last", "outside",
+                                  "disk"])
+def test_walk_split_matches_single_wave(ctx, case):
+    # The walk may split each 64-body group over 2, 4 or 8 waves, each taking a share of
+    # the root's octants (the root itself, and lanes that accept it, stay with the first).
+    # Every lane's traversal is unchanged, so the applied-node counts are identical and
+    # the forces differ only in the order their compensated partial sums are added: a
+    # rounding of the parts' sums, which is large next to the total only where the
+    # octants' pulls cancel (bodies near the centre of mass).
+    size, theta, leaf = 64.0, 0.5, 1
+    if case == "plummer":
+        b = ics.plummer(20000, seed=31)
+    elif case == "root_bucket":
+        b, leaf = ics.plummer(12, seed=32), 16
+    elif case == "theta0":
+        b, theta = ics.plummer(3000, seed=33), 0.0
+    elif case.startswith("octant"):
+        b = _one_octant(5000, 34, 1.0 if case == "octant_first" else -1.0)
+    elif case == "outside":
+        b, size = ics.plummer(4000, seed=35), 8.0
+    else:
+        b, size, leaf = ics.disk_bulge(30000, seed=36), 128.0, 16
+    p = bh_params(size=size, theta=theta, eps=0.01, leaf_size=leaf)
+    try:
+        got = {}
+        for split in (1, 2, 4, 8):
+            ctx.bh_configure(split)
+            ctx.upload_bodies(b)
+            visits, total = ctx.bh_interactions(p)
+            ctx.upload_bodies(b)
+            ctx.accelerate(p)
+            got[split] = (visits.copy(), total, ctx.download_bodies()[:, 8:11].copy())
+        v1, t1, a1 = got[1]
+        # with a bucket root, or every lane applying the root whole, the later parts have
+        # nothing to walk; otherwise they take a share (at 8 parts, one octant each) and the
+        # sums' order changes
+        idle = case in ("root_bucket", "theta0")
+        assert np.array_equal(got[8][2], a1) == idle
+        for split in (2, 4, 8):
+            v, t, a = got[split]
+            assert t == t1 and np.array_equal(v, v1), split
+            if idle:
+                assert np.array_equal(a, a1), split
+            e = rel_err(a, a1)
+            print(f"{case} split {split}: rel diff 99% {np.quantile(e, 0.99):.2e} max {e.max():.2e}")
+            assert np.quantile(e, 0.99) <= 1e-6 and e.max() <= 1e-5, split
+    finally:
+        ctx.bh_configure(0)
+
+
+def test_walk_split_fused_step(ctx):
+    # the fused kick-drift runs in the first wave of a group once the parts' sums are in
+    b = ics.disk_bulge(6000, seed=37)
+    p = bh_params(size=128.0, theta=0.5, eps=0.01, dt=1e-3, wrap=1, leaf_size=8)
+    try:
+        out = {}
+        for split in (1, 8):
+            ctx.bh_configure(split)
+            ctx.upload_bodies(b)
+            ctx.step(p, 2)
+            out[split] = ctx.download_bodies()
+        x1, x8 = out[1][:, 0:3].astype(np.float64), out[8][:, 0:3].astype(np.float64)
+        assert np.abs(x8 - x1).max() <= 1e-6 * np.abs(x1).max()
+        with pytest.raises(_lib.HipError):
+            ctx.bh_configure(3)
+    finally:
+        ctx.bh_configure(0)
+
+
 def test_step_fused_equals_accelerate_then_integrate(ctx):
     b = ics.disk_bulge(5000, seed=6)
     p = bh_params(size=128.0, theta=0.5, eps=0.01, dt=1e-3, wrap=1, leaf_size=16)
