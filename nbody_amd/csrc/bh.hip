@@ -106,22 +106,39 @@ constexpr int SC_IPT = 8;
 constexpr int SC_TILE = SC_BLOCK * SC_IPT;
 
 template <typename T>
+__device__ __forceinline__ T shfl_up_item(const T& v, int off)
+{
+    return __shfl_up(v, off);
+}
+
+template <>
+__device__ __forceinline__ Dbl4 shfl_up_item<Dbl4>(const Dbl4& v, int off)
+{
+    return Dbl4{ __shfl_up(v.m, off), __shfl_up(v.x, off), __shfl_up(v.y, off), __shfl_up(v.z, off) };
+}
+
+template <typename T>
 __device__ T block_exclusive_scan(T v, T* sh, T& total)
 {
-    // Hillis-Steele inclusive scan over the block's 256 values, then shift.
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int off = 1; off < SC_BLOCK; off <<= 1) {
-        const T add = t >= off ? sh[t - off] : T{};
-        __syncthreads();
-        if (t >= off) sh[t] = sh[t] + add;
-        __syncthreads();
+    // Scan within each wave by lane shuffles (offsets 1 .. 32), then add the totals of the
+    // waves before, in wave order: one barrier for the exchange, one before `sh` is reused.
+    static_assert(SC_BLOCK == 256, "four waves per block");
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    T inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const T u = shfl_up_item(inc, off);
+        if (lane >= off) inc = inc + u;
     }
-    total = sh[SC_BLOCK - 1];
-    const T excl = t > 0 ? sh[t - 1] : T{};
+    T ex = shfl_up_item(inc, 1);
+    if (lane == 0) ex = T{};
+    if (lane == 63) sh[w] = inc;
     __syncthreads();
-    return excl;
+    T pre{};
+    for (int i = 0; i < w; ++i) pre = pre + sh[i];
+    total = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+    __syncthreads();
+    return w == 0 ? ex : pre + ex;
 }
 
 template <typename T>
@@ -603,15 +620,25 @@ __global__ __launch_bounds__(SC_BLOCK) void k_gather(const unsigned* perm, const
 {
     __shared__ Dbl4 sh[SC_BLOCK];
     const int base = blockIdx.x * SC_TILE + threadIdx.x;
-    Dbl4 sum{};
+    // every index load, then every body load, in flight together
+    unsigned j[SC_IPT];
+#pragma unroll
     for (int i = 0; i < SC_IPT; ++i) {
         const int k = base + i * SC_BLOCK;
-        if (k >= n) break;
-        const unsigned j = perm[k];
-        const float4 p = j < unsigned(n) ? pm[j] : make_float4(0.f, 0.f, 0.f, 0.f);
-        spm[k] = p;
-        const double m = p.w;
-        sum = sum + Dbl4{ m, m * double(p.x), m * double(p.y), m * double(p.z) };
+        j[i] = k < n ? perm[k] : unsigned(n);
+    }
+    float4 p[SC_IPT];
+#pragma unroll
+    for (int i = 0; i < SC_IPT; ++i) p[i] = j[i] < unsigned(n) ? pm[j[i]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    Dbl4 sum{};
+#pragma unroll
+    for (int i = 0; i < SC_IPT; ++i) {
+        const int k = base + i * SC_BLOCK;
+        if (k < n) {
+            spm[k] = p[i];
+            const double m = p[i].w;
+            sum = sum + Dbl4{ m, m * double(p[i].x), m * double(p[i].y), m * double(p[i].z) };
+        }
     }
     Dbl4 total;
     (void)block_exclusive_scan(sum, sh, total);
