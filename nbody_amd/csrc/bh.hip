@@ -1287,6 +1287,19 @@ __device__ __forceinline__ unsigned xcd_block()
     return x * (nb / XCDS) + min(x, nb % XCDS) + q;
 }
 
+// The per-body records the walk reads and writes by body index (velocities, accelerations,
+// positions) go around the caches' normal allocation, so they do not push out the nodes.
+using F4v = float __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load_stream(const float4* p)
+{
+    const F4v v = __builtin_nontemporal_load(reinterpret_cast<const F4v*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void store_stream(float4* p, const float4 v)
+{
+    __builtin_nontemporal_store(F4v{ v.x, v.y, v.z, v.w }, reinterpret_cast<F4v*>(p));
+}
+
 // K waves share each 64-body group: wave part p of the group walks the root's children in
 // its octants [8p/K, 8(p+1)/K), which are one contiguous run of the pre-order, and part 0
 // also the root itself.  An octant without an entry in the child table only moves its
@@ -1442,11 +1455,11 @@ __global__ __launch_bounds__(walk_threads<K>()) void k_walk(WalkArgs a)
     if (!valid) return;
     const float3 t = acc.total();
     const float ax = a.G * t.x, ay = a.G * t.y, az = a.G * t.z;
-    a.acc[body] = make_float4(ax, ay, az, 0.f);
+    store_stream(a.acc + body, make_float4(ax, ay, az, 0.f));
     if (COUNT) a.visits[body] = applied;
     if (STEP) {
 #pragma clang fp contract(off)
-        float4 v = a.vr[body];
+        float4 v = load_stream(a.vr + body);
         float4 p = x;
         v.x = v.x + ax * a.dt;
         v.y = v.y + ay * a.dt;
@@ -1460,8 +1473,8 @@ __global__ __launch_bounds__(walk_threads<K>()) void k_walk(WalkArgs a)
             p.y = fmodf(fmodf(p.y + half, a.size) + a.size, a.size) - half;
             p.z = fmodf(fmodf(p.z + half, a.size) + a.size, a.size) - half;
         }
-        a.pm[body] = p;
-        a.vr[body] = v;
+        store_stream(a.pm + body, p);
+        store_stream(a.vr + body, v);
     }
 }
 
