@@ -539,25 +539,31 @@ constexpr SortShape sort_shape()
     return SortShape{ RB, IPT, k_rs_pass<RB, IPT, STAGE> };
 }
 
-// Block shape of a pass (NBHIP_RS_SHAPE = index into this table overrides).  At 2^20 keys
-// 1024 x 4 measured fastest: 0.684 ms per Barnes-Hut step against 0.691 (512 x 8), 0.696
-// (256 x 16), 0.722 (512 x 4) and 0.733 (256 x 8).
-const SortShape& sort_config()
+// Block shape of a pass for n keys (NBHIP_RS_SHAPE = index into this table overrides).
+// Measured per Barnes-Hut step (disk + bulge, leaf 16, hipGraph replays): below 2^19 keys
+// the 2048-key tiles of 512 x 4 are faster (2^14 0.143 ms against 0.157 with 1024 x 4,
+// 2^15 0.151 / 0.163, 2^17 0.190 / 0.199, 2^18 0.248 / 0.257): more, shorter tiles shorten
+// each block's chain of phases when there are few of them.  From 2^19 keys 1024 x 4 is
+// (2^19 0.344 / 0.361, 2^20 0.523 / 0.549, 2^21 0.882 / 0.943); at 2^20 it also beat 512 x 8
+// and 256 x 16.
+const SortShape& sort_config(int n)
 {
     static const SortShape shapes[] = { sort_shape<1024, 4, true>(), sort_shape<512, 8, true>(),
                                         sort_shape<1024, 4, false>(), sort_shape<256, 16, true>(),
-                                        sort_shape<512, 4, true>(),  sort_shape<512, 8, false>() };
+                                        sort_shape<512, 4, true>(),  sort_shape<512, 8, false>(),
+                                        sort_shape<256, 4, true>() };
     static const int pick = [] {
         const char* e = std::getenv("NBHIP_RS_SHAPE");
-        const int k = e ? std::atoi(e) : 0;
-        return (k >= 0 && k < int(sizeof shapes / sizeof shapes[0])) ? k : 0;
+        const int k = e ? std::atoi(e) : -1;
+        return (k >= 0 && k < int(sizeof shapes / sizeof shapes[0])) ? k : -1;
     }();
-    return shapes[pick];
+    if (pick >= 0) return shapes[pick];
+    return n < (1 << 19) ? shapes[4] : shapes[0];
 }
 
 int sort_blocks(int n)
 {
-    const SortShape& sh = sort_config();
+    const SortShape& sh = sort_config(n);
     return std::max(1, (n + sh.rb * sh.ipt - 1) / (sh.rb * sh.ipt));
 }
 
@@ -589,7 +595,7 @@ int radix_sort(nbhip_ctx* c, int n, int src)
 {
     // look-back words carry 30-bit positions; 2^30 bodies would need over 288 GB here anyway
     if (n >= int(LB_COUNT)) fail(NBHIP_ERR_INVALID, "Barnes-Hut sorts at most 2^30 - 1 bodies, not %d", n);
-    const SortShape& sh = sort_config();
+    const SortShape& sh = sort_config(n);
     const int nb = sort_blocks(n);
     rs_reserve(c, n);
     unsigned* st = c->bh_rs_state.ptr;
