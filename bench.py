@@ -93,6 +93,13 @@ def load_traffic(name: str):
         return None
 
 
+def momentum_drift(d0, d1) -> float:
+    """|P_end - P_start| over the momentum scale sqrt(2 M KE_start) (= M v_rms), from two
+    nbhip_diagnostics results (KE, PE, P xyz, L xyz, M)."""
+    dp = np.linalg.norm(np.asarray(d1[2:5], np.float64) - np.asarray(d0[2:5], np.float64))
+    return float(dp / np.sqrt(2.0 * d0[8] * d0[0]))
+
+
 def run_direct(args) -> dict:
     from nbody_amd import _lib, ics
     n = N_C2
@@ -164,6 +171,7 @@ def run_direct(args) -> dict:
             "rel_err_vs_f64_median": float(np.median(rel)),
             "sample": "1024 seeded bodies, FP64 direct sum on the device",
             "energy_drift": float((E1 - E0) / abs(E0)),
+            "momentum_drift": momentum_drift(e0, e1),
             "energy_steps": args.warmup + args.steps,
         },
     }
@@ -298,7 +306,14 @@ def run_bh(args) -> dict:
     ctx = _lib.Context(0)
     ctx.upload_bodies(bodies)
     e0 = ctx.diagnostics(p)
-    _, inter = ctx.bh_interactions(p)
+    # the textbook criterion at the same theta first, for comparison; then the reference's
+    pt = _lib.params(mode=_lib.MODE_NLOGN, theta=0.5, leaf_size=16, eps=0.01, dt=1e-3, size=size,
+                     wrap=1, opening=_lib.OPEN_TEXTBOOK)
+    sample = np.random.default_rng(SEED).choice(n, 1024, replace=False).astype(np.uint32)
+    ctx.accelerate(pt)
+    _, rel_text = ctx.force_error_f64(pt, sample)
+    _, inter = ctx.bh_interactions(p)    # also leaves the first pass's accelerations
+    _, rel = ctx.force_error_f64(p, sample)
     steps = 10
     ctx.step(p, 1)
     ctx.synchronize()
@@ -320,6 +335,21 @@ def run_bh(args) -> dict:
         "interactions_per_step": int(inter),
         "value": inter * steps / (ms / 1e3) / 1e9, "unit": "G-interactions/s",
         "energy_drift": float(((e1[0] + e1[1]) - (e0[0] + e0[1])) / abs(e0[0] + e0[1])),
+        "accuracy": {
+            "rel_err_vs_f64_median": float(np.median(rel)),
+            "rel_err_vs_f64_p99": float(np.quantile(rel, 0.99)),
+            "rel_err_vs_f64_max": float(rel.max()),
+            "sample": "1024 seeded bodies at the initial positions, FP64 direct sum on the device: "
+                      "the approximation error of the reference's criterion (a cell is taken whole "
+                      "when |com - x| > theta * edge) at theta 0.5, not rounding",
+            "textbook_opening": {"rel_err_vs_f64_median": float(np.median(rel_text)),
+                                 "rel_err_vs_f64_p99": float(np.quantile(rel_text, 0.99)),
+                                 "rel_err_vs_f64_max": float(rel_text.max()),
+                                 "criterion": "|com - x| > edge / theta, theta 0.5"},
+            "energy_drift": float(((e1[0] + e1[1]) - (e0[0] + e0[1])) / abs(e0[0] + e0[1])),
+            "momentum_drift": momentum_drift(e0, e1),
+            "energy_steps": 1 + 3 * steps,
+        },
     }
     if not args.no_cpu:
         from oracle import oracle as O
