@@ -49,7 +49,9 @@ def test_keys_match_reference_golden(ctx, size):
     assert np.array_equal(perm, np.argsort(keys, kind="stable").astype(np.uint32))
 
 
-@pytest.mark.parametrize("n,size", [(1, 64.0), (2, 64.0), (5000, 64.0), (100000, 128.0), (70001, 10000.0)])
+# 2^19 - 1 and 2^19 keys sit either side of the switch from 2048- to 4096-key radix tiles
+@pytest.mark.parametrize("n,size", [(1, 64.0), (2, 64.0), (5000, 64.0), (100000, 128.0), (70001, 10000.0),
+                                    ((1 << 19) - 1, 128.0), (1 << 19, 128.0)])
 def test_keys_and_permutation_bit_exact(ctx, orc, n, size):
     b = ics.disk_bulge(n, seed=n) if n > 2 else ics.plummer(n, seed=n)
     if n > 10:
