@@ -333,7 +333,7 @@ def _one_octant(n, seed, sign):
 
 
 @pytest.mark.parametrize("case", ["plummer", "root_bucket", "theta0", "octant_first", "octant_last", "outside",
-                                  "disk"])
+                                  "disk", "cutoff"])
 def test_walk_split_matches_single_wave(ctx, case):
     # The walk may split each 64-body group over 2, 4 or 8 waves, each taking a share of
     # the root's octants (the root itself, and lanes that accept it, stay with the first).
@@ -352,9 +352,14 @@ def test_walk_split_matches_single_wave(ctx, case):
         b = _one_octant(5000, 34, 1.0 if case == "octant_first" else -1.0)
     elif case == "outside":
         b, size = ics.plummer(4000, seed=35), 8.0
-    else:
+    elif case == "disk":
         b, size, leaf = ics.disk_bulge(30000, seed=36), 128.0, 16
-    p = bh_params(size=size, theta=theta, eps=0.01, leaf_size=leaf)
+    else:
+        # the reference's law: no softening, each receiving body's radius cuts off
+        b = ics.plummer(8000, seed=38)
+        b[:, 3] = np.random.default_rng(38).uniform(0.0, 0.02, len(b)).astype(np.float32)
+    eps = 0.0 if case == "cutoff" else 0.01
+    p = bh_params(size=size, theta=theta, eps=eps, leaf_size=leaf)
     try:
         got = {}
         for split in (1, 2, 4, 8):
