@@ -575,9 +575,11 @@ size_t rs_status_tiles(int n)
 }
 
 constexpr int RS_COUNT_BLOCK = 1024;
-// Blocks of k_rs_count: at 2^20 keys 64 measured fastest (0.617 ms per Barnes-Hut step,
-// against 0.620 with 32, 0.634 with 128 and 0.662 with 256: more blocks, more global adds).
-int rs_count_blocks(int n) { return std::max(1, std::min(64, (n + 4095) / 4096)); }
+// Blocks of k_rs_count, one per 4096 keys up to 256: measured per Barnes-Hut step (hipGraph
+// replays, two runs each) at 2^20 keys 0.522 ms against 0.527 with at most 64 blocks, at
+// 2^19 0.343 against 0.345; the extra global adds of the block histograms cost less than
+// the longer per-block loops they replace.
+int rs_count_blocks(int n) { return std::max(1, std::min(256, (n + 4095) / 4096)); }
 
 // The sort's buffers for n keys; the state words are zeroed when first allocated (never
 // under graph capture: bh_reserve allocates before a capture starts).
