@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void k_keys(const float4* pm, int n, float siz
 
 // ---- scans (two phases: tile totals, then tile scans that add the totals before them) -----
 constexpr int SC_BLOCK = 256;
-constexpr int SC_IPT = 8;
+constexpr int SC_IPT = 4;
 constexpr int SC_TILE = SC_BLOCK * SC_IPT;
 
 template <typename T>
@@ -296,7 +296,6 @@ constexpr int RS_PASSES = 8;
 constexpr int RS_COUNTS = 0, RS_BASES = RS_PASSES * 256, RS_TICKETS = 2 * RS_PASSES * 256,
               RS_FAULT = RS_TICKETS + RS_PASSES, RS_WORDS = RS_FAULT + 8;
 constexpr unsigned LB_COUNT = 1u << 30, LB_TOTAL = 1u << 31, LB_VALUE = LB_COUNT - 1u;
-constexpr unsigned RS_SUPER = 16;  // tiles per super-tile of the two-level look-back
 // Look-back waits end here, about a second in: the tile then takes zero for the unpublished
 // value and sets the fault word, which the next synchronisation reports; the sort is then
 // wrong but every block still finishes.
@@ -361,7 +360,7 @@ __device__ __forceinline__ unsigned lb_wait(unsigned* word, unsigned* fault)
 // with 8 ballots, and the wave's own LDS counters give each item its rank among the wave's
 // earlier items of that digit.  The waves' counts then give every wave its start within each
 // digit, and the tile's counts go to the two-level look-back below.
-template <int RB, int IPT, bool STAGE>
+template <int RB, int IPT, bool STAGE, unsigned LB_SUPER>
 __global__ __launch_bounds__(RB) void k_rs_pass(const unsigned long long* kin, const unsigned* vin,
                                                 unsigned long long* kout, unsigned* vout, int n, int shift,
                                                 const unsigned* bases, unsigned* ticket, unsigned* status,
@@ -369,7 +368,7 @@ __global__ __launch_bounds__(RB) void k_rs_pass(const unsigned long long* kin, c
 {
     static_assert(RB >= 256, "one thread per digit for the look-back");
     constexpr int WAVES = RB / 64, LB_WINDOW = 8;
-    constexpr unsigned LB_SUPER = RS_SUPER;
+    static_assert(LB_SUPER >= 2, "a super-tile of at least two tiles");
     __shared__ unsigned s_tile;
     __shared__ unsigned wh[WAVES][256];  // a wave's count of each digit, then its start in the tile's run
     __shared__ unsigned start[256];      // where the tile's run of each digit begins in the output
@@ -529,14 +528,15 @@ __global__ __launch_bounds__(RB) void k_rs_pass(const unsigned long long* kin, c
 
 struct SortShape {
     int rb, ipt;
+    unsigned super;  // tiles per super-tile of the two-level look-back
     void (*pass)(const unsigned long long*, const unsigned*, unsigned long long*, unsigned*, int, int,
                  const unsigned*, unsigned*, unsigned*, unsigned*);
 };
 
-template <int RB, int IPT, bool STAGE>
+template <int RB, int IPT, bool STAGE, unsigned SUPER>
 constexpr SortShape sort_shape()
 {
-    return SortShape{ RB, IPT, k_rs_pass<RB, IPT, STAGE> };
+    return SortShape{ RB, IPT, SUPER, k_rs_pass<RB, IPT, STAGE, SUPER> };
 }
 
 // Block shape of a pass for n keys (NBHIP_RS_SHAPE = index into this table overrides).
@@ -545,13 +545,14 @@ constexpr SortShape sort_shape()
 // 2^15 0.151 / 0.163, 2^17 0.190 / 0.199, 2^18 0.248 / 0.257): more, shorter tiles shorten
 // each block's chain of phases when there are few of them.  From 2^19 keys 1024 x 4 is
 // (2^19 0.344 / 0.361, 2^20 0.523 / 0.549, 2^21 0.882 / 0.943); at 2^20 it also beat 512 x 8
-// and 256 x 16.
+// and 256 x 16.  Look-back super-tiles: 16 tiles below 2^19 keys (2^15: 0.145 ms against
+// 0.149 with 8), 8 from there (2^20: 0.510 ms against 0.520 with 16; two runs each).
 const SortShape& sort_config(int n)
 {
-    static const SortShape shapes[] = { sort_shape<1024, 4, true>(), sort_shape<512, 8, true>(),
-                                        sort_shape<1024, 4, false>(), sort_shape<256, 16, true>(),
-                                        sort_shape<512, 4, true>(),  sort_shape<512, 8, false>(),
-                                        sort_shape<256, 4, true>() };
+    static const SortShape shapes[] = { sort_shape<1024, 4, true, 8>(),   sort_shape<512, 8, true, 16>(),
+                                        sort_shape<1024, 4, false, 16>(), sort_shape<256, 16, true, 16>(),
+                                        sort_shape<512, 4, true, 16>(),   sort_shape<512, 8, false, 16>(),
+                                        sort_shape<256, 4, true, 16>(),   sort_shape<1024, 4, true, 16>() };
     static const int pick = [] {
         const char* e = std::getenv("NBHIP_RS_SHAPE");
         const int k = e ? std::atoi(e) : -1;
@@ -570,8 +571,8 @@ int sort_blocks(int n)
 // Look-back words per pass: one row of 256 per tile, then one per super-tile.
 size_t rs_status_tiles(int n)
 {
-    const size_t nb = size_t(sort_blocks(n));
-    return nb + (nb + RS_SUPER - 1) / RS_SUPER;
+    const size_t nb = size_t(sort_blocks(n)), sup = sort_config(n).super;
+    return nb + (nb + sup - 1) / sup;
 }
 
 constexpr int RS_COUNT_BLOCK = 1024;
