@@ -37,7 +37,8 @@
 //                  search over the levels per position).  Pre-order is
 //                  "for a = 0..n-1: nodes starting at a, shallow to deep, then leaf a",
 //                  so a scan of per-position counts gives every node's index, and a node
-//                  covering bodies [a, b] has next = base[b + 1].
+//                  covering bodies [a, b] has next = base[b + 1].  k_emit also records
+//                  the root's children by octant, for the split traversal.
 // Cells of 21 levels that still hold several bodies, and (leaf_size > 1) cells of at
 // most leaf_size bodies, become buckets: tested like an internal node, applied body by
 // body when opened.
