@@ -295,10 +295,11 @@ def run_batch(args) -> dict:
 
 
 def run_bh(args) -> dict:
-    """N = 2^20 exponential disk (R_d 3, z_0 0.3, 80% of the mass) + Plummer bulge (a 0.5),
-    theta 0.5 (reference criterion), leaf 16, eps 0.01, dt 1e-3, 10 steps, root cube 128."""
+    """N = 2^20 (--bh-bodies) exponential disk (R_d 3, z_0 0.3, 80% of the mass) + Plummer
+    bulge (a 0.5), theta 0.5 (reference criterion), leaf 16, eps 0.01, dt 1e-3, 10 steps,
+    root cube 128."""
     from nbody_amd import _lib, ics
-    n = 1 << 20
+    n = max(2, args.bh_bodies)
     size = 128.0
     bodies = ics.disk_bulge(n, seed=SEED)
     p = _lib.params(mode=_lib.MODE_NLOGN, theta=0.5, leaf_size=16, eps=0.01, dt=1e-3, size=size,
@@ -309,7 +310,7 @@ def run_bh(args) -> dict:
     # the textbook criterion at the same theta first, for comparison; then the reference's
     pt = _lib.params(mode=_lib.MODE_NLOGN, theta=0.5, leaf_size=16, eps=0.01, dt=1e-3, size=size,
                      wrap=1, opening=_lib.OPEN_TEXTBOOK)
-    sample = np.random.default_rng(SEED).choice(n, 1024, replace=False).astype(np.uint32)
+    sample = np.random.default_rng(SEED).choice(n, min(n, 1024), replace=False).astype(np.uint32)
     ctx.accelerate(pt)
     _, rel_text = ctx.force_error_f64(pt, sample)
     _, inter = ctx.bh_interactions(p)    # also leaves the first pass's accelerations
@@ -328,7 +329,7 @@ def run_bh(args) -> dict:
     e1 = ctx.diagnostics(p)
     ctx.close()
     out = {
-        "workload": "Barnes-Hut N=2^20 disk+bulge, theta=0.5 (reference criterion), leaf 16, "
+        "workload": f"Barnes-Hut N={n} disk+bulge, theta=0.5 (reference criterion), leaf 16, "
                     "eps=0.01, dt=1e-3, 10 steps, tree rebuilt every step",
         "steps_per_s": steps / (ms / 1e3), "ms_per_step": ms / steps,
         "ms_per_step_graph": ms_graph / steps,
@@ -427,6 +428,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--bh-bodies", type=int, default=1 << 20,
+                    help="bodies of the Barnes-Hut leg (the C3 workload is 2^20; smaller sizes are "
+                         "the served jobs' range)")
     ap.add_argument("--only", choices=["direct", "batch", "integrate", "bh", "served"], default=None,
                     help="run one workload (the headline fields then describe it)")
     args = ap.parse_args()
