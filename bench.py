@@ -436,7 +436,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the multi-GPU "
-              "sharding of this solver is a design, not a build; see DESIGN.md",
+              "sharding of this solver is designed but not built",
               file=sys.stderr)
         return 2
     if args.only in (None, "direct"):
