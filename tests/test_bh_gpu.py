@@ -387,6 +387,30 @@ def test_walk_split_matches_single_wave(ctx, case):
         ctx.bh_configure(0)
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 65, 129, 1000])
+def test_small_and_ragged_systems_every_split(ctx, orc, n):
+    # groups that are mostly empty lanes, and a last group cut short, under every split:
+    # the traversal counts equal the FP64-centre reference's and the forces match it
+    b = ics.plummer(n, seed=40 + n)
+    p = bh_params(size=64.0, theta=0.5, eps=0.01)
+    x64 = b.copy()
+    _, xv = orc.bh_accelerate_f64com(x64, 64.0, theta=0.5, eps=0.01, visits=True)
+    try:
+        for split in (1, 2, 4, 8):
+            ctx.bh_configure(split)
+            ctx.upload_bodies(b)
+            visits, total = ctx.bh_interactions(p)
+            out = ctx.download_bodies()
+            assert np.array_equal(visits, xv.astype(visits.dtype)), split
+            assert total == visits.sum()
+            if n == 1:
+                assert np.all(out[:, 8:11] == 0.0)
+            else:
+                assert rel_err(out[:, 8:11], x64[:, 8:11]).max() <= 1e-5, split
+    finally:
+        ctx.bh_configure(0)
+
+
 def test_walk_split_fused_step(ctx):
     # the fused kick-drift runs in the first wave of a group once the parts' sums are in
     b = ics.disk_bulge(6000, seed=37)
