@@ -411,6 +411,28 @@ def test_small_and_ragged_systems_every_split(ctx, orc, n):
         ctx.bh_configure(0)
 
 
+def test_walk_split_change_recaptures_graph(ctx):
+    # a step graph captured under one split is not replayed under another: each replay
+    # equals eager stepping under the split in force
+    b = ics.disk_bulge(20000, seed=39)
+    p = bh_params(size=128.0, theta=0.5, eps=0.01, dt=1e-3, wrap=1, leaf_size=16)
+    try:
+        res = {}
+        for split in (2, 8):
+            ctx.bh_configure(split)
+            ctx.upload_bodies(b)
+            ctx.step(p, 3)
+            eager = ctx.download_bodies()
+            ctx.upload_bodies(b)
+            ctx.step_graph(p, 3)
+            graph = ctx.download_bodies()
+            assert np.array_equal(graph.view(np.uint32), eager.view(np.uint32)), split
+            res[split] = eager
+        assert not np.array_equal(res[2], res[8])
+    finally:
+        ctx.bh_configure(0)
+
+
 def test_walk_split_fused_step(ctx):
     # the fused kick-drift runs in the first wave of a group once the parts' sums are in
     b = ics.disk_bulge(6000, seed=37)
