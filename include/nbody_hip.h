@@ -103,15 +103,21 @@ int nbhip_create(int device, nbhip_ctx** out);
 /* Release every device resource of the context.  NULL is a no-op. */
 void nbhip_destroy(nbhip_ctx* ctx);
 
-/* Text of the last failure on `ctx` (or of the last failed nbhip_create when ctx is
- * NULL).  Empty string when there was none.  Valid until the next call on ctx. */
+/* Text of the last failure on `ctx` (or, when ctx is NULL, of this thread's last failed
+ * nbhip_create or call made with a NULL context).  Empty string when there was none.
+ * Valid until the next call on ctx. */
 const char* nbhip_last_error(const nbhip_ctx* ctx);
 
 /* ---- body storage (reference GpuDevice::reserve_bodies / map_bodies / download,
  *      source/gpu.h:169-189) --------------------------------------------------------- */
 
 /* Set the body count to n, growing device storage if needed (it never shrinks).  A
- * grow discards device contents; the caller must upload the whole range again. */
+ * grow discards device contents; the caller must upload the whole range again.
+ * NBHIP_ERR_INVALID past NBHIP_MAX_BODIES (kernels index bodies with int).  A
+ * Barnes-Hut build (NBHIP_MODE_NLOGN) takes at most NBHIP_MAX_NLOGN_BODIES and returns
+ * NBHIP_ERR_UNSUPPORTED beyond. */
+#define NBHIP_MAX_BODIES       2147418112u   /* 2^31 - 2^16 */
+#define NBHIP_MAX_NLOGN_BODIES 1073741823u   /* 2^30 - 1    */
 int nbhip_reserve(nbhip_ctx* ctx, size_t n);
 
 /* Current body count. */
@@ -161,6 +167,13 @@ int nbhip_step_graph(nbhip_ctx* ctx, const nbhip_params* p, int steps);
  * invalid.  (The wait only ends this way if the device stops running a block that has
  * started, which a working device does not.) */
 int nbhip_synchronize(nbhip_ctx* ctx);
+
+/* Every call that waits for Barnes-Hut results to reach the host (nbhip_download,
+ * nbhip_download_bodies, nbhip_collect_bodies) reports such a give-up the same way, so a
+ * caller that never calls nbhip_synchronize still sees it.
+ * Testing hook, no reference counterpart: sets that fault from the host, as a give-up
+ * would, without running a kernel, so a test can check that it is reported. */
+int nbhip_test_raise_sort_fault(nbhip_ctx* ctx);
 
 /* Non-blocking: *idle = 1 when all work enqueued on the context has finished, else 0
  * (hipStreamQuery).  Lets a scheduler keep several contexts busy at once. */

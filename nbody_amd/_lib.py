@@ -24,6 +24,9 @@ MODE_NLOGN = 1
 OPEN_REFERENCE = 0
 OPEN_TEXTBOOK = 1
 
+MAX_BODIES = 2147418112         # NBHIP_MAX_BODIES, 2^31 - 2^16
+MAX_NLOGN_BODIES = (1 << 30) - 1  # NBHIP_MAX_NLOGN_BODIES
+
 RB_NONE = 0
 RB_POSITIONS = 1
 RB_VELOCITIES = 2
@@ -110,6 +113,7 @@ SIGNATURES = {
     "nbhip_step": (C.c_int, [_vp, C.POINTER(Params), C.c_int]),
     "nbhip_step_graph": (C.c_int, [_vp, C.POINTER(Params), C.c_int]),
     "nbhip_synchronize": (C.c_int, [_vp]),
+    "nbhip_test_raise_sort_fault": (C.c_int, [_vp]),
     "nbhip_query": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "nbhip_readback": (C.c_int, [_vp]),
     "nbhip_collect_bodies": (C.c_int, [_vp, _vp, _sz, C.POINTER(C.c_int)]),
@@ -313,6 +317,10 @@ class Context:
 
     def synchronize(self) -> None:
         self._check(self.L.nbhip_synchronize(self.h))
+
+    def raise_sort_fault(self) -> None:
+        """Testing hook: flag a radix-sort give-up from the host (no kernel runs)."""
+        self._check(self.L.nbhip_test_raise_sort_fault(self.h))
 
     def idle(self) -> bool:
         flag = C.c_int()

@@ -143,6 +143,7 @@ struct nbhip_batch {
     long g_version = -1;
     int g_P = 0, g_W = 0;
     bool g_persistent = false;
+    const unsigned* g_counters = nullptr;  // the step counters the graph was captured with
 
     std::map<int, int> grid_cache;  // persistent grid per (P, W)
 
@@ -361,7 +362,8 @@ int nbhip_batch_add(nbhip_batch* b, const void* bodies, size_t n, const nbhip_pa
 {
     return batch_guarded(b, [&] {
         if (!bodies || !p || n == 0) fail(NBHIP_ERR_INVALID, "a batch system needs bodies and params");
-        if (n > (size_t(1) << 30)) fail(NBHIP_ERR_INVALID, "system of %zu bodies too large", n);
+        if (n > nbhip::MAX_BODIES)
+            fail(NBHIP_ERR_INVALID, "system of %zu bodies exceeds the limit of %zu", n, nbhip::MAX_BODIES);
         if (p->mode != NBHIP_MODE_N2) fail(NBHIP_ERR_UNSUPPORTED, "the batch runs direct-sum systems only");
         if (!(p->eps >= 0.f)) fail(NBHIP_ERR_INVALID, "eps must be >= 0");
         hipStream_t s = b->ctx->stream;
@@ -394,7 +396,8 @@ int nbhip_batch_add_generated(nbhip_batch* b, const nbhip_ics* spec, size_t n, c
     return batch_guarded(b, [&] {
         nbhip::check_ics(spec, n);
         if (!p || n == 0) fail(NBHIP_ERR_INVALID, "a batch system needs bodies and params");
-        if (n > (size_t(1) << 30)) fail(NBHIP_ERR_INVALID, "system of %zu bodies too large", n);
+        if (n > nbhip::MAX_BODIES)
+            fail(NBHIP_ERR_INVALID, "system of %zu bodies exceeds the limit of %zu", n, nbhip::MAX_BODIES);
         if (p->mode != NBHIP_MODE_N2) fail(NBHIP_ERR_UNSUPPORTED, "the batch runs direct-sum systems only");
         if (!(p->eps >= 0.f)) fail(NBHIP_ERR_INVALID, "eps must be >= 0");
         hipStream_t s = b->ctx->stream;
@@ -455,7 +458,8 @@ int nbhip_batch_step(nbhip_batch* b, int steps, unsigned flags)
         if (flags & NBHIP_BATCH_GRAPH) {
             const bool reuse = b->exec && b->g_steps == steps && b->g_parity == b->parity &&
                                b->g_flags == int(flags) && b->g_version == b->version && b->g_P == b->P &&
-                               b->g_W == b->W && b->g_persistent == b->persistent;
+                               b->g_W == b->W && b->g_persistent == b->persistent &&
+                               b->g_counters == b->d_counters.ptr && size_t(steps) <= b->d_counters.cap;
             if (!reuse) {
                 b->drop_graph();
                 nbhip::prepare_steps(b, steps);  // allocate and query before capture
@@ -477,6 +481,7 @@ int nbhip_batch_step(nbhip_batch* b, int steps, unsigned flags)
                 b->g_P = b->P;
                 b->g_W = b->W;
                 b->g_persistent = b->persistent;
+                b->g_counters = b->d_counters.ptr;
             }
             NBHIP_CHECK(hipGraphLaunch(b->exec, s));
         } else {

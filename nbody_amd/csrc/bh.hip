@@ -1608,10 +1608,17 @@ __global__ __launch_bounds__(256) void k_tree_emit(TreeArgs t)
 
 unsigned grid(size_t n) { return unsigned((n + 255) / 256); }
 
+void check_nlogn_count(size_t n)
+{
+    if (n > MAX_NLOGN_BODIES)
+        fail(NBHIP_ERR_UNSUPPORTED, "a Barnes-Hut build takes at most %zu bodies, not %zu", MAX_NLOGN_BODIES, n);
+}
+
 }  // namespace
 
 void bh_reserve(nbhip_ctx* c)
 {
+    check_nlogn_count(c->count);
     const int n = int(c->count);
     hipStream_t s = c->stream;
     c->bh_keys[0].grow(n, 0, s);
@@ -1644,8 +1651,20 @@ void bh_check_fault(nbhip_ctx* c)
                         "last synchronisation are invalid");
 }
 
+void bh_raise_sort_fault(nbhip_ctx* c)
+{
+    if (!c->bh_rs_state.ptr) {
+        c->bh_rs_state.grow(RS_WORDS, 0, c->stream);
+        NBHIP_CHECK(hipMemsetAsync(c->bh_rs_state.ptr, 0, RS_WORDS * sizeof(unsigned), c->stream));
+    }
+    const unsigned one = 1u;
+    NBHIP_CHECK(hipMemcpyAsync(c->bh_rs_state.ptr + RS_FAULT, &one, sizeof one, hipMemcpyHostToDevice, c->stream));
+    NBHIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
 void bh_build(nbhip_ctx* c, const nbhip_params& p)
 {
+    check_nlogn_count(c->count);
     const int n = int(c->count);
     hipStream_t s = c->stream;
     c->bh_keys[0].grow(n, 0, s);
@@ -1784,6 +1803,7 @@ void bh_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse, bool count)
 
 void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm)
 {
+    check_nlogn_count(c->count);
     const int n = int(c->count);
     hipStream_t s = c->stream;
     for (int i = 0; i < 2; ++i) {

@@ -31,6 +31,14 @@ struct Error {
                           hipGetErrorString(_e), __FILE__, __LINE__);                      \
     } while (0)
 
+// Largest body count of a context or batch system.  Kernels index bodies with int, and a
+// tile's padding lanes reach up to a few thousand positions past the last body, so the
+// limit keeps every such index below INT_MAX.
+constexpr size_t MAX_BODIES = (size_t(1) << 31) - (size_t(1) << 16);
+// Largest body count of a Barnes-Hut build: the radix sort's look-back words carry digit
+// totals in 30 bits.
+constexpr size_t MAX_NLOGN_BODIES = (size_t(1) << 30) - 1;
+
 // Bumped whenever any device buffer moves: captured graphs bake buffer addresses and are
 // replayed only while the epoch they were captured at is current.
 extern std::atomic<unsigned long long> g_buffer_epoch;
@@ -221,6 +229,8 @@ void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm
 void bh_reserve(nbhip_ctx* c);
 // Reports (and clears) a radix-sort look-back that gave up waiting; call with the stream idle.
 void bh_check_fault(nbhip_ctx* c);
+// Sets the fault word from the host, as a look-back that gave up waiting would (tests only).
+void bh_raise_sort_fault(nbhip_ctx* c);
 size_t bh_tree(nbhip_ctx* c, const nbhip_params* p);
 void check_ics(const nbhip_ics* s, size_t n);
 bool ics_have_radius(const nbhip_ics& s);

@@ -320,7 +320,7 @@ unsigned grid(size_t n) { return unsigned((n + 255) / 256); }
 void check_ics(const nbhip_ics* s, size_t n)
 {
     if (!s) fail(NBHIP_ERR_INVALID, "null initial-condition spec");
-    if (n > (size_t(1) << 31)) fail(NBHIP_ERR_INVALID, "body count %zu too large", n);
+    if (n > MAX_BODIES) fail(NBHIP_ERR_INVALID, "body count %zu exceeds the limit of %zu", n, MAX_BODIES);
     if (!(s->G >= 0.f)) fail(NBHIP_ERR_INVALID, "G must be >= 0");
     switch (s->kind) {
     case NBHIP_IC_PLUMMER:

@@ -29,24 +29,32 @@ namespace {
 
 thread_local std::string g_create_error;  // nbhip_last_error(NULL)
 
-// Run `f`, turning any library error into a status code and the context's message.
+// Run `f`, turning any library error into a status code and the context's message (the
+// thread's NULL-context message when the call had no context).
 template <typename F>
 int guarded(nbhip_ctx* c, F&& f)
 {
+    std::string& err = c ? c->err : g_create_error;
     try {
-        if (c) c->err.clear();
+        err.clear();
         f();
         return NBHIP_OK;
     } catch (const Error& e) {
-        if (c) c->err = e.msg;
+        err = e.msg;
         return e.status;
     } catch (const std::bad_alloc&) {
-        if (c) c->err = "host allocation failed";
+        err = "host allocation failed";
         return NBHIP_ERR_NOMEM;
     } catch (...) {
-        if (c) c->err = "unknown error";
+        err = "unknown error";
         return NBHIP_ERR_HIP;
     }
+}
+
+void check_count(size_t n)
+{
+    if (n > nbhip::MAX_BODIES)
+        fail(NBHIP_ERR_INVALID, "body count %zu exceeds the limit of %zu", n, nbhip::MAX_BODIES);
 }
 
 void check_ctx(const nbhip_ctx* c)
@@ -192,9 +200,9 @@ const char* nbhip_last_error(const nbhip_ctx* c)
 int nbhip_reserve(nbhip_ctx* c, size_t n)
 {
     return guarded(c, [&] {
+        check_count(n);
         check_ctx(c);
         ++c->body_version;  // the staged readback no longer matches
-        if (n > (size_t(1) << 31)) fail(NBHIP_ERR_INVALID, "body count %zu too large", n);
         NBHIP_CHECK(hipSetDevice(c->device));
         const size_t keep = c->count;
         c->pm[0].grow(n, keep, c->stream);
@@ -264,6 +272,7 @@ int nbhip_download(nbhip_ctx* c, unsigned mask, float* pos_mass, float* vel_radi
                                        c->stream));
         }
         NBHIP_CHECK(hipStreamSynchronize(c->stream));
+        nbhip::bh_check_fault(c);
     });
 }
 
@@ -285,8 +294,8 @@ int nbhip_upload_bodies(nbhip_ctx* c, const void* bodies, size_t n)
 int nbhip_generate(nbhip_ctx* c, const nbhip_ics* spec, size_t n)
 {
     int st = guarded(c, [&] {
-        check_ctx(c);
         nbhip::check_ics(spec, n);
+        check_ctx(c);
     });
     if (st == NBHIP_OK) st = nbhip_reserve(c, n);
     if (st != NBHIP_OK) return st;
@@ -340,6 +349,7 @@ int nbhip_collect_bodies(nbhip_ctx* c, void* bodies, size_t n, int* staged)
             const size_t k = n < c->count ? n : c->count;
             if (k && !bodies) fail(NBHIP_ERR_INVALID, "null bodies");
             NBHIP_CHECK(hipEventSynchronize(c->ev_staged));
+            nbhip::bh_check_fault(c);
             if (k) std::memcpy(bodies, c->staging, 48 * k);
             if (staged) *staged = 1;
         });
@@ -359,6 +369,7 @@ int nbhip_download_bodies(nbhip_ctx* c, void* bodies, size_t n)
         nbhip::split_to_aos(c, k);
         NBHIP_CHECK(hipMemcpyAsync(bodies, c->aos.ptr, 48 * k, hipMemcpyDeviceToHost, c->stream));
         NBHIP_CHECK(hipStreamSynchronize(c->stream));
+        nbhip::bh_check_fault(c);
     });
 }
 
@@ -488,6 +499,15 @@ int nbhip_synchronize(nbhip_ctx* c)
         NBHIP_CHECK(hipSetDevice(c->device));
         NBHIP_CHECK(hipStreamSynchronize(c->stream));
         nbhip::bh_check_fault(c);
+    });
+}
+
+int nbhip_test_raise_sort_fault(nbhip_ctx* c)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::bh_raise_sort_fault(c);
     });
 }
 

@@ -42,3 +42,27 @@ def test_no_device_is_reported_not_raised():
     from nbody_amd import _lib
     ok, reason = _lib.probe()
     assert ok or reason
+
+
+def test_body_limits_refused_without_a_device():
+    # the count checks come before the context check, so they answer on any machine; the
+    # text of a failure with no context is the thread's NULL-context message
+    import ctypes as C
+    from nbody_amd import _lib
+    L = _lib.load()
+    assert L.nbhip_reserve(None, _lib.MAX_BODIES + 1) == _lib.ERR_INVALID
+    assert b"exceeds the limit" in L.nbhip_last_error(None)
+    assert L.nbhip_reserve(None, 2 ** 31) == _lib.ERR_INVALID
+    assert L.nbhip_reserve(None, 2 ** 40) == _lib.ERR_INVALID
+    assert b"exceeds the limit" in L.nbhip_last_error(None)
+    # within the limit the missing context is what is reported
+    assert L.nbhip_reserve(None, _lib.MAX_BODIES) == _lib.ERR_INVALID
+    assert b"null context" in L.nbhip_last_error(None)
+    spec = _lib.ics(_lib.IC_PLUMMER)
+    assert L.nbhip_generate(None, C.byref(spec), 2 ** 31) == _lib.ERR_INVALID
+    assert b"exceeds the limit" in L.nbhip_last_error(None)
+    assert L.nbhip_generate(None, C.byref(spec), 1000) == _lib.ERR_INVALID
+    assert b"null context" in L.nbhip_last_error(None)
+    # the largest count keeps a tile's padding lanes inside int
+    assert _lib.MAX_BODIES + 4096 < 2 ** 31 - 1
+    assert _lib.MAX_NLOGN_BODIES < 2 ** 30

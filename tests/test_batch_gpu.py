@@ -148,3 +148,28 @@ def test_rejects_bad_input(ctx):
             batch.remove(12345)
         with pytest.raises(_lib.HipError):
             batch.configure(3, 3)
+
+
+def test_graph_after_long_eager_call_equals_eager(ctx):
+    # an eager call of more steps than the step counters hold reallocates them; a graph
+    # captured before must not be replayed on the freed counters afterwards
+    sy = systems()[:4]
+    res = []
+    for graph in (True, False):
+        with _lib.Batch(ctx) as batch:
+            ids = [batch.add(b, p) for b, p in sy]
+            f = _lib.BATCH_GRAPH if graph else 0
+            batch.step(2, f)
+            batch.step(300)
+            batch.step(2, f)
+            res.append([batch.download(i) for i in ids])
+    for g, e in zip(*res):
+        assert np.array_equal(bits(g), bits(e))
+
+
+def test_rejects_systems_past_the_body_limit(ctx):
+    spec = _lib.ics(_lib.IC_PLUMMER)
+    with _lib.Batch(ctx) as batch:
+        with pytest.raises(_lib.HipError) as e:
+            batch.add_generated(spec, _lib.MAX_BODIES + 1, _lib.params(eps=0.01))
+        assert e.value.status == _lib.ERR_INVALID
