@@ -16,6 +16,11 @@
  *   orc_bh_accelerate_f64com  the same tree and traversal with FP64 centres of mass and
  *                        FP64 sums (separates the reference's FP32 rounding from the
  *                        Barnes-Hut approximation itself)
+ *   orc_bh_accelerate_buckets  the reference tree and traversal with the library's leaf
+ *                        buckets: a cell of at most leaf_size bodies (or 21 levels deep) is
+ *                        tested whole and, when opened, applied body by body; FP64 centres
+ *                        and sums (no reference counterpart: the reference keeps one body
+ *                        per leaf, source/bhtree.cpp:14-95)
  *   orc_morton_keys      the Bounds::quadrant descent of source/bhtree.cpp:19-24 written
  *                        as a key: 3 bits per level, digit = reference octant code
  *   orc_stable_sort      stable ascending sort of keys, ties by index (std::stable_sort)
@@ -632,6 +637,225 @@ void orc_bh_accelerate_f64com(float* bodies, size_t n, const orc_node* nodes, si
     orc_bh64_arg a = { bodies, nodes, nn, theta, G, eps * eps, nm, visits };
     orc_parallel_blocks(n, nthreads, orc_bh64_block, &a);
     free(nm);
+}
+
+/* Barnes-Hut with leaf buckets, stated on the reference's own tree.  No reference
+ * counterpart: the reference subdivides every cell of two or more bodies
+ * (source/bhtree.cpp:14-95).  The library's leaf_size option stops subdividing a cell
+ * once it holds at most leaf_size bodies, or once it sits 21 levels below the root (the
+ * depth of a 63-bit key): such a cell is a bucket.  The traversal is the reference's
+ * Tree::apply (source/bhtree.cpp:97-138) with one change: a bucket cell is tested as any
+ * cell is, and when the test opens it the walk does not descend into its subcells but
+ * applies each of its bodies one by one (every one counted as a visit, the receiving
+ * body itself included, which adds nothing).  Subcells of a bucket that hold all of its
+ * bodies (a chain of one-child cells) are walked first, as the reference walks them:
+ * each is tested in turn, so the bucket is applied whole when its deepest such cell
+ * passes, the same outcome as testing that cell alone.
+ *
+ * Body counts, centres and masses come from the bodies themselves: each body descends
+ * the reference tree by the reference's octant test (source/bhtree.cpp:19-24) and adds
+ * its mass and moments, in FP64, to every cell on its way.  The acceptance test uses the
+ * FP64 centre rounded to FP32 with the reference's FP32 operation order and edge
+ * (bhtree.cpp:125-128; the textbook criterion when `textbook` is set, size^2 / theta^2 as
+ * the threshold); applied forces are summed in FP64.  With leaf_size 1 and no two bodies
+ * in one 21-level cell this is orc_bh_accelerate_f64com.  The tree must come from
+ * orc_bh_build on the same bodies with a root edge of at least 2^21 FLT_EPSILON (~0.25),
+ * so that the reference keeps splitting to 21 levels.  Returns 0, or -1 when out of
+ * memory. */
+typedef struct {
+    float* b; const orc_node* nodes; size_t nn; float theta; int textbook, leaf;
+    double G, eps2;
+    const uint32_t* cnt;      /* bodies per cell */
+    const double* mom;        /* per cell: m, m x, m y, m z */
+    const uint8_t* level;     /* levels below the root */
+    const uint32_t* list_of;  /* bucket cell -> its bucket's list, UINT32_MAX if none */
+    const uint32_t* lstart;   /* list k: bodies lbody[lstart[k] .. lstart[k + 1]) */
+    const uint32_t* lbody;
+    uint64_t* visits;
+} orc_bhb_arg;
+
+static void orc_bhb_block(void* p, size_t begin, size_t end)
+{
+    const orc_bhb_arg* a = (const orc_bhb_arg*)p;
+    const float theta_sq = a->theta * a->theta;
+    for (size_t i = begin; i < end; ++i) {
+        float* body = a->b + i * BF;
+        const double x = body[0], y = body[1], z = body[2];
+        const double r2 = (double)body[3] * (double)body[3];
+        double acc[3] = { 0, 0, 0 };
+        uint64_t visits = 0;
+        uint32_t idx = 0;
+        do {
+            const orc_node* nd = &a->nodes[idx];
+            const double M = a->mom[4 * idx];
+            const uint32_t c = a->cnt[idx];
+            if (c == 0 || M == 0) {
+                idx = nd->next;
+                continue;
+            }
+            const double cx = a->mom[4 * idx + 1] / M, cy = a->mom[4 * idx + 2] / M, cz = a->mom[4 * idx + 3] / M;
+            int apply = c == 1;
+            if (!apply) {
+                const float fx = (float)cx - body[0], fy = (float)cy - body[1], fz = (float)cz - body[2];
+                const float dist_sq = ((fx * fx) + (fy * fy)) + (fz * fz);
+                const float s2 = nd->size * nd->size;
+                apply = dist_sq > (a->textbook ? s2 / theta_sq : s2 * theta_sq);
+            }
+            if (apply) {
+                const double dx = cx - x, dy = cy - y, dz = cz - z;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 > r2) {
+                    const double e2 = d2 + a->eps2;
+                    const double w = a->G * M / (e2 * sqrt(e2));
+                    acc[0] += w * dx;
+                    acc[1] += w * dy;
+                    acc[2] += w * dz;
+                }
+                ++visits;
+                idx = nd->next;
+                continue;
+            }
+            const uint32_t list = a->list_of[idx];
+            if (list == UINT32_MAX) {
+                idx = nd->children; /* an ordinary cell opens */
+                continue;
+            }
+            /* a bucket cell opens: down its chain while one subcell holds all its bodies */
+            int chain = 0;
+            if (a->level[idx] < 21 && nd->children != 0)
+                for (uint32_t q = 0; q < 8; ++q)
+                    if (a->cnt[nd->children + q] == c)
+                        chain = 1;
+            if (chain) {
+                idx = nd->children;
+                continue;
+            }
+            for (uint32_t k = a->lstart[list]; k < a->lstart[list + 1]; ++k) {
+                const float* s = a->b + (size_t)a->lbody[k] * BF;
+                const double dx = (double)s[0] - x, dy = (double)s[1] - y, dz = (double)s[2] - z;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 > r2) {
+                    const double e2 = d2 + a->eps2;
+                    const double w = a->G * (double)s[7] / (e2 * sqrt(e2));
+                    acc[0] += w * dx;
+                    acc[1] += w * dy;
+                    acc[2] += w * dz;
+                }
+            }
+            visits += c;
+            idx = nd->next;
+        } while (0 < idx && idx < a->nn);
+        /* other threads read this body's position and mass (floats 0-2, 7), never these */
+        a->visits[i] = visits;
+        body[8] = (float)acc[0];
+        body[9] = (float)acc[1];
+        body[10] = (float)acc[2];
+    }
+}
+
+int orc_bh_accelerate_buckets(float* bodies, size_t n, const orc_node* nodes, size_t nn, float theta,
+                              int textbook, int leaf_size, double G, double eps, uint64_t* visits,
+                              int nthreads)
+{
+    if (nn == 0 || leaf_size < 1)
+        return -1;
+    uint32_t* cnt = (uint32_t*)calloc(nn, sizeof(uint32_t));
+    double* mom = (double*)calloc(4 * nn, sizeof(double));
+    uint8_t* level = (uint8_t*)calloc(nn, 1);
+    uint32_t* list_of = (uint32_t*)malloc(nn * sizeof(uint32_t));
+    uint32_t* root_of = (uint32_t*)malloc((n ? n : 1) * sizeof(uint32_t));
+    uint32_t* lstart = NULL;
+    uint32_t* lbody = (uint32_t*)malloc((n ? n : 1) * sizeof(uint32_t));
+    uint64_t* own_visits = visits ? NULL : (uint64_t*)calloc(n ? n : 1, sizeof(uint64_t));
+    int rc = -1;
+    if (!cnt || !mom || !level || !list_of || !root_of || !lbody || (!visits && !own_visits))
+        goto out;
+    /* levels: children always follow their parent in the node array */
+    for (size_t i = 0; i < nn; ++i)
+        if (nodes[i].children != 0)
+            for (uint32_t q = 0; q < 8; ++q)
+                level[nodes[i].children + q] = (uint8_t)(level[i] + 1);
+    /* counts and FP64 moments along each body's descent */
+    for (size_t i = 0; i < n; ++i) {
+        const float* p = bodies + i * BF;
+        const double m = p[7];
+        uint32_t idx = 0;
+        for (;;) {
+            cnt[idx] += 1;
+            mom[4 * idx] += m;
+            mom[4 * idx + 1] += m * (double)p[0];
+            mom[4 * idx + 2] += m * (double)p[1];
+            mom[4 * idx + 3] += m * (double)p[2];
+            if (nodes[idx].children == 0)
+                break;
+            idx = nodes[idx].children + orc_quadrant(&nodes[idx], p);
+        }
+    }
+    /* each body's bucket: the first cell on its descent that stops subdividing (at most
+     * leaf_size bodies, 21 levels deep, or a leaf of the reference), if it holds two or
+     * more bodies */
+    for (size_t k = 0; k < nn; ++k)
+        list_of[k] = UINT32_MAX;
+    size_t lists = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const float* p = bodies + i * BF;
+        uint32_t idx = 0;
+        while (!(cnt[idx] <= (uint32_t)leaf_size || level[idx] >= 21 || nodes[idx].children == 0))
+            idx = nodes[idx].children + orc_quadrant(&nodes[idx], p);
+        root_of[i] = cnt[idx] >= 2 ? idx : UINT32_MAX;
+        if (cnt[idx] >= 2 && list_of[idx] == UINT32_MAX)
+            list_of[idx] = (uint32_t)lists++;
+    }
+    lstart = (uint32_t*)calloc(lists + 1, sizeof(uint32_t));
+    if (!lstart)
+        goto out;
+    for (size_t i = 0; i < n; ++i)
+        if (root_of[i] != UINT32_MAX)
+            lstart[list_of[root_of[i]] + 1] += 1;
+    for (size_t k = 0; k < lists; ++k)
+        lstart[k + 1] += lstart[k];
+    {
+        uint32_t* fill = (uint32_t*)malloc((lists ? lists : 1) * sizeof(uint32_t));
+        if (!fill)
+            goto out;
+        memcpy(fill, lstart, (lists ? lists : 1) * sizeof(uint32_t));
+        for (size_t i = 0; i < n; ++i)
+            if (root_of[i] != UINT32_MAX)
+                lbody[fill[list_of[root_of[i]]]++] = (uint32_t)i;
+        free(fill);
+    }
+    /* the cells of a bucket's chain (one subcell holding all its bodies) share its list */
+    for (size_t k = 0; k < nn; ++k) {
+        if (list_of[k] == UINT32_MAX)
+            continue;
+        uint32_t idx = (uint32_t)k;
+        while (level[idx] < 21 && nodes[idx].children != 0) {
+            uint32_t only = UINT32_MAX;
+            for (uint32_t q = 0; q < 8; ++q)
+                if (cnt[nodes[idx].children + q] == cnt[idx])
+                    only = nodes[idx].children + q;
+            if (only == UINT32_MAX)
+                break;
+            list_of[only] = list_of[k];
+            idx = only;
+        }
+    }
+    {
+        orc_bhb_arg a = { bodies, nodes, nn, theta, textbook, leaf_size, G, eps * eps, cnt, mom, level,
+                          list_of, lstart, lbody, visits ? visits : own_visits };
+        orc_parallel_blocks(n, nthreads, orc_bhb_block, &a);
+    }
+    rc = 0;
+out:
+    free(cnt);
+    free(mom);
+    free(level);
+    free(list_of);
+    free(root_of);
+    free(lstart);
+    free(lbody);
+    free(own_visits);
+    return rc;
 }
 
 size_t orc_node_size(void) { return sizeof(orc_node); }

@@ -237,3 +237,95 @@ def test_o3_build_is_bit_identical(orc):
             orc.variant("o2")
     for a, c in zip(runs[0], runs[1]):
         assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(c).view(np.uint8))
+
+
+# ---- the bucket-aware Barnes-Hut oracle (leaf_size > 1) ---------------------------------
+
+def _clumps(n, seed):
+    # many bodies in a few deep cells (long one-child chains) and exact duplicates (cells
+    # still holding several bodies at 21 levels)
+    rng = np.random.default_rng(seed)
+    b = ics.plummer(n, seed=seed)
+    centres = rng.normal(size=(12, 3)).astype(np.float32)
+    pick = rng.integers(0, 12, n)
+    b[:, 0:3] = centres[pick] + (rng.normal(size=(n, 3)) * 10.0 ** rng.integers(-6, -1, (n, 1))).astype(np.float32)
+    b[::37, 0:3] = b[1::37, 0:3][: b[::37].shape[0]]
+    return b
+
+
+def test_bucket_oracle_leaf1_is_the_f64com_oracle(orc):
+    # one body per leaf and no two bodies in a 21-level cell: the reference traversal with
+    # FP64 centres, visit for visit (the two oracles sum the centres in different orders)
+    for b, size in [(ics.disk_bulge(20000, seed=3), 128.0), (ics.plummer(5000, seed=9), 64.0)]:
+        keys = orc.morton_keys(b, size, 21)
+        assert np.unique(keys).size == keys.size
+        for theta in (0.5, 2.0):
+            x1 = b.copy()
+            _, v1 = orc.bh_accelerate_f64com(x1, size, theta=theta, eps=0.01, visits=True)
+            x2 = b.copy()
+            _, v2 = orc.bh_accelerate_buckets(x2, size, theta=theta, leaf_size=1, eps=0.01)
+            assert np.array_equal(v1, v2)
+            ref = x1[:, 8:11].astype(np.float64)
+            rel = np.linalg.norm(x2[:, 8:11] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+            assert rel.max() <= 1e-6
+
+
+@pytest.mark.parametrize("leaf", [1, 4, 16, 40])
+def test_bucket_oracle_every_cell_open_is_the_direct_sum(orc, leaf):
+    # theta -> infinity opens every cell and every bucket: each body visits every body once
+    b = ics.disk_bulge(6000, seed=leaf)
+    x = b.copy()
+    _, v = orc.bh_accelerate_buckets(x, 128.0, theta=1e6, leaf_size=leaf, eps=0.01)
+    assert np.all(v == b.shape[0])
+    a64 = orc.direct_f64(b, eps=0.01)
+    assert (np.linalg.norm(x[:, 8:11] - a64, axis=1) / np.linalg.norm(a64, axis=1)).max() <= 2e-7
+
+
+@pytest.mark.parametrize("leaf", [1, 2, 5, 16, 17, 40])
+@pytest.mark.parametrize("kind,textbook", [("disk", False), ("clumps", False), ("disk", True)])
+def test_bucket_oracle_matches_definition(orc, leaf, kind, textbook):
+    # the oracle (the reference's insertion tree, marked with buckets) against a traversal
+    # of the bucket tree written from its definition over key prefixes (tests/bh_definition.py):
+    # same visits body for body, forces equal to FP64 rounding
+    from bh_definition import bucket_tree, walk
+    n, size = 3000, 64.0
+    b = ics.disk_bulge(n, seed=20 + leaf) if kind == "disk" else _clumps(n, 20 + leaf)
+    theta = 0.6 if textbook else 0.7
+    x = b.copy()
+    _, v = orc.bh_accelerate_buckets(x, size, theta=theta, leaf_size=leaf, eps=0.01, textbook=textbook)
+    keys = orc.morton_keys(b, size, 21)
+    perm = orc.stable_sort(keys)
+    spm = b[perm][:, [0, 1, 2, 7]]
+    rows = bucket_tree(keys[perm], spm, size, theta, leaf, textbook=textbook)
+    acc, dv = walk(rows, spm, eps=0.01)
+    assert np.array_equal(v[perm], dv)
+    a = x[perm, 8:11].astype(np.float64)
+    rel = np.linalg.norm(a - acc, axis=1) / np.linalg.norm(acc, axis=1)
+    assert rel.max() <= 2e-7   # the oracle's FP32 output
+    if leaf > 2 and kind == "disk":
+        # buckets of three or more bodies change what is applied (an opened pair is its
+        # two leaves either way): the comparison would fail for a leaf-1 traversal
+        _, v1 = orc.bh_accelerate_buckets(b.copy(), size, theta=theta, leaf_size=1, eps=0.01, textbook=textbook)
+        assert not np.array_equal(v1, v)
+
+
+def test_bucket_oracle_catches_a_shallow_threshold(orc):
+    # a planted mistake in the definition's walk -- a bucket tested at the edge of the
+    # shallowest cell of its chain instead of the deepest -- must change the visits on
+    # clumpy input, so the comparison above would catch the same slip in the oracle
+    from bh_definition import bucket_tree, walk
+    n, size, theta, leaf = 3000, 64.0, 0.7, 16
+    b = _clumps(n, 7)
+    keys = orc.morton_keys(b, size, 21)
+    perm = orc.stable_sort(keys)
+    spm = b[perm][:, [0, 1, 2, 7]]
+    rows = bucket_tree(keys[perm], spm, size, theta, leaf)
+    _, good = walk(rows, spm, eps=0.01)
+    bad_rows = [list(r) for r in rows]
+    for r in bad_rows:
+        if r[1] > 1:
+            r[3] = np.float32(r[3] * np.float32(4.0))   # one level shallower
+    _, bad = walk(bad_rows, spm, eps=0.01)
+    assert not np.array_equal(good, bad)
+    _, v = orc.bh_accelerate_buckets(b.copy(), size, theta=theta, leaf_size=leaf, eps=0.01)
+    assert np.array_equal(v[perm], good)
