@@ -147,6 +147,23 @@ int nbhip_download_bodies(nbhip_ctx* ctx, void* bodies, size_t n);
 /* Accelerations of every body from the current positions (mode N2 or NLogN). */
 int nbhip_accelerate(nbhip_ctx* ctx, const nbhip_params* p);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, hosts that forbid hipMalloc).
+ * nbhip_accelerate in NBHIP_MODE_N2 over n bodies held in the CALLER's device arrays on
+ * the context's device, in the split layout (pos_mass, vel_radius read; acc written, 4*n
+ * floats each, 16-byte aligned), enqueued on the context's stream; does not wait and
+ * leaves the context's own bodies alone.  Allocates nothing: its scratch (the j-split
+ * partial sums and one flag) is a static 32 MiB device array that each GPU holds until
+ * the process ends; every call on a GPU shares it, so calls that overlap on one GPU (from
+ * any context or thread) race.  Results equal nbhip_accelerate's on a context holding
+ * the same bodies with the default launch shape (no nbhip_tune).  NBHIP_ERR_UNSUPPORTED
+ * in NLogN mode; NBHIP_ERR_INVALID for n past NBHIP_MAX_BODIES, arrays that are null,
+ * misaligned, not device memory of the context's device or shorter than n bodies, or a
+ * launch shape whose partial sums exceed the scratch (the default shapes need at most
+ * ~25 MiB at any n). */
+int nbhip_accelerate_noalloc(nbhip_ctx* ctx, const float* pos_mass, const float* vel_radius, float* acc,
+                             size_t n, const nbhip_params* p);
+
 /* Semi-implicit Euler kick-drift with optional torus wrap, from the stored
  * accelerations (reference detail::integrate_euler, source/detail/physics.h:60-70). */
 int nbhip_integrate(nbhip_ctx* ctx, const nbhip_params* p);

@@ -110,6 +110,7 @@ SIGNATURES = {
     "nbhip_download_bodies": (C.c_int, [_vp, _vp, _sz]),
     "nbhip_accelerate": (C.c_int, [_vp, C.POINTER(Params)]),
     "nbhip_integrate": (C.c_int, [_vp, C.POINTER(Params)]),
+    "nbhip_accelerate_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, _sz, C.POINTER(Params)]),
     "nbhip_step": (C.c_int, [_vp, C.POINTER(Params), C.c_int]),
     "nbhip_step_graph": (C.c_int, [_vp, C.POINTER(Params), C.c_int]),
     "nbhip_synchronize": (C.c_int, [_vp]),
@@ -304,6 +305,11 @@ class Context:
     # ---- stepping ----
     def accelerate(self, p: Params) -> None:
         self._check(self.L.nbhip_accelerate(self.h, C.byref(p)))
+
+    def accelerate_noalloc(self, pos_mass_dev: int, vel_radius_dev: int, acc_dev: int, n: int, p: Params) -> None:
+        """Direct-sum accelerations of n bodies in the caller's device arrays (addresses),
+        enqueued on this context's stream without any allocation."""
+        self._check(self.L.nbhip_accelerate_noalloc(self.h, pos_mass_dev, vel_radius_dev, acc_dev, n, C.byref(p)))
 
     def integrate(self, p: Params) -> None:
         self._check(self.L.nbhip_integrate(self.h, C.byref(p)))

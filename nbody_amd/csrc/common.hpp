@@ -218,6 +218,10 @@ void split_to_aos_raw(hipStream_t s, float* aos, size_t n, const float4* pm, con
                       const float4* acc);
 void split_to_aos(nbhip_ctx* c, size_t n);
 DirectConfig choose_direct_config(size_t n);
+// nbhip_accelerate_noalloc: takes the static scratch's address on `device` (current) once
+void noalloc_setup(int device);
+void accelerate_noalloc(nbhip_ctx* c, const float4* pm, const float4* vr, float4* acc, size_t n,
+                        const nbhip_params& p);
 void tune_direct(nbhip_ctx* c, const nbhip_params& p, std::vector<nbhip_tune_entry>& all,
                  nbhip_tune_entry& best);
 

@@ -9,6 +9,7 @@
 // order before finishing, so results never depend on the launch shape's timing.
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -34,6 +35,8 @@ struct ForceArgs {
     float dt;
     float size;
     int wrap;
+    const unsigned* gate;  // non-null: run only when (*gate != 0) == gate_want
+    int gate_want;
 };
 
 // Write the final acceleration of body i and, for OUT_STEP, advance it.
@@ -53,6 +56,7 @@ template <int P, int W, bool CUT>
 __global__ __launch_bounds__(WAVE* W) void k_force(ForceArgs a)
 {
     __shared__ ForceShared<P, W> sh;
+    if (a.gate && ((*a.gate != 0u) != (a.gate_want != 0))) return;  // the other variant's launch
     const int n = a.n;
     const int j0 = blockIdx.y * a.jchunk;
     const int j1 = min(n, j0 + a.jchunk);
@@ -131,6 +135,20 @@ __global__ __launch_bounds__(256) void k_max_radius(const float4* vr, int n, flo
         __syncthreads();
     }
     if (threadIdx.x == 0) out[blockIdx.x] = m[0];
+}
+
+// flag = 1 if any body has a radius (|radius| > 0, as k_max_radius decides); the flag is
+// cleared by k_clear_word first
+__global__ __launch_bounds__(256) void k_radius_flag(const float4* vr, int n, unsigned* flag)
+{
+    bool any = false;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) any |= fabsf(vr[i].w) > 0.f;
+    if (__ballot(any) != 0ull && (threadIdx.x & (WAVE - 1)) == 0) atomicOr(flag, 1u);
+}
+
+__global__ void k_clear_word(unsigned* w)
+{
+    if (threadIdx.x == 0) *w = 0u;
 }
 
 // ---- potential (diagnostics): phi_i = sum_j m_j / sqrt(|d|^2 + eps^2), cutoff as above --
@@ -352,23 +370,24 @@ void ensure_radius_flag(nbhip_ctx* c)
     c->radius_known = true;
 }
 
-void direct_pass(hipStream_t stream, const SysView& v, const nbhip_params& p, bool fuse,
-                 DevBuf<float>& partial, const DirectConfig& cfg)
+namespace {
+
+// j-chunk of a launch shape: whole tiles, at most cfg.jsplit chunks; returns the chunk
+// count actually used.
+int split_chunks(size_t n, const DirectConfig& cfg, size_t& jchunk)
 {
-    const size_t n = v.n;
-    const bool cut = v.any_radius || p.eps == 0.f;
-    ForceKernel k = pick_force(cfg.P, cfg.W, cut);
-    if (!k) fail(NBHIP_ERR_INVALID, "no force kernel for P=%d W=%d", cfg.P, cfg.W);
-
     const size_t tile = size_t(WAVE) * cfg.W;
-    size_t jchunk = (n + cfg.jsplit - 1) / cfg.jsplit;
+    jchunk = (n + cfg.jsplit - 1) / cfg.jsplit;
     jchunk = (jchunk + tile - 1) / tile * tile;
-    const int jsplit = int((n + jchunk - 1) / jchunk);
+    return int((n + jchunk - 1) / jchunk);
+}
 
+ForceArgs force_args(const SysView& v, const nbhip_params& p, size_t jchunk)
+{
     ForceArgs a{};
     a.pm = v.pm;
     a.vr = v.vr;
-    a.n = int(n);
+    a.n = int(v.n);
     a.G = p.G;
     a.eps2 = p.eps * p.eps;
     a.jchunk = int(jchunk);
@@ -378,24 +397,150 @@ void direct_pass(hipStream_t stream, const SysView& v, const nbhip_params& p, bo
     a.dt = p.dt;
     a.size = p.size;
     a.wrap = p.wrap;
-    const int final_mode = fuse ? OUT_STEP : OUT_ACC;
+    return a;
+}
 
+// The force kernel of one pass: with one j-chunk it finishes every body (out_mode), with
+// several it writes the partial sums for k_reduce.
+void launch_force(hipStream_t stream, ForceArgs a, int jsplit, const DirectConfig& cfg, bool cut)
+{
+    ForceKernel k = pick_force(cfg.P, cfg.W, cut);
+    if (!k) fail(NBHIP_ERR_INVALID, "no force kernel for P=%d W=%d", cfg.P, cfg.W);
     const size_t per_block = size_t(WAVE) * cfg.P;
-    const dim3 grid(unsigned((n + per_block - 1) / per_block), unsigned(jsplit));
+    const dim3 grid(unsigned((size_t(a.n) + per_block - 1) / per_block), unsigned(jsplit));
+    hipLaunchKernelGGL(k, grid, dim3(WAVE * cfg.W), 0, stream, a);
+    NBHIP_CHECK(hipGetLastError());
+}
+
+void launch_reduce(hipStream_t stream, const ForceArgs& a, int jsplit)
+{
+    hipLaunchKernelGGL(k_reduce, dim3(grid1(size_t(a.n))), dim3(256), 0, stream, a, jsplit);
+    NBHIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+void direct_pass(hipStream_t stream, const SysView& v, const nbhip_params& p, bool fuse,
+                 DevBuf<float>& partial, const DirectConfig& cfg)
+{
+    const size_t n = v.n;
+    const bool cut = v.any_radius || p.eps == 0.f;
+    size_t jchunk = 0;
+    const int jsplit = split_chunks(n, cfg, jchunk);
+    ForceArgs a = force_args(v, p, jchunk);
+    const int final_mode = fuse ? OUT_STEP : OUT_ACC;
     if (jsplit == 1) {
         a.out_mode = final_mode;
-        hipLaunchKernelGGL(k, grid, dim3(WAVE * cfg.W), 0, stream, a);
-        NBHIP_CHECK(hipGetLastError());
+        launch_force(stream, a, 1, cfg, cut);
     } else {
         partial.grow(size_t(jsplit) * 3 * n, 0, stream);
         a.partial = partial.ptr;
         a.out_mode = OUT_PARTIAL;
-        hipLaunchKernelGGL(k, grid, dim3(WAVE * cfg.W), 0, stream, a);
-        NBHIP_CHECK(hipGetLastError());
+        launch_force(stream, a, jsplit, cfg, cut);
         a.out_mode = final_mode;
-        hipLaunchKernelGGL(k_reduce, dim3(grid1(n)), dim3(256), 0, stream, a, jsplit);
-        NBHIP_CHECK(hipGetLastError());
+        launch_reduce(stream, a, jsplit);
     }
+}
+
+// ---- accelerations without allocation (nbhip_accelerate_noalloc) ----------------------
+// The scratch of that entry point: a flag word (does any body have a radius?) in the first
+// 256 bytes, then the j-split partial sums, 3 floats per body per j-chunk.  File-scope
+// device memory: each GPU the library runs on holds its own copy until the process ends,
+// and every call on that GPU shares it, so calls that overlap on one GPU race.  Its
+// address on each device is taken once, with that device current, by noalloc_setup; host
+// code only ever passes that address, never the array's name.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(32) << 20;
+constexpr size_t NOALLOC_PARTIAL_OFFSET = 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// the device array behind [ptr, ptr + bytes) must be on `device`
+void check_device_range(const void* ptr, size_t bytes, int device, const char* what)
+{
+    if (!ptr) fail(NBHIP_ERR_INVALID, "null %s", what);
+    if (reinterpret_cast<uintptr_t>(ptr) % 16 != 0) fail(NBHIP_ERR_INVALID, "%s is not 16-byte aligned", what);
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, ptr) != hipSuccess || at.type != hipMemoryTypeDevice || at.device != device) {
+        (void)hipGetLastError();
+        fail(NBHIP_ERR_INVALID, "%s is not device memory of device %d", what, device);
+    }
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<void*>(ptr)) != hipSuccess) {
+        (void)hipGetLastError();
+        fail(NBHIP_ERR_INVALID, "%s: no allocation range", what);
+    }
+    const uintptr_t b = reinterpret_cast<uintptr_t>(base), q = reinterpret_cast<uintptr_t>(ptr);
+    if (q + bytes > b + size) fail(NBHIP_ERR_INVALID, "%s holds fewer than the %zu bytes of n bodies", what, bytes);
+}
+
+}  // namespace
+
+void noalloc_setup(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        fail(NBHIP_ERR_UNSUPPORTED, "device %d beyond the %d the static scratch tracks", device, NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;
+    NBHIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));  // the current device's copy
+    if (!ptr) fail(NBHIP_ERR_HIP, "no device address for the static scratch");
+    g_scratch_ptr[device] = ptr;
+}
+
+void accelerate_noalloc(nbhip_ctx* c, const float4* pm, const float4* vr, float4* acc, size_t n,
+                        const nbhip_params& p)
+{
+    check_device_range(pm, n * sizeof(float4), c->device, "pos_mass");
+    check_device_range(vr, n * sizeof(float4), c->device, "vel_radius");
+    check_device_range(acc, n * sizeof(float4), c->device, "acc");
+    noalloc_setup(c->device);
+    unsigned char* base;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = static_cast<unsigned char*>(g_scratch_ptr[c->device]);
+    }
+    unsigned* flag = reinterpret_cast<unsigned*>(base);
+    float* partial = reinterpret_cast<float*>(base + NOALLOC_PARTIAL_OFFSET);
+    const size_t partial_cap = (NOALLOC_SCRATCH_BYTES - NOALLOC_PARTIAL_OFFSET) / sizeof(float);
+
+    const DirectConfig cfg = choose_direct_config(n);
+    size_t jchunk = 0;
+    const int jsplit = split_chunks(n, cfg, jchunk);
+    if (jsplit > 1 && size_t(jsplit) * 3 * n > partial_cap)
+        fail(NBHIP_ERR_INVALID, "%zu bodies in %d j-chunks need %zu bytes of partial sums, beyond the %zu-byte "
+             "static scratch", n, jsplit, size_t(jsplit) * 3 * n * sizeof(float), partial_cap * sizeof(float));
+    const SysView v{ pm, nullptr, const_cast<float4*>(vr), acc, n, false };
+    ForceArgs a = force_args(v, p, jchunk);
+    a.out_mode = jsplit == 1 ? OUT_ACC : OUT_PARTIAL;
+    a.partial = partial;
+    if (p.eps == 0.f) {
+        launch_force(c->stream, a, jsplit, cfg, true);  // the reference law always cuts off
+    } else {
+        // the cutoff variant exactly when some body has a radius, as direct_accelerate
+        // decides on the host: both launches read the flag, one of them returns at once
+        hipLaunchKernelGGL(k_clear_word, dim3(1), dim3(64), 0, c->stream, flag);
+        NBHIP_CHECK(hipGetLastError());
+        const unsigned nb = std::max(1u, std::min<unsigned>(1024, grid1(n)));
+        hipLaunchKernelGGL(k_radius_flag, dim3(nb), dim3(256), 0, c->stream, vr, int(n), flag);
+        NBHIP_CHECK(hipGetLastError());
+        a.gate = flag;
+        a.gate_want = 1;
+        launch_force(c->stream, a, jsplit, cfg, true);
+        a.gate_want = 0;
+        launch_force(c->stream, a, jsplit, cfg, false);
+        a.gate = nullptr;
+    }
+    if (jsplit > 1) {
+        a.out_mode = OUT_ACC;
+        launch_reduce(c->stream, a, jsplit);
+    }
+    c->launches += 1;
 }
 
 std::atomic<unsigned long long> g_buffer_epoch{ 0 };

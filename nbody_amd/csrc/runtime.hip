@@ -153,6 +153,7 @@ int nbhip_create(int device, nbhip_ctx** out)
         NBHIP_CHECK(hipEventCreate(&c->ev_start));
         NBHIP_CHECK(hipEventCreate(&c->ev_stop));
         NBHIP_CHECK(hipEventCreateWithFlags(&c->ev_staged, hipEventDisableTiming));
+        nbhip::noalloc_setup(device);  // this device's copy of the no-allocation scratch
         *out = c;
         return NBHIP_OK;
     } catch (const Error& e) {
@@ -385,6 +386,22 @@ int nbhip_accelerate(nbhip_ctx* c, const nbhip_params* p)
             nbhip::direct_accelerate(c, *p, false);
         else
             nbhip::bh_accelerate(c, *p, false, false);
+    });
+}
+
+int nbhip_accelerate_noalloc(nbhip_ctx* c, const float* pos_mass, const float* vel_radius, float* acc, size_t n,
+                             const nbhip_params* p)
+{
+    return guarded(c, [&] {
+        check_count(n);
+        check_ctx(c);
+        check_params(p);
+        if (p->mode != NBHIP_MODE_N2)
+            fail(NBHIP_ERR_UNSUPPORTED, "the no-allocation entry point runs the direct sum (NBHIP_MODE_N2) only");
+        if (n == 0) return;
+        NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::accelerate_noalloc(c, reinterpret_cast<const float4*>(pos_mass),
+                                  reinterpret_cast<const float4*>(vel_radius), reinterpret_cast<float4*>(acc), n, *p);
     });
 }
 
