@@ -11,6 +11,8 @@ library's stream.  The CPU baseline (oracle restatement of the reference brute-f
 loop) is timed on a bounded subset of rows on this host's cores.
 
 Secondary objects in the same line:
+  "c1"         the reference's CPU-runnable case: N = 4096 Plummer, 100 steps, GPU and the
+               CPU brute-force restatement over all 100 steps;
   "batched"    512 Plummer systems, N log-uniform in [256, 32768], 50 steps replayed from
                a hipGraph, against the same systems stepped one launch per system;
   "barnes_hut" N = 2^20 exponential disk + Plummer bulge, theta 0.5, leaf 16, 10 steps,
@@ -40,6 +42,37 @@ DT_C2 = 5e-4
 SEED = 2026
 CPU_THREADS_MAX = 16              # the GPU box's CPU share for one GPU
 CPU_TARGET_S = 10.0
+HBM_PEAK_GBS = 8000.0              # MI355X published HBM3E bandwidth
+
+
+def cpu_threads() -> int:
+    """Threads for the CPU baselines: every core this process may run on, capped at 16,
+    the CPU share of one GPU on the benchmark hosts (the host's other cores belong to
+    the other GPUs' jobs)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return max(1, min(CPU_THREADS_MAX, avail))
+
+
+def cpu_facts() -> dict:
+    """What the CPU baseline ran on: model, logical CPUs, usable CPUs, threads used."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": avail, "threads": cpu_threads(),
+            "thread_cap": f"{CPU_THREADS_MAX} (one GPU's CPU share on the benchmark hosts)"}
 
 
 def cpu_baseline(bodies: np.ndarray, eps: float) -> dict:
@@ -48,7 +81,7 @@ def cpu_baseline(bodies: np.ndarray, eps: float) -> dict:
     from oracle import oracle as O  # the CPU baseline leg is the only oracle user here
     if not (os.path.exists(O.LIB_PATH) and os.path.exists(O.LIB_O3_PATH)):
         O.build(ref=False)
-    threads = max(1, min(CPU_THREADS_MAX, os.cpu_count() or 1))
+    threads = cpu_threads()
     n = bodies.shape[0]
     rng = np.random.default_rng(SEED)
     work = bodies.copy()
@@ -80,7 +113,7 @@ def cpu_baseline(bodies: np.ndarray, eps: float) -> dict:
     return {"value": rate / 1e9, "unit": "G-interactions/s", "cores": threads, "kind": "port",
             "sample": f"{rows} seeded rows x all {n} bodies, 1 force pass, {t:.1f} s, "
                       f"oracle/nbody_oracle.c brute force (-O2, {threads} threads)",
-            "o3_march_x86_64_v3": {"value": rows * n / t3 / 1e9, "seconds": t3}}
+            "o3_march_x86_64_v3": {"value": rows * n / t3 / 1e9, "seconds": t3}, **cpu_facts()}
 
 
 def load_traffic(name: str):
@@ -180,6 +213,63 @@ def run_direct(args) -> dict:
     return out
 
 
+def run_c1(args) -> dict:
+    """The reference's CPU-runnable case: N = 4096 Plummer (a = 1, M = 1, G = 1), eps 0.01,
+    dt 1e-3, wrap off, 100 steps, on the GPU (autotuned launch shape; steps enqueued one
+    call per step, and replayed from a captured hipGraph) and on the CPU (the brute-force
+    restatement and the reference's integrator over all 100 steps).  At this size the
+    force pass is launch- and occupancy-bound: 32 blocks of 128 bodies per j-chunk."""
+    from nbody_amd import _lib, ics
+    n, steps = 4096, 100
+    bodies = ics.plummer(n, seed=SEED)
+    p = _lib.params(dt=1e-3, eps=0.01, size=10000.0, wrap=0, G=1.0)
+    ctx = _lib.Context(0)
+    ctx.upload_bodies(bodies)
+    _, tuned = ctx.tune(p)
+    ctx.upload_bodies(bodies)
+    ctx.step(p, 2)
+    ctx.synchronize()
+    ctx.timer_start()
+    for _ in range(steps):
+        ctx.step(p, 1)
+    ms, launches = ctx.timer_stop()
+    ctx.upload_bodies(bodies)
+    ctx.step_graph(p, steps)          # capture (untimed)
+    ctx.upload_bodies(bodies)
+    ctx.synchronize()
+    ctx.timer_start()
+    ctx.step_graph(p, steps)
+    ms_graph, _ = ctx.timer_stop()
+    gpu = ctx.download_bodies()
+    ctx.close()
+    inter = float(n) * n * steps
+    out = {
+        "workload": "N=4096 Plummer (a=1, M=1, G=1), eps=0.01, dt=1e-3, wrap off, 100 steps",
+        "value": inter / (ms / 1e3) / 1e9, "unit": "G-interactions/s",
+        "ms_per_step": ms / steps, "steps_per_s": steps / (ms / 1e3),
+        "graph": {"value": inter / (ms_graph / 1e3) / 1e9, "ms_per_step": ms_graph / steps},
+        "pct_fp32_peak": 100.0 * FLOPS_PER_INTERACTION * inter / (ms / 1e3) / (FP32_PEAK_TFLOPS * 1e12),
+        "launch": {"P": tuned[0], "W": tuned[1], "jsplit": tuned[2], "autotuned": True},
+        "bound": "launch/occupancy: 4096 bodies give 32 i-blocks of 128",
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = cpu_threads()
+        cpu = bodies.copy()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            O.brute_force(cpu, eps=0.01, threads=threads)
+            O.integrate(cpu, np.float32(1e-3), 10000.0, False, threads=threads)
+        t = time.perf_counter() - t0
+        dx = float(np.abs(gpu[:, 0:3].astype(np.float64) - cpu[:, 0:3]).max())
+        out["cpu_baseline"] = {"value": inter / t / 1e9, "unit": "G-interactions/s", "cores": threads,
+                               "kind": "port", "seconds": t, "steps_per_s": steps / t,
+                               "sample": "all 100 steps: oracle/nbody_oracle.c brute force + integrate (-O2)",
+                               **cpu_facts()}
+        out["max_abs_dx_gpu_vs_cpu_after_100_steps"] = dx
+    return out
+
+
 def run_integrate(args) -> dict:
     """The standalone semi-implicit Euler kick-drift (nbhip_integrate, k_integrate) on
     N = 2^24 device-drawn Plummer bodies, with and without the torus wrap: 80 B of HBM
@@ -270,7 +360,7 @@ def run_batch(args) -> dict:
     }
     if not args.no_cpu:
         from oracle import oracle as O
-        threads = max(1, min(CPU_THREADS_MAX, os.cpu_count() or 1))
+        threads = cpu_threads()
         pick = np.random.default_rng(SEED).choice(len(systems), 64, replace=False)
         def timed(variant: str) -> tuple[float, float]:
             O.variant(variant)
@@ -290,7 +380,8 @@ def run_batch(args) -> dict:
                                "kind": "port",
                                "sample": f"one full force pass of 64 of the 512 systems (seeded pick), "
                                          f"{t:.1f} s, -O2",
-                               "o3_march_x86_64_v3": {"value": work / t3 / 1e9, "seconds": t3}}
+                               "o3_march_x86_64_v3": {"value": work / t3 / 1e9, "seconds": t3},
+                               **cpu_facts()}
     return out
 
 
@@ -327,7 +418,21 @@ def run_bh(args) -> dict:
     ctx.step_graph(p, steps)             # ... and replay
     ms_graph, _ = ctx.timer_stop()
     e1 = ctx.diagnostics(p)
+    # the phases of a step, timed by HIP events on the context's stream
+    build_ms, walk_ms = ctx.bh_profile(p, steps)
+    import ctypes
+    cnt = ctypes.c_size_t()
+    ctx._check(ctx.L.nbhip_bh_nodes(ctx.h, ctypes.byref(p), None, 0, ctypes.byref(cnt)))
+    nodes = int(cnt.value)
     ctx.close()
+    # algorithmic bytes of one build per body: keys 16 in + 12 out; radix sort 8 (the
+    # digit counts of all passes in one read) + 8 passes x 24; gather 20 in + 16 out;
+    # moment scan 16 in + 32 out; levels 8 in + ~1.07 out; next-smaller 4 out (325 B so
+    # far); node count and emit ~5 (per-position counts and kept flags) -- 330 B -- plus
+    # the 32 B of every node written
+    build_bytes = 330.0 * n + 32.0 * nodes
+    build_gbs = build_bytes / (build_ms * 1e-3) / 1e9
+    walk_tflops = FLOPS_PER_INTERACTION * inter / (walk_ms * 1e-3) / 1e12
     out = {
         "workload": f"Barnes-Hut N={n} disk+bulge, theta=0.5 (reference criterion), leaf 16, "
                     "eps=0.01, dt=1e-3, 10 steps, tree rebuilt every step",
@@ -335,6 +440,16 @@ def run_bh(args) -> dict:
         "ms_per_step_graph": ms_graph / steps,
         "interactions_per_step": int(inter),
         "value": inter * steps / (ms / 1e3) / 1e9, "unit": "G-interactions/s",
+        "roofline": {
+            "build": {"bound": "hbm", "achieved": build_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": build_gbs / HBM_PEAK_GBS, "ms": build_ms,
+                      "algorithmic_bytes": build_bytes, "nodes": nodes,
+                      "bytes_formula": "330 B per body + 32 B per node"},
+            "walk": {"bound": "valu", "achieved": walk_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": walk_tflops / FP32_PEAK_TFLOPS, "ms": walk_ms,
+                     "flops": f"{FLOPS_PER_INTERACTION} per body-node interaction"},
+            "timing": f"HIP events around each phase of {steps} fused steps (nbhip_bh_profile)",
+        },
         "energy_drift": float(((e1[0] + e1[1]) - (e0[0] + e0[1])) / abs(e0[0] + e0[1])),
         "accuracy": {
             "rel_err_vs_f64_median": float(np.median(rel)),
@@ -354,7 +469,7 @@ def run_bh(args) -> dict:
     }
     if not args.no_cpu:
         from oracle import oracle as O
-        threads = max(1, min(CPU_THREADS_MAX, os.cpu_count() or 1))
+        threads = cpu_threads()
         def timed(variant: str):
             O.variant(variant)
             try:
@@ -377,7 +492,8 @@ def run_bh(args) -> dict:
                                          "(serial build, leaf 1, threaded walk), -O2",
                                "o3_march_x86_64_v3": {"value": 1.0 / (t3_build + t3_walk),
                                                       "build_ms": t3_build * 1e3,
-                                                      "walk_ms": t3_walk * 1e3}}
+                                                      "walk_ms": t3_walk * 1e3},
+                               **cpu_facts()}
     return out
 
 
@@ -398,6 +514,15 @@ def run_served(args) -> dict:
         serve.poisson_arrivals(jobs, rate=rate, seed=SEED)
         rep = srv.run(jobs)
         leaves = {f"2^{c}": leaf for c, leaf in sorted(srv.leaf_by_class.items())}
+    # the same arrivals under one first-come-first-served queue (the earlier policy)
+    with serve.Server(hbm_budget=200e9, admission="fcfs") as srv:
+        fcfs = srv.run(jobs).summary()
+    # the same jobs one at a time through the plain single-system path; the Barnes-Hut jobs
+    # take the leaf sizes the server's autotuner picked
+    for j in jobs:
+        if j.mode == 1:
+            j.params.leaf_size = leaves.get(f"2^{max(0, int(j.n).bit_length() - 1)}", j.params.leaf_size)
+    seq = serve.run_sequential(jobs)
     s = rep.summary()
     n_direct = sum(1 for j in jobs if j.mode == 0)
     return {
@@ -412,13 +537,23 @@ def run_served(args) -> dict:
         "steps_per_s": s["steps_per_s"], "unit": "job-steps/s",
         "jobs_per_s": s["jobs_per_s"],
         "body_steps_per_s": s["body_steps_per_s"],
-        "interactions_per_s": s["interactions_per_s"],
+        "direct_interactions_per_s": s["interactions_per_s"],
         "latency_p50_s": s["latency_p50_s"], "latency_p99_s": s["latency_p99_s"],
         "capacity_jobs_per_s": capacity, "arrival_rate_jobs_per_s": rate,
         "saturated_steps_per_s": sat.job_steps / sat.wall_s,
         "max_concurrent_jobs": s["max_concurrent_jobs"],
         "peak_device_bytes": s["peak_device_bytes"],
         "bh_leaf_by_size_class": leaves,
+        "admission": "per-class queues (direct sum, Barnes-Hut), budget in arrival order",
+        "fcfs_same_arrivals": {"steps_per_s": fcfs["steps_per_s"], "latency_p50_s": fcfs["latency_p50_s"],
+                               "latency_p99_s": fcfs["latency_p99_s"]},
+        "sequential_steps_per_s": seq.job_steps / seq.wall_s,
+        "sequential_wall_s": seq.wall_s,
+        "speedup_vs_sequential": (sat.job_steps / sat.wall_s) / (seq.job_steps / seq.wall_s),
+        "sequential": "the same 160 jobs one at a time, each alone on one context via nbhip_step "
+                      "(one launch per direct-sum step with the single-system launch shape; the "
+                      "Barnes-Hut build and walk per step), all queued at t=0; compared with the "
+                      "server's saturated run of the same jobs",
     }
 
 
@@ -431,7 +566,7 @@ def main() -> int:
     ap.add_argument("--bh-bodies", type=int, default=1 << 20,
                     help="bodies of the Barnes-Hut leg (the C3 workload is 2^20; smaller sizes are "
                          "the served jobs' range)")
-    ap.add_argument("--only", choices=["direct", "batch", "integrate", "bh", "served"], default=None,
+    ap.add_argument("--only", choices=["direct", "c1", "batch", "integrate", "bh", "served"], default=None,
                     help="run one workload (the headline fields then describe it)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -443,6 +578,8 @@ def main() -> int:
         out = run_direct(args)
     else:
         out = {"metric": METRIC, "n_gpus": 1}
+    if args.only in (None, "c1"):
+        out["c1"] = run_c1(args)
     if args.only in (None, "batch"):
         out["batched"] = run_batch(args)
     if args.only in (None, "integrate"):

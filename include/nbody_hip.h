@@ -251,6 +251,13 @@ int nbhip_bh_tune(nbhip_ctx* ctx, const nbhip_params* p, const int* leaf_sizes, 
  * differ between choices by rounding only.  NBHIP_ERR_INVALID for any other value. */
 int nbhip_bh_configure(nbhip_ctx* ctx, int walk_split);
 
+/* No counterpart in the reference (its tree build is a host call that
+ * tests/source/bench_gpu.cpp:86-109 times on its own).  `steps` fused Barnes-Hut steps,
+ * exactly as nbhip_step runs them (the bodies move), with HIP events on the context's
+ * stream around each step's tree build and its traversal; *build_ms and *walk_ms receive
+ * the means.  Synchronises. */
+int nbhip_bh_profile(nbhip_ctx* ctx, const nbhip_params* p, int steps, double* build_ms, double* walk_ms);
+
 /* ---- diagnostics (FP64 reductions on the device) --------------------------------- */
 
 /* out[0] kinetic energy, out[1] potential energy (softened, with the radius cutoff),

@@ -121,6 +121,7 @@ SIGNATURES = {
     "nbhip_tune": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, C.POINTER(_sz), C.POINTER(TuneEntry)]),
     "nbhip_bh_tune": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, _vp, C.POINTER(C.c_int)]),
     "nbhip_bh_configure": (C.c_int, [_vp, C.c_int]),
+    "nbhip_bh_profile": (C.c_int, [_vp, C.POINTER(Params), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "nbhip_diagnostics": (C.c_int, [_vp, C.POINTER(Params), _vp]),
     "nbhip_force_error_f64": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, _vp, _vp]),
     "nbhip_bh_keys": (C.c_int, [_vp, C.POINTER(Params), _vp, _vp]),
@@ -358,6 +359,12 @@ class Context:
         ms = np.zeros(ls.size, np.float32)
         self._check(self.L.nbhip_bh_tune(self.h, C.byref(p), _ptr(ls), ls.size, _ptr(ms), C.byref(best)))
         return [int(v) for v in ls], ms, best.value
+
+    def bh_profile(self, p: Params, steps: int) -> tuple[float, float]:
+        """`steps` fused Barnes-Hut steps; mean (build ms, walk ms) from HIP events."""
+        b, w = C.c_double(), C.c_double()
+        self._check(self.L.nbhip_bh_profile(self.h, C.byref(p), steps, C.byref(b), C.byref(w)))
+        return b.value, w.value
 
     def bh_configure(self, walk_split: int = 0) -> None:
         """Waves per 64-body group in the Barnes-Hut walk: 1, 2, 4, 8, or 0 for by body count."""

@@ -1764,10 +1764,9 @@ int walk_split(const nbhip_ctx* c, int n)
     return 2;
 }
 
-void bh_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse, bool count)
+void bh_walk(nbhip_ctx* c, const nbhip_params& p, bool fuse, bool count)
 {
     const size_t n = c->count;
-    bh_build(c, p);
     ensure_radius_flag(c);
     const bool cut = c->any_radius || p.eps == 0.f;
     if (count) c->bh_visits.grow(n, 0, c->stream);
@@ -1799,6 +1798,45 @@ void bh_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse, bool count)
     hipLaunchKernelGGL(k, dim3(unsigned((n + size_t(bodies) - 1) / size_t(bodies))), dim3(threads), 0, c->stream, a);
     NBHIP_CHECK(hipGetLastError());
     c->launches += 1;
+}
+
+void bh_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse, bool count)
+{
+    bh_build(c, p);
+    bh_walk(c, p, fuse, count);
+}
+
+void bh_profile_steps(nbhip_ctx* c, const nbhip_params& p, int reps, double* build_ms, double* walk_ms)
+{
+    ensure_radius_flag(c);  // its host read stays out of the timed region
+    bh_reserve(c);
+    std::vector<hipEvent_t> ev(3 * size_t(reps), nullptr);
+    try {
+        for (auto& e : ev) NBHIP_CHECK(hipEventCreate(&e));
+        for (int r = 0; r < reps; ++r) {
+            NBHIP_CHECK(hipEventRecord(ev[3 * r], c->stream));
+            bh_build(c, p);
+            NBHIP_CHECK(hipEventRecord(ev[3 * r + 1], c->stream));
+            bh_walk(c, p, true, false);
+            NBHIP_CHECK(hipEventRecord(ev[3 * r + 2], c->stream));
+        }
+        NBHIP_CHECK(hipEventSynchronize(ev.back()));
+        double b = 0, w = 0;
+        for (int r = 0; r < reps; ++r) {
+            float t0 = 0.f, t1 = 0.f;
+            NBHIP_CHECK(hipEventElapsedTime(&t0, ev[3 * r], ev[3 * r + 1]));
+            NBHIP_CHECK(hipEventElapsedTime(&t1, ev[3 * r + 1], ev[3 * r + 2]));
+            b += t0;
+            w += t1;
+        }
+        *build_ms = b / reps;
+        *walk_ms = w / reps;
+    } catch (...) {
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (auto e : ev) NBHIP_CHECK(hipEventDestroy(e));
 }
 
 void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm)

@@ -228,6 +228,9 @@ void tune_direct(nbhip_ctx* c, const nbhip_params& p, std::vector<nbhip_tune_ent
 // bh.hip
 void bh_build(nbhip_ctx* c, const nbhip_params& p);
 void bh_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse_integrate, bool count);
+void bh_walk(nbhip_ctx* c, const nbhip_params& p, bool fuse_integrate, bool count);
+// `reps` fused Barnes-Hut steps with HIP events around the build and the walk of each
+void bh_profile_steps(nbhip_ctx* c, const nbhip_params& p, int reps, double* build_ms, double* walk_ms);
 void bh_release(nbhip_ctx* c);
 void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm);
 void bh_reserve(nbhip_ctx* c);

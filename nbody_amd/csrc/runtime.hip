@@ -685,6 +685,23 @@ int nbhip_bh_interactions(nbhip_ctx* c, const nbhip_params* p, uint32_t* per_bod
     });
 }
 
+int nbhip_bh_profile(nbhip_ctx* c, const nbhip_params* p, int steps, double* build_ms, double* walk_ms)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        ++c->body_version;  // the bodies move
+        check_params(p);
+        if (p->mode != NBHIP_MODE_NLOGN) fail(NBHIP_ERR_INVALID, "Barnes-Hut steps need NBHIP_MODE_NLOGN");
+        if (steps < 1) fail(NBHIP_ERR_INVALID, "at least one step");
+        if (!build_ms || !walk_ms) fail(NBHIP_ERR_INVALID, "null output");
+        *build_ms = *walk_ms = 0.0;
+        if (c->count == 0) return;
+        NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::bh_profile_steps(c, *p, steps, build_ms, walk_ms);
+        nbhip::bh_check_fault(c);
+    });
+}
+
 int nbhip_bh_configure(nbhip_ctx* c, int walk_split)
 {
     return guarded(c, [&] {

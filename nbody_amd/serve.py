@@ -6,8 +6,10 @@ admitted job's state on the device until its last step, and runs:
     step for all of them, with jobs joining and leaving between chunks of steps;
   * Barnes-Hut jobs each on their own context from a fixed pool, so their streams run
     concurrently with the batch and with each other.
-Admission is first come, first served, against a device-memory budget (bytes per body
-estimated per mode) and the free contexts of the pool; a job that does not fit waits.
+Admission keeps one first-come-first-served queue per resource class (direct sum, Barnes-Hut):
+the device-memory budget (bytes per body estimated per mode) is granted in arrival order
+across both, but a class whose own resource is busy (the batch between chunk boundaries,
+no free context) does not hold up the other class; a job that does not fit waits.
 A job brings its bodies either as host records (uploaded at admission) or as a generator
 spec (nbhip_ics), in which case its bodies are drawn on the device and never cross PCIe.
 Each resource advances in chunks of at most `chunk` steps, never past the step count of
@@ -104,8 +106,23 @@ class _BhSlot:
 class Server:
     def __init__(self, hbm_budget: float = 200e9, bh_contexts: int = 8, chunk: int = 16,
                  direct_below: int = 16384, keep_results: bool = False, batch_shape=None,
-                 bh_graph: bool = True, bh_leaf="auto"):
+                 bh_graph: bool = True, bh_leaf="auto", admission: str = "per_class",
+                 count_bh_interactions: bool = False, final_forces: bool = False):
         self.budget = hbm_budget
+        # end each job with one accelerate-only pass, so its result carries the forces at
+        # its final positions (a fused step leaves those of the positions before it); for
+        # checking served results, it costs a force pass per job
+        self.final_forces = final_forces
+        # "per_class": a busy resource class does not block the other's queue; "fcfs": one
+        # queue, the earliest waiting job blocks every later one (the earlier policy, kept
+        # for comparison)
+        if admission not in ("per_class", "fcfs"):
+            raise ValueError(admission)
+        self.admission = admission
+        # count each Barnes-Hut job's body-node interactions with one extra force pass after
+        # its result is back (off the admission path); off, Report.interactions covers the
+        # direct-sum jobs only
+        self.count_bh_interactions = count_bh_interactions
         # Barnes-Hut leaf size: a number for every job, or "auto": the autotuner
         # (nbhip_bh_tune) times the candidates on the first job of each power-of-two size
         # class, on that job's own bodies, and the class keeps its pick for the run.
@@ -184,6 +201,8 @@ class Server:
 
             # -- the batch: account a finished chunk, retire jobs, admit, launch --------
             if batch_inflight and self.batch_ctx.idle():
+                if self.final_forces and any(rem <= batch_inflight for _, rem in direct.values()):
+                    self.batch.step(1, _lib.BATCH_ACCELERATE)   # positions do not move
                 for sid in list(direct):
                     job, rem = direct[sid]
                     rem -= batch_inflight
@@ -199,31 +218,47 @@ class Server:
                         direct[sid] = (job, rem)
                 batch_inflight = 0
 
-            # -- admission, first come first served ---------------------------------------
-            while waiting:
-                job = waiting[0]
-                if in_use + job.device_bytes() > self.budget:
+            # -- admission: per-class queues, the budget granted in arrival order --------
+            # A job whose class resource is busy keeps its place for the budget (the bytes
+            # of such jobs ahead are held back) but does not stop later jobs of the other
+            # class; under "fcfs" it stops every later job.
+            blocked: set[int] = set()
+            reserved = 0
+            i = 0
+            while i < len(waiting):
+                job = waiting[i]
+                if job.mode in blocked:
+                    reserved += job.device_bytes()
+                    i += 1
+                    continue
+                if in_use + reserved + job.device_bytes() > self.budget:
                     break
                 if job.mode == _lib.MODE_N2:
-                    if batch_inflight:
-                        break        # joins at the next chunk boundary
+                    busy = batch_inflight > 0       # joins at the next chunk boundary
+                    slot = None
+                else:
+                    slot = next((s for s in self.slots if s.job is None), None)
+                    busy = slot is None
+                if busy:
+                    if self.admission == "fcfs":
+                        break
+                    blocked.add(job.mode)
+                    reserved += job.device_bytes()
+                    i += 1
+                    continue
+                if job.mode == _lib.MODE_N2:
                     sid = (self.batch.add(job.bodies, job.params) if job.bodies is not None
                            else self.batch.add_generated(job.ics, job.n, job.params))
                     direct[sid] = (job, job.steps)
                     job.interactions_per_step = float(job.n) ** 2
                 else:
-                    slot = next((s for s in self.slots if s.job is None), None)
-                    if slot is None:
-                        break
                     if job.bodies is not None:
                         slot.ctx.upload_bodies(job.bodies)
                     else:
                         slot.ctx.generate(job.ics, job.n)
                     job.params.leaf_size = self._leaf_for(job, slot.ctx)
-                    _, inter = slot.ctx.bh_interactions(job.params)
-                    job.interactions_per_step = float(inter)
                     slot.job, slot.remaining, slot.inflight = job, job.steps, 0
-                waiting.popleft()
+                del waiting[i]
                 job.admitted = now()
                 in_use += job.device_bytes()
                 peak = max(peak, in_use)
@@ -241,12 +276,17 @@ class Server:
                     s.remaining -= s.inflight
                     job_steps += s.inflight
                     body_steps += s.inflight * s.job.n
-                    interactions += s.inflight * s.job.interactions_per_step
                     s.inflight = 0
                     if s.remaining <= 0:
+                        if self.final_forces:
+                            s.ctx.accelerate(s.job.params)
                         res = s.ctx.download_bodies()
                         job, s.job = s.job, None
                         finish(job, res)
+                        if self.count_bh_interactions:
+                            _, inter = s.ctx.bh_interactions(job.params)   # at the final positions
+                            job.interactions_per_step = float(inter)
+                            interactions += job.steps * job.interactions_per_step
                         continue
                 if not s.inflight:
                     k = min(self.chunk, s.remaining)
@@ -266,6 +306,30 @@ class Server:
         wall = now()
         lat = np.array([j.latency for j in finished])
         return Report(len(finished), wall, job_steps, body_steps, interactions, lat, peak, max_conc)
+
+
+def run_sequential(jobs: list[Job], clock=time.perf_counter) -> Report:
+    """The same jobs one at a time, each alone on one context through the plain
+    single-system path (nbhip_step: one force launch per direct-sum step with that path's
+    own launch shape; the Barnes-Hut build and walk per step), in arrival order, every job
+    queued at the start.  The baseline the continuous-batching server is compared with."""
+    t0 = clock()
+    lat = []
+    job_steps = 0
+    body_steps = 0.0
+    with _lib.Context(0) as ctx:
+        for j in sorted(jobs, key=lambda j: j.arrival):
+            if j.bodies is not None:
+                ctx.upload_bodies(j.bodies)
+            else:
+                ctx.generate(j.ics, j.n)
+            ctx.step(j.params, j.steps)
+            ctx.download_bodies()
+            job_steps += j.steps
+            body_steps += j.steps * j.n
+            lat.append(clock() - t0)
+    wall = clock() - t0
+    return Report(len(jobs), wall, job_steps, body_steps, 0.0, np.array(lat), 0, 1)
 
 
 def make_jobs(count: int, seed: int, n_range=(1024, 262144), steps_range=(10, 1000),

@@ -173,3 +173,47 @@ def test_rejects_systems_past_the_body_limit(ctx):
         with pytest.raises(_lib.HipError) as e:
             batch.add_generated(spec, _lib.MAX_BODIES + 1, _lib.params(eps=0.01))
         assert e.value.status == _lib.ERR_INVALID
+
+
+def c4_workload():
+    # the benchmark's 512 systems: N log-uniform in [256, 32768], seeds 2026 + k
+    rng = np.random.default_rng(2026)
+    ns = np.exp(rng.uniform(np.log(256), np.log(32768), 512)).astype(int)
+    return [ics.plummer(int(n), seed=2026 + k) for k, n in enumerate(ns)]
+
+
+def test_c4_workload_50_step_graph(ctx, orc):
+    # The full C4 workload as the benchmark runs it: autotuned, 50 fused steps replayed from
+    # one hipGraph; then one accelerate-only pass, whose forces on sampled bodies of the 8
+    # largest and 8 smallest systems must be within 1e-5 of FP64.  Three systems (one above
+    # 16k bodies) stepped alone in a batch of one with the same launch shape must match
+    # their state in the full batch bit for bit.
+    systems = c4_workload()
+    p = _lib.params(dt=1e-3, eps=0.01, size=10000.0, wrap=1)
+    with _lib.Batch(ctx) as batch:
+        ids = [batch.add(b, p) for b in systems]
+        batch.tune()
+        info = batch.info()
+        assert info.systems == 512
+        batch.step(50, _lib.BATCH_GRAPH)
+        batch.step(1, _lib.BATCH_ACCELERATE)
+        order = np.argsort([b.shape[0] for b in systems])
+        check = list(order[:8]) + list(order[-8:])
+        big = int(next(k for k in order[::-1] if 16384 < systems[k].shape[0]))
+        alone_pick = [int(order[0]), int(order[len(order) // 2]), big]
+        got = {k: batch.download(ids[k]) for k in set(check) | set(alone_pick)}
+    rng = np.random.default_rng(4)
+    for k in check:
+        g = got[k]
+        s = rng.choice(g.shape[0], min(g.shape[0], 128), replace=False).astype(np.uint32)
+        a64 = orc.direct_f64(g, s, eps=0.01)
+        e = rel_err(g[s, 8:11], a64)
+        assert e.max() <= 1e-5, (k, g.shape[0], e.max())
+    for k in alone_pick:
+        with _lib.Batch(ctx) as one:
+            one.configure(info.P, info.W, bool(info.persistent))
+            i = one.add(systems[k], p)
+            one.step(50, _lib.BATCH_GRAPH)
+            one.step(1, _lib.BATCH_ACCELERATE)
+            solo = one.download(i)
+        assert np.array_equal(bits(solo), bits(got[k])), (k, systems[k].shape[0])

@@ -596,3 +596,19 @@ def test_nlogn_refuses_more_than_2p30_minus_1_bodies():
         with pytest.raises(_lib.HipError) as e:
             c.reserve(_lib.MAX_BODIES + 1)
         assert e.value.status == _lib.ERR_INVALID
+
+
+def test_profile_steps_equal_plain_steps(ctx):
+    # nbhip_bh_profile runs the same fused steps as nbhip_step, with events around the phases
+    b = ics.disk_bulge(30000, seed=43)
+    p = bh_params(size=128.0, theta=0.5, eps=0.01, dt=1e-3, wrap=1, leaf_size=16)
+    ctx.upload_bodies(b)
+    ctx.step(p, 3)
+    plain = ctx.download_bodies()
+    ctx.upload_bodies(b)
+    build_ms, walk_ms = ctx.bh_profile(p, 3)
+    prof = ctx.download_bodies()
+    assert np.array_equal(prof.view(np.uint32), plain.view(np.uint32))
+    assert build_ms > 0 and walk_ms > 0
+    with pytest.raises(_lib.HipError):
+        ctx.bh_profile(_lib.params(size=128.0), 1)   # direct-sum params

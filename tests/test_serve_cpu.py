@@ -40,3 +40,107 @@ def test_device_ics_jobs_carry_specs(orc):
         b = orc.generate(j.ics, j.n)   # what the device draws, within the root cube
         assert np.abs(b[:, 0:3]).max() < j.params.size / 2
 
+
+
+class _FakeCtx:
+    """Stands in for a device context: a chunk of k steps stays in flight for k polls."""
+
+    def __init__(self, device=0):
+        self.busy = 0
+        self.n = 0
+
+    def idle(self):
+        if self.busy > 0:
+            self.busy -= 1
+            return False
+        return True
+
+    def generate(self, spec, n):
+        self.n = n
+
+    def upload_bodies(self, b):
+        self.n = b.shape[0]
+
+    def step(self, p, k):
+        self.busy = k
+
+    step_graph = step
+
+    def download_bodies(self):
+        return np.zeros((self.n, 12), np.float32)
+
+    def bh_tune(self, p):
+        return [16], np.array([0.1]), 16
+
+    def close(self):
+        pass
+
+
+class _FakeBatch:
+    def __init__(self, ctx):
+        self.ctx, self.sizes, self.next = ctx, {}, 1
+
+    def add_generated(self, spec, n, p):
+        self.sizes[self.next] = n
+        self.next += 1
+        return self.next - 1
+
+    def add(self, bodies, p):
+        return self.add_generated(None, bodies.shape[0], p)
+
+    def remove(self, sid):
+        del self.sizes[sid]
+
+    def step(self, k, flags=0):
+        self.ctx.busy = k
+
+    def download(self, sid):
+        return np.zeros((self.sizes[sid], 12), np.float32)
+
+    def configure(self, *a):
+        pass
+
+    def close(self):
+        pass
+
+
+def _admission_order(monkeypatch, policy):
+    monkeypatch.setattr(_lib, "Context", _FakeCtx)
+    monkeypatch.setattr(_lib, "Batch", _FakeBatch)
+    jobs = serve.make_jobs(3, seed=1, n_range=(1024, 1025), steps_range=(64, 64), device_ics=True)
+    d1, d2, b1 = jobs
+    b1.params = _lib.params(mode=_lib.MODE_NLOGN, size=128.0, dt=1e-3, eps=0.01)
+    b1.n_bodies = 20000
+    d1.arrival, d2.arrival, b1.arrival = 0.0, 0.05, 0.06   # d2 and b1 arrive while d1's chunk runs
+    ticks = iter(range(10 ** 7))
+    with serve.Server(chunk=64, bh_contexts=1, admission=policy) as srv:
+        rep = srv.run(jobs, clock=lambda: next(ticks) * 1e-3)
+    assert rep.jobs == 3
+    return d1, d2, b1
+
+
+def test_per_class_admission_does_not_block_across_classes(monkeypatch):
+    # the batch is busy with d1 when d2 (direct) and b1 (Barnes-Hut) arrive: per class,
+    # b1 takes the free context at once while d2 waits for the chunk boundary; under one
+    # FCFS queue d2 at the head holds b1 back until then
+    d1, d2, b1 = _admission_order(monkeypatch, "per_class")
+    assert b1.admitted < d2.admitted
+    d1, d2, b1 = _admission_order(monkeypatch, "fcfs")
+    assert b1.admitted >= d2.admitted
+
+
+def test_budget_held_in_arrival_order(monkeypatch):
+    # a direct job blocked by the busy batch keeps its place for the budget: a later
+    # Barnes-Hut job that would only fit by taking that place waits
+    monkeypatch.setattr(_lib, "Context", _FakeCtx)
+    monkeypatch.setattr(_lib, "Batch", _FakeBatch)
+    jobs = serve.make_jobs(3, seed=1, n_range=(1024, 1025), steps_range=(64, 64), device_ics=True)
+    d1, d2, b1 = jobs
+    b1.params = _lib.params(mode=_lib.MODE_NLOGN, size=128.0, dt=1e-3, eps=0.01)
+    b1.n_bodies = 2048
+    d1.arrival, d2.arrival, b1.arrival = 0.0, 0.05, 0.06
+    budget = d1.device_bytes() + d2.device_bytes() + b1.device_bytes() - 1
+    ticks = iter(range(10 ** 7))
+    with serve.Server(hbm_budget=budget, chunk=64, bh_contexts=1) as srv:
+        srv.run(jobs, clock=lambda: next(ticks) * 1e-3)
+    assert b1.admitted > d2.admitted

@@ -84,3 +84,46 @@ def test_rejects_impossible_jobs():
     with serve.Server(hbm_budget=1000, chunk=8, bh_contexts=1) as srv:
         with pytest.raises(ValueError):
             srv.run(jobs)
+
+
+def test_served_mix_at_workload_scale_matches_oracles(orc):
+    # A served mix that reaches the workload's extremes (a Barnes-Hut job of 230,000 bodies
+    # over 600 steps, direct jobs up to 1,000 steps), each job ending with a force pass at
+    # its final positions.  Direct jobs: every job's forces on 256 sampled bodies against
+    # the FP64 direct sum (1e-5).  Barnes-Hut jobs: the served forces equal a standalone
+    # force pass on the same final bodies bit for bit, and that pass matches the bucket
+    # oracle at the job's leaf size (>= 99.99% identical traversals, 1e-5 at 99.9%, 1e-4).
+    jobs = serve.make_jobs(20, seed=17, device_ics=True)
+    big = next(j for j in jobs if j.mode == _lib.MODE_NLOGN)
+    big.n_bodies, big.steps = 230000, 600
+    longest = next(j for j in jobs if j.mode == _lib.MODE_N2)
+    longest.steps = 1000
+    assert max(j.n for j in jobs) >= 200000 and max(j.steps for j in jobs) >= 500
+    serve.poisson_arrivals(jobs, rate=40.0, seed=17)
+    with serve.Server(chunk=16, bh_contexts=4, keep_results=True, final_forces=True) as srv:
+        rep = srv.run(jobs)
+    assert rep.jobs == len(jobs)
+    rng = np.random.default_rng(17)
+    with _lib.Context(0) as c:
+        for j in jobs:
+            res = j.result
+            eps = float(j.params.eps)
+            if j.mode == _lib.MODE_N2:
+                s = rng.choice(j.n, min(j.n, 256), replace=False).astype(np.uint32)
+                a64 = orc.direct_f64(res, s, G=float(j.params.G), eps=eps)
+                e = np.linalg.norm(res[s, 8:11] - a64, axis=1) / np.linalg.norm(a64, axis=1)
+                assert e.max() <= 1e-5, (j.id, j.n, e.max())
+                continue
+            c.upload_bodies(res)
+            visits, _ = c.bh_interactions(j.params)
+            fresh = c.download_bodies()
+            assert np.array_equal(fresh[:, 8:11].view(np.uint32), res[:, 8:11].view(np.uint32)), j.id
+            x = res.copy()
+            _, xv = orc.bh_accelerate_buckets(x, float(j.params.size), theta=float(j.params.theta),
+                                              leaf_size=int(j.params.leaf_size), eps=eps)
+            eq = visits == xv
+            e = np.linalg.norm(fresh[:, 8:11] - x[:, 8:11], axis=1) / np.linalg.norm(x[:, 8:11], axis=1)
+            print(f"job {j.id}: n={j.n} steps={j.steps} leaf={j.params.leaf_size} same traversal "
+                  f"{eq.mean():.5f} 99.9% {np.quantile(e[eq], 0.999):.2e} max {e[eq].max():.2e}")
+            assert eq.mean() >= 0.9999
+            assert np.quantile(e[eq], 0.999) <= 1e-5 and e[eq].max() <= 1e-4
