@@ -220,7 +220,8 @@ typedef struct nbhip_tune_entry {
     float ms;
 } nbhip_tune_entry;
 
-/* Time every compiled (P, W) shape at j-splits 1, 2, 4 and 8 on the current bodies
+/* Time every compiled (P, W) shape at j-splits 1, 2, 4 and 8 (and 16, 32 where the
+ * i-blocks of the shape times the split stay within 16384 blocks) on the current bodies
  * (accelerations only: positions and velocities do not move), and use the fastest for
  * this body count from now on.  Up to `cap` timings are written to `out` (may be NULL),
  * *count receives how many were taken, and *best (may be NULL) the chosen one.

@@ -20,6 +20,8 @@ namespace {
 
 enum : int { OUT_PARTIAL = 0, OUT_ACC = 1, OUT_STEP = 2 };
 
+constexpr int MAX_JSPLIT = 32;  // j-chunks per i-block the reduction takes
+
 struct ForceArgs {
     const float4* pm;  // current {x, y, z, m}
     const float4* vr;  // {vx, vy, vz, radius}
@@ -73,19 +75,30 @@ __global__ __launch_bounds__(WAVE* W) void k_force(ForceArgs a)
                           });
 }
 
-// Combine j-split partials in a fixed order, then finish as OUT_ACC / OUT_STEP.
+// Combine j-split partials in a fixed order, then finish as OUT_ACC / OUT_STEP.  MAXJ >=
+// jsplit: every partial load is issued before the first add (at small N the pass is one
+// memory round trip, not jsplit of them); a missing chunk adds an exact zero.
+template <int MAXJ>
 __global__ __launch_bounds__(256) void k_reduce(ForceArgs a, int jsplit)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= a.n) return;
     const size_t n = a.n;
+    float v[MAXJ][3];
+#pragma unroll
+    for (int y = 0; y < MAXJ; ++y)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[y][k] = y < jsplit ? a.partial[(size_t(y) * 3 + k) * n + i] : 0.f;
+    const float4 xi = a.pm[i];
     float tot[3];
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
         float s = 0.f, c = 0.f;
-        for (int y = 0; y < jsplit; ++y) two_sum(s, c, a.partial[(size_t(y) * 3 + k) * n + i]);
+#pragma unroll
+        for (int y = 0; y < MAXJ; ++y) two_sum(s, c, v[y][k]);
         tot[k] = s + c;
     }
-    finish_body(a, i, a.pm[i], a.G * tot[0], a.G * tot[1], a.G * tot[2]);
+    finish_body(a, i, xi, a.G * tot[0], a.G * tot[1], a.G * tot[2]);
 }
 
 // IPT bodies per thread, 256 apart, every load issued before the arithmetic; NT streams
@@ -338,16 +351,16 @@ DirectConfig choose_direct_config(size_t n)
     const size_t iblocks = (n + per_block - 1) / per_block;
     const int forced = env_int("NBHIP_DIRECT_JSPLIT", 0);
     if (forced > 0) {
-        c.jsplit = forced;
+        c.jsplit = std::min(forced, MAX_JSPLIT);
     } else {
         // Aim for >= 8192 blocks (32 per CU over 256 CUs) so the last wave of blocks is a
         // small tail and small systems still fill the chip; never split below one tile
-        // of j per block.
+        // of j per block, nor into more chunks than the reduction takes.
         const size_t target = 8192;
         size_t js = iblocks >= target ? 1 : (target + iblocks - 1) / iblocks;
         const size_t tile = size_t(WAVE) * c.W;
         const size_t max_js = std::max<size_t>(1, (n + tile - 1) / tile);
-        c.jsplit = int(std::min(js, max_js));
+        c.jsplit = int(std::min({ js, max_js, size_t(MAX_JSPLIT) }));
     }
     return c;
 }
@@ -376,6 +389,8 @@ namespace {
 // count actually used.
 int split_chunks(size_t n, const DirectConfig& cfg, size_t& jchunk)
 {
+    if (cfg.jsplit < 1 || cfg.jsplit > MAX_JSPLIT)
+        fail(NBHIP_ERR_INVALID, "j-split %d outside 1..%d", cfg.jsplit, MAX_JSPLIT);
     const size_t tile = size_t(WAVE) * cfg.W;
     jchunk = (n + cfg.jsplit - 1) / cfg.jsplit;
     jchunk = (jchunk + tile - 1) / tile * tile;
@@ -414,7 +429,10 @@ void launch_force(hipStream_t stream, ForceArgs a, int jsplit, const DirectConfi
 
 void launch_reduce(hipStream_t stream, const ForceArgs& a, int jsplit)
 {
-    hipLaunchKernelGGL(k_reduce, dim3(grid1(size_t(a.n))), dim3(256), 0, stream, a, jsplit);
+    using R = void (*)(ForceArgs, int);
+    const R k = jsplit <= 2 ? (R)k_reduce<2> : jsplit <= 4 ? (R)k_reduce<4> : jsplit <= 8 ? (R)k_reduce<8>
+              : jsplit <= 16 ? (R)k_reduce<16> : (R)k_reduce<32>;
+    hipLaunchKernelGGL(k, dim3(grid1(size_t(a.n))), dim3(256), 0, stream, a, jsplit);
     NBHIP_CHECK(hipGetLastError());
 }
 
@@ -575,12 +593,15 @@ void tune_direct(nbhip_ctx* c, const nbhip_params& p, std::vector<nbhip_tune_ent
     const size_t n = c->count;
     best = nbhip_tune_entry{ 0, 0, 0, 1e30f };
     for (const auto& sh : shapes)
-        for (int js : { 1, 2, 4, 8 }) {
+        for (int js : { 1, 2, 4, 8, 16, 32 }) {
             DirectConfig cfg;
             cfg.P = sh[0];
             cfg.W = sh[1];
             const size_t tile = size_t(WAVE) * cfg.W;
             if (js > 1 && size_t(js) * tile > n) continue;  // a j-chunk must hold a tile
+            // the finer splits only where the i-blocks alone leave the GPU underfilled
+            const size_t iblocks = (n + size_t(WAVE) * cfg.P - 1) / (size_t(WAVE) * cfg.P);
+            if (js > 8 && iblocks * size_t(js) > 16384) continue;
             cfg.jsplit = js;
             direct_pass(c->stream, v, p, false, c->partial, cfg);  // warm
             NBHIP_CHECK(hipEventRecord(c->ev_start, c->stream));

@@ -61,8 +61,9 @@ namespace nbody
         [[nodiscard]] std::span<const bh::Node> nodes() const;
 
     private:
-        void sync_solver() const;
-        void apply_extensions();
+        StateRef hand_over();          // the running solver's State, brought up to date
+        void refresh() const;          // ingest the host copy once after a mutation
+        void push_extensions();        // settings the reference State has no field for
 
         std::shared_ptr<Context> _context;
         StateRef _state;
@@ -70,6 +71,6 @@ namespace nbody
         Variant _variant = Variant::HipBarnesHut;
         std::string _last_error;
         float _softening = 0.f;
-        mutable uint64_t _synced_revision = 0;
+        mutable uint64_t _ingested_revision = 0;
     };
 }

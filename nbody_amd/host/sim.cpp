@@ -1,9 +1,23 @@
-// Standalone mirror of the reference facade (source/sim.cpp:17-335) hosting the two HIP
-// variants.  Inside the reference the HIP entries are added to its own infos() and
-// factories() tables instead; nothing here is needed there.
-#include <array>
-#include <cassert>
+// sim.cpp -- a small host for the two HIP variants behind the reference's Sim interface
+// (include/nbody/sim.h:23-117), for this repository's stand-alone build and its
+// conformance tests.  Inside the reference, its own facade registers the variants instead
+// (the patch in the integration notes); nothing here is needed there.
+//
+// Three pieces:
+//   * VariantTable -- one row per Variant value.  The HIP rows carry a solver factory and
+//     are probed once, on first use; the reference's own rows are listed as provided by
+//     the reference library.  A HIP row whose solver fails to come up is demoted, with the
+//     failure as its reason, so later queries report it unavailable.
+//   * the hand-over -- a variant switch is one transaction: the running solver is brought
+//     up to date and hands over the State it holds, the new solver is built on that State
+//     and adopts it, and only then is it installed; any failure leaves the running solver
+//     as it was.  The Solver contract (source/solver.h:41-44) says a solver keeps the very
+//     State object it adopts; that is checked in every build.
+//   * revision-driven ingest -- host code that mutates the bodies bumps State::revision;
+//     before the solver next reads or steps, it is told once to ingest the host copy.
+#include <mutex>
 #include <stdexcept>
+#include <vector>
 
 #include "nbody/sim.h"
 #include "context.h"
@@ -11,252 +25,251 @@
 #include "hip_solver.h"
 #include "solver.h"
 
-using nbody::Sim;
-using nbody::Variant;
-using nbody::VariantInfo;
-
-namespace
+namespace nbody
 {
-    using Factory = std::unique_ptr<nbody::Solver> (*)(std::shared_ptr<nbody::Context>, nbody::StateRef);
-    constexpr size_t variant_count = static_cast<size_t>(Variant::Count);
-
-    template <nbody::HipMode mode>
-    std::unique_ptr<nbody::Solver> make_hip(std::shared_ptr<nbody::Context> context, nbody::StateRef state)
+    namespace
     {
-        return std::make_unique<nbody::HipSolver>(std::move(context), std::move(state), mode);
-    }
+        using SolverFactory = std::unique_ptr<Solver> (*)(std::shared_ptr<Context>, StateRef);
 
-    std::array<VariantInfo, variant_count>& infos()
-    {
-        static std::array<VariantInfo, variant_count> table = []
+        template <HipMode M>
+        std::unique_ptr<Solver> build_hip(std::shared_ptr<Context> context, StateRef state)
         {
-            std::array<VariantInfo, variant_count> t{};
-            const char* elsewhere = "provided by the reference library, not this standalone build";
-            t[size_t(Variant::CpuBarnesHut)] = { Variant::CpuBarnesHut, "CPU Barnes-Hut", "", false, elsewhere };
-            t[size_t(Variant::CpuBruteForce)] = { Variant::CpuBruteForce, "CPU brute force", "", false, elsewhere };
-            t[size_t(Variant::GpuBarnesHut)] = { Variant::GpuBarnesHut, "GPU Barnes-Hut", "", false, elsewhere };
-            t[size_t(Variant::GpuBruteForce)] = { Variant::GpuBruteForce, "GPU brute force", "", false, elsewhere };
-            t[size_t(Variant::GpuBarnesHutSoA)] = { Variant::GpuBarnesHutSoA, "GPU Barnes-Hut (SoA)", "", false, elsewhere };
-            t[size_t(Variant::GpuBruteForceSoA)] = { Variant::GpuBruteForceSoA, "GPU brute force (SoA)", "", false, elsewhere };
-            t[size_t(Variant::HipBarnesHut)] = {
-                Variant::HipBarnesHut, "HIP Barnes-Hut", "MI355X, GPU-built octree, O(n log n)", false, "not probed" };
-            t[size_t(Variant::HipBruteForce)] = {
-                Variant::HipBruteForce, "HIP brute force", "MI355X, exact O(n^2) summation", false, "not probed" };
-            return t;
-        }();
-        return table;
-    }
-
-    std::array<Factory, variant_count>& factories()
-    {
-        static std::array<Factory, variant_count> table = []
-        {
-            std::array<Factory, variant_count> t{};
-            t[size_t(Variant::HipBarnesHut)] = &make_hip<nbody::HipMode::NLogN>;
-            t[size_t(Variant::HipBruteForce)] = &make_hip<nbody::HipMode::N2>;
-            return t;
-        }();
-        return table;
-    }
-
-    bool valid(const Variant v) { return static_cast<size_t>(v) < variant_count; }
-    bool is_hip(const Variant v) { return v == Variant::HipBarnesHut || v == Variant::HipBruteForce; }
-
-    // Probe once, on the first availability query (reference probe_gpu_once,
-    // source/sim.cpp:95-110).  The table is not synchronized: query and switch variants
-    // from one thread, as with the reference.
-    void probe_hip_once()
-    {
-        static const bool probed = []
-        {
-            const std::string reason = nbody::HipDevice::probe();
-            for (const Variant v : { Variant::HipBarnesHut, Variant::HipBruteForce })
-            {
-                infos()[size_t(v)].available = reason.empty();
-                infos()[size_t(v)].unavailable_reason = reason;
-            }
-            return true;
-        }();
-        (void)probed;
-    }
-
-    // A HIP variant that advertised itself but failed to come up is demoted with the
-    // reason (reference record_unavailable, source/sim.cpp:120-126).
-    void record_unavailable(const Variant v, const std::string& reason)
-    {
-        if (!is_hip(v))
-            return;
-        infos()[size_t(v)].available = false;
-        infos()[size_t(v)].unavailable_reason = reason;
-    }
-}
-
-std::span<const VariantInfo> Sim::variants()
-{
-    probe_hip_once();
-    return infos();
-}
-
-const VariantInfo& Sim::info(const Variant v)
-{
-    probe_hip_once();
-    if (!valid(v))
-    {
-        static const VariantInfo invalid{ Variant::Count, "invalid", "", false, "not a valid variant" };
-        return invalid;
-    }
-    return infos()[size_t(v)];
-}
-
-bool Sim::available(const Variant v) { return info(v).available; }
-
-Sim::Sim() : Sim(Variant::HipBarnesHut) {}
-
-Sim::Sim(const Variant variant)
-    : _context(std::make_shared<Context>())
-    , _state(std::make_shared<State>())
-{
-    if (set_variant(variant))
-        return;
-    const std::string requested_error = _last_error;
-    for (const Variant fallback : { Variant::HipBarnesHut, Variant::HipBruteForce })
-        if (fallback != variant && set_variant(fallback))
-        {
-            _last_error = requested_error;
-            return;
+            return std::make_unique<HipSolver>(std::move(context), std::move(state), M);
         }
-    throw std::runtime_error("no simulation variant could be constructed: " + requested_error);
-}
 
-Sim::~Sim() = default;
-Sim::Sim(Sim&&) noexcept = default;
-Sim& Sim::operator=(Sim&&) noexcept = default;
+        class VariantTable
+        {
+        public:
+            static VariantTable& instance()
+            {
+                static VariantTable table;
+                return table;
+            }
 
-bool Sim::set_variant(const Variant v)
-{
-    if (!valid(v))
-    {
-        _last_error = "unknown variant";
-        return false;
+            std::span<const VariantInfo> all()
+            {
+                probe();
+                return { _info.data(), _info.size() };
+            }
+
+            // nullptr for a value outside the enum
+            const VariantInfo* find(const Variant v)
+            {
+                probe();
+                const auto k = static_cast<size_t>(v);
+                return k < _info.size() ? &_info[k] : nullptr;
+            }
+
+            SolverFactory factory(const Variant v) const { return _factory[static_cast<size_t>(v)]; }
+
+            void demote(const Variant v, const std::string& why)
+            {
+                const auto k = static_cast<size_t>(v);
+                if (k >= _info.size() || !_factory[k]) return;  // only HIP rows are ours to change
+                _info[k].available = false;
+                _info[k].unavailable_reason = why;
+            }
+
+        private:
+            VariantTable()
+            {
+                const char* theirs = "provided by the reference library, not this standalone build";
+                const auto add = [this](Variant v, const char* name, const char* what, const char* why, SolverFactory f)
+                {
+                    _info.push_back(VariantInfo{ v, name, what, false, why });
+                    _factory.push_back(f);
+                };
+                add(Variant::CpuBarnesHut, "CPU Barnes-Hut", "", theirs, nullptr);
+                add(Variant::CpuBruteForce, "CPU brute force", "", theirs, nullptr);
+                add(Variant::GpuBarnesHut, "GPU Barnes-Hut", "", theirs, nullptr);
+                add(Variant::GpuBruteForce, "GPU brute force", "", theirs, nullptr);
+                add(Variant::GpuBarnesHutSoA, "GPU Barnes-Hut (SoA)", "", theirs, nullptr);
+                add(Variant::GpuBruteForceSoA, "GPU brute force (SoA)", "", theirs, nullptr);
+                add(Variant::HipBarnesHut, "HIP Barnes-Hut", "MI355X, GPU-built octree, O(n log n)", "not probed",
+                    &build_hip<HipMode::NLogN>);
+                add(Variant::HipBruteForce, "HIP brute force", "MI355X, exact O(n^2) summation", "not probed",
+                    &build_hip<HipMode::N2>);
+                if (_info.size() != static_cast<size_t>(Variant::Count))
+                    throw std::logic_error("the variant table does not cover every Variant value");
+            }
+
+            // The device is asked once whether it can run; every HIP row takes the answer.
+            // Like the rest of the facade, queries and switches come from one thread.
+            void probe()
+            {
+                std::call_once(_probed, [this]
+                {
+                    const std::string reason = HipDevice::probe();
+                    for (size_t k = 0; k < _info.size(); ++k)
+                        if (_factory[k])
+                        {
+                            _info[k].available = reason.empty();
+                            _info[k].unavailable_reason = reason;
+                        }
+                });
+            }
+
+            std::vector<VariantInfo> _info;
+            std::vector<SolverFactory> _factory;
+            std::once_flag _probed;
+        };
+
+        // what Sim(v) tries, in order: the request, then the other HIP variants
+        std::vector<Variant> construction_order(const Variant requested)
+        {
+            std::vector<Variant> order{ requested };
+            for (const Variant v : { Variant::HipBarnesHut, Variant::HipBruteForce })
+                if (v != requested) order.push_back(v);
+            return order;
+        }
     }
-    if (v == _variant && _solver)
+
+    std::span<const VariantInfo> Sim::variants() { return VariantTable::instance().all(); }
+
+    const VariantInfo& Sim::info(const Variant v)
+    {
+        static const VariantInfo no_such_variant{ Variant::Count, "invalid", "", false, "not a valid variant" };
+        const VariantInfo* row = VariantTable::instance().find(v);
+        return row ? *row : no_such_variant;
+    }
+
+    bool Sim::available(const Variant v) { return info(v).available; }
+
+    Sim::Sim() : Sim(Variant::HipBarnesHut) {}
+
+    Sim::Sim(const Variant variant) : _context(std::make_shared<Context>()), _state(std::make_shared<State>())
+    {
+        std::string why_not_requested;
+        for (const Variant v : construction_order(variant))
+        {
+            if (set_variant(v))
+            {
+                if (v != variant) _last_error = why_not_requested;  // say why the request fell back
+                return;
+            }
+            if (v == variant) why_not_requested = _last_error;
+        }
+        throw std::runtime_error("no simulation variant could be constructed: " + why_not_requested);
+    }
+
+    Sim::~Sim() = default;
+    Sim::Sim(Sim&&) noexcept = default;
+    Sim& Sim::operator=(Sim&&) noexcept = default;
+
+    bool Sim::set_variant(const Variant v)
+    {
+        const auto refuse = [this](std::string why)
+        {
+            _last_error = std::move(why);
+            return false;
+        };
+        VariantTable& table = VariantTable::instance();
+        const VariantInfo* row = table.find(v);
+        if (!row) return refuse("unknown variant");
+        if (_solver && v == _variant) return true;
+        if (!row->available) return refuse(row->unavailable_reason);
+        const SolverFactory make = table.factory(v);
+        if (!make) return refuse("variant has no implementation registered");
+
+        StateRef handed = hand_over();
+        std::unique_ptr<Solver> incoming;
+        try
+        {
+            incoming = make(_context, handed);
+            incoming->adopt(handed);
+        }
+        catch (const std::exception& e)
+        {
+            table.demote(v, e.what());
+            return refuse(e.what());
+        }
+        if (incoming->state() != handed)
+            throw std::logic_error("Solver::adopt must keep the State object it is given");
+
+        _solver = std::move(incoming);
+        _state = std::move(handed);
+        _variant = v;
+        _ingested_revision = _state->revision;
+        _last_error.clear();
+        push_extensions();
         return true;
-
-    const VariantInfo& want = info(v);
-    if (!want.available)
-    {
-        _last_error = want.unavailable_reason;
-        return false;
-    }
-    Factory factory = factories()[size_t(v)];
-    if (!factory)
-    {
-        _last_error = "variant has no implementation registered";
-        return false;
     }
 
-    // 1. pull the canonical state out of the outgoing solver, syncing first so a pending
-    //    mutation is not materialized over
-    if (_solver)
-        sync_solver();
-    StateRef state = _solver ? _solver->state() : _state;
-
-    // 2. build the incoming solver; on failure the current one keeps running
-    std::unique_ptr<Solver> next;
-    try
+    // The State the next solver starts from: the running solver's, once it is current.
+    StateRef Sim::hand_over()
     {
-        next = factory(_context, state);
-        next->adopt(state);
-    }
-    catch (const std::exception& e)
-    {
-        _last_error = e.what();
-        record_unavailable(v, _last_error);
-        return false;
+        if (!_solver) return _state;
+        refresh();
+        return _solver->state();
     }
 
-    // 3. commit
-    _solver = std::move(next);
-    _state = std::move(state);
-    _variant = v;
-    _synced_revision = _state->revision;
-    _last_error.clear();
-    apply_extensions();
-    assert(_solver->state() == _state && "solver substituted its own State in adopt()");
-    return true;
-}
+    void Sim::refresh() const
+    {
+        if (_state->revision == _ingested_revision) return;
+        _solver->ingest();
+        _ingested_revision = _state->revision;
+    }
 
-void Sim::apply_extensions()
-{
-    if (auto* hip = dynamic_cast<HipSolver*>(_solver.get()))
-        hip->set_softening(_softening);
-}
+    void Sim::push_extensions()
+    {
+        if (auto* hip = dynamic_cast<HipSolver*>(_solver.get())) hip->set_softening(_softening);
+    }
 
-void Sim::set_softening(const float eps)
-{
-    _softening = eps;
-    apply_extensions();
-}
+    void Sim::set_softening(const float eps)
+    {
+        _softening = eps;
+        push_extensions();
+    }
 
-void Sim::sync_solver() const
-{
-    if (_synced_revision == _state->revision)
-        return;
-    _solver->ingest();
-    _synced_revision = _state->revision;
-}
+    std::shared_ptr<const State> Sim::state() const
+    {
+        refresh();
+        return _solver->state();
+    }
 
-std::shared_ptr<const nbody::State> Sim::state() const
-{
-    sync_solver();
-    return _solver->state();
-}
+    const std::vector<Body>& Sim::bodies() const { return state()->bodies; }
 
-const std::vector<nbody::Body>& Sim::bodies() const
-{
-    sync_solver();
-    return _solver->state()->bodies;
-}
+    std::vector<Body>& Sim::mutable_bodies()
+    {
+        refresh();
+        const StateRef current = _solver->state();
+        current->touch();  // the host copy is now the one to ingest
+        return current->bodies;
+    }
 
-std::vector<nbody::Body>& Sim::mutable_bodies()
-{
-    sync_solver();
-    const StateRef s = _solver->state();
-    s->touch();
-    return s->bodies;
-}
+    float Sim::size() const { return _state->size; }
+    void Sim::set_size(const float v) { _state->size = v; }
+    float Sim::theta() const { return _state->theta; }
+    void Sim::set_theta(const float v) { _state->theta = v; }
+    float Sim::gravity() const { return _state->gravity; }
+    void Sim::set_gravity(const float v) { _state->gravity = v; }
+    bool Sim::wrap() const { return _state->wrap; }
+    void Sim::set_wrap(const bool v) { _state->wrap = v; }
 
-float Sim::size() const { return _state->size; }
-void Sim::set_size(const float v) { _state->size = v; }
-float Sim::theta() const { return _state->theta; }
-void Sim::set_theta(const float v) { _state->theta = v; }
-float Sim::gravity() const { return _state->gravity; }
-void Sim::set_gravity(const float v) { _state->gravity = v; }
-bool Sim::wrap() const { return _state->wrap; }
-void Sim::set_wrap(const bool v) { _state->wrap = v; }
+    void Sim::update(const float dt)
+    {
+        refresh();
+        _solver->update(dt);
+    }
 
-void Sim::update(const float dt)
-{
-    sync_solver();
-    _solver->update(dt);
-}
+    void Sim::accelerate()
+    {
+        refresh();
+        _solver->accelerate();
+    }
 
-void Sim::accelerate()
-{
-    sync_solver();
-    _solver->accelerate();
-}
+    void Sim::integrate(const float dt)
+    {
+        refresh();
+        _solver->integrate(dt);
+    }
 
-void Sim::integrate(const float dt)
-{
-    sync_solver();
-    _solver->integrate(dt);
-}
+    const bh::Tree* Sim::tree() const { return _solver->tree(); }
 
-const nbody::bh::Tree* Sim::tree() const { return _solver->tree(); }
-
-std::span<const nbody::bh::Node> Sim::nodes() const
-{
-    const bh::Tree* t = tree();
-    return t ? std::span<const bh::Node>(t->nodes()) : std::span<const bh::Node>{};
+    std::span<const bh::Node> Sim::nodes() const
+    {
+        const bh::Tree* t = tree();
+        if (!t) return {};
+        return t->nodes();
+    }
 }
