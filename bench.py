@@ -186,7 +186,8 @@ def run_direct(args) -> dict:
             "workload": "direct-sum two-Plummer collision, N=262144, eps=0.005, dt=5e-4, "
                         "fused force+kick-drift step, FP32 with TwoSum-compensated force sum",
             "n_bodies": n, "eps": EPS_C2, "dt": DT_C2, "seed": SEED,
-            "launch": {"P": tuned[0], "W": tuned[1], "jsplit": tuned[2], "autotuned": True},
+            "launch": {"P": tuned[0], "W": tuned[1], "jsplit": tuned[2], "pair_symmetric": tuned[2] == 0,
+                       "autotuned": True},
         },
         "roofline": {
             "bound": "valu",
@@ -194,8 +195,11 @@ def run_direct(args) -> dict:
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP32_PEAK_TFLOPS,
-            "traffic": load_traffic("traffic_direct_n262144.json"),
-            "kernel": "k_force (fused step)",
+            "traffic": load_traffic("traffic_direct_sym_n262144.json" if tuned[2] == 0
+                                    else "traffic_direct_n262144.json"),
+            "kernel": ("k_force_sym + k_reduce_slots (one fused step: each pair once, then each "
+                       "body's per-block partial sums added and the kick-drift)" if tuned[2] == 0
+                       else "k_force (fused step)"),
             "kernel_ms": per_launch_ms,
             "launches": launches,
         },
@@ -249,7 +253,8 @@ def run_c1(args) -> dict:
         "ms_per_step": ms / steps, "steps_per_s": steps / (ms / 1e3),
         "graph": {"value": inter / (ms_graph / 1e3) / 1e9, "ms_per_step": ms_graph / steps},
         "pct_fp32_peak": 100.0 * FLOPS_PER_INTERACTION * inter / (ms / 1e3) / (FP32_PEAK_TFLOPS * 1e12),
-        "launch": {"P": tuned[0], "W": tuned[1], "jsplit": tuned[2], "autotuned": True},
+        "launch": {"P": tuned[0], "W": tuned[1], "jsplit": tuned[2], "pair_symmetric": tuned[2] == 0,
+                   "autotuned": True},
         "bound": "launch/occupancy: 4096 bodies give 32 i-blocks of 128",
     }
     if not args.no_cpu:

@@ -144,7 +144,10 @@ int nbhip_download_bodies(nbhip_ctx* ctx, void* bodies, size_t n);
 /* ---- stepping (reference GpuDevice::accelerate / integrate / step,
  *      source/gpu.h:191-197; Solver::accelerate/integrate/update, source/solver.h:57-59) */
 
-/* Accelerations of every body from the current positions (mode N2 or NLogN). */
+/* Accelerations of every body from the current positions (mode N2 or NLogN).  In N2 mode
+ * a context evaluates each pair once (the pair-symmetric pass, forces applied both ways)
+ * from 65,536 bodies unless tuned otherwise, else in tiles that sum each body's pairs on
+ * its own; the two agree within 2e-6 relative. */
 int nbhip_accelerate(nbhip_ctx* ctx, const nbhip_params* p);
 
 /* No counterpart in the reference: for callers that cannot allocate device memory (code
@@ -155,8 +158,10 @@ int nbhip_accelerate(nbhip_ctx* ctx, const nbhip_params* p);
  * leaves the context's own bodies alone.  Allocates nothing: its scratch (the j-split
  * partial sums and one flag) is a static 32 MiB device array that each GPU holds until
  * the process ends; every call on a GPU shares it, so calls that overlap on one GPU (from
- * any context or thread) race.  Results equal nbhip_accelerate's on a context holding
- * the same bodies with the default launch shape (no nbhip_tune).  NBHIP_ERR_UNSUPPORTED
+ * any context or thread) race.  Results equal, bit for bit, those of nbhip_accelerate's
+ * tiled pass with its default launch shape on a context holding the same bodies; where
+ * nbhip_accelerate picks its pair-symmetric pass instead (by default from 65,536 bodies)
+ * the two agree within 2e-6 relative.  NBHIP_ERR_UNSUPPORTED
  * in NLogN mode; NBHIP_ERR_INVALID for n past NBHIP_MAX_BODIES, arrays that are null,
  * misaligned, not device memory of the context's device or shorter than n bodies, or a
  * launch shape whose partial sums exceed the scratch (the default shapes need at most
@@ -221,7 +226,9 @@ typedef struct nbhip_tune_entry {
 } nbhip_tune_entry;
 
 /* Time every compiled (P, W) shape at j-splits 1, 2, 4 and 8 (and 16, 32 where the
- * i-blocks of the shape times the split stay within 16384 blocks) on the current bodies
+ * i-blocks of the shape times the split stay within 16384 blocks), and the pair-symmetric
+ * pass (each pair evaluated once, in tiles of 64*P*W-body blocks; reported with jsplit 0)
+ * where it has at least 64 tiles, on the current bodies
  * (accelerations only: positions and velocities do not move), and use the fastest for
  * this body count from now on.  Up to `cap` timings are written to `out` (may be NULL),
  * *count receives how many were taken, and *best (may be NULL) the chosen one.

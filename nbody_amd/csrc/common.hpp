@@ -80,6 +80,14 @@ struct DirectConfig {
     int P = 4;       // i-bodies per lane
     int W = 4;       // waves per block (all waves share the block's i-set, split j)
     int jsplit = 1;  // blocks along j per i-block (>1 -> partial sums + reduce pass)
+    int sym = 0;     // 1: each pair once, in tiles of 64*P*W-body blocks (jsplit unused)
+};
+
+// The pair-symmetric pass's tile table: block pairs (a, b), a <= b, for m blocks of B.
+struct SymTiles {
+    DevBuf<int2> dev;
+    size_t m = 0;
+    int B = 0;
 };
 
 // One system's device arrays, as the direct pass sees them.
@@ -140,6 +148,7 @@ struct nbhip_ctx {
 
     // scratch: j-split partial sums, staging for AoS transfers, reductions
     nbhip::DevBuf<float> partial;
+    nbhip::SymTiles sym_tiles;
     nbhip::DevBuf<float> aos;
     nbhip::DevBuf<double> dred;
     nbhip::DevBuf<float> fscratch;
@@ -206,7 +215,10 @@ namespace nbhip {
 // direct.hip
 void direct_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse_integrate);
 void direct_pass(hipStream_t stream, const SysView& v, const nbhip_params& p, bool fuse,
-                 DevBuf<float>& partial, const DirectConfig& cfg);
+                 DevBuf<float>& partial, const DirectConfig& cfg, SymTiles* tiles = nullptr);
+// Allocates (and uploads the tile table) for a pass of this shape; run before a capture.
+void direct_prepare(hipStream_t stream, size_t n, const DirectConfig& cfg, DevBuf<float>& partial,
+                    SymTiles* tiles);
 void integrate(nbhip_ctx* c, const nbhip_params& p);
 void diagnostics(nbhip_ctx* c, const nbhip_params& p, double out[9]);
 void force_error_f64(nbhip_ctx* c, const nbhip_params& p, const uint32_t* sample, size_t ns,
@@ -243,5 +255,6 @@ void check_ics(const nbhip_ics* s, size_t n);
 bool ics_have_radius(const nbhip_ics& s);
 void generate(nbhip_ctx* c, const nbhip_ics& s, size_t n, float4* pm, float4* vr, float4* acc);
 DirectConfig context_config(const nbhip_ctx* c);
+void direct_prepare_context(nbhip_ctx* c, const DirectConfig& cfg);
 
 }  // namespace nbhip

@@ -75,6 +75,204 @@ __global__ __launch_bounds__(WAVE* W) void k_force(ForceArgs a)
                           });
 }
 
+// ---- the pair-symmetric pass -----------------------------------------------------------
+// Bodies fall into blocks of B = 64*P*W; tile (a, b), a <= b, is one workgroup.  Its W waves
+// hold block a's bodies ("homes", P per lane, in registers); block b's bodies ("visitors")
+// pass through LDS 64 at a time.  Within such a slice, at step k lane l meets visitor
+// (l - k) mod 64, so over 64 steps every lane meets every visitor once, and no two lanes of
+// a wave meet the same visitor in the same step.  Each pair is evaluated once: one
+// |d|^2 + eps^2, one v_rsq_f32, and the force goes both ways -- onto the home (its
+// registers) and, with the opposite sign, onto the visitor (the wave's LDS row for that
+// visitor, a plain read-modify-write: one lane per slot per step, in step order).  The W
+// waves' rows for a slice are then added in wave order.  A diagonal tile (a == b) takes
+// the home side only, every pair within the block met from both ends.
+// Every body thus receives one partial sum per other block (slot b for its home side in
+// tile (block, b), slot a for its visitor side in tile (a, block)), written exactly once
+// into partial[slot][3][n]; k_reduce_slots adds a body's M slots in slot order.  Each
+// cutoff applies to its receiving body (home: its radius; visitor: its radius).
+struct SymArgs {
+    const float4* pm;
+    const float4* vr;
+    int n;
+    float eps2;
+    const int2* tiles;  // (a, b) per workgroup
+    float* partial;     // [M][3][n]
+};
+
+template <int P, int W, bool CUT>
+__global__ __launch_bounds__(WAVE* W) void k_force_sym(SymArgs s)
+{
+    constexpr int B = WAVE * P * W;
+    constexpr int SLICES = P * W;
+    // visitors of the slice in use and of the next one; each wave's rows of visitor sums for
+    // the slice in use and for the previous one, whose fold overlaps the next slice
+    __shared__ float4 vis[2][WAVE];
+    __shared__ float vr2[2][WAVE];
+    __shared__ float row[2][W][3][WAVE];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / WAVE);
+    const int2 t = s.tiles[blockIdx.x];
+    const bool diag = t.x == t.y;
+    const int n = s.n;
+    const int slices = min(SLICES, (n - t.y * B + WAVE - 1) / WAVE);  // block b may be short
+
+    float4 x[P];
+    float r2[P];
+    float S[P][3], C[P][3], T[P][3];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int h = t.x * B + (wave * P + p) * WAVE + lane;
+        x[p] = h < n ? s.pm[h] : make_float4(0.f, 0.f, 0.f, 0.f);  // padding: mass 0, adds 0 both ways
+        if (CUT) {
+            const float r = h < n ? s.vr[h].w : 0.f;
+            r2[p] = r * r;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) S[p][k] = C[p][k] = T[p][k] = 0.f;
+    }
+    const auto load_visitor = [&](int sl, float4& q, float& r) {
+        const int vj = t.y * B + sl * WAVE + lane;
+        q = vj < n ? s.pm[vj] : make_float4(0.f, 0.f, 0.f, 0.f);
+        r = 0.f;
+        if (CUT && vj < n) {
+            const float rad = s.vr[vj].w;
+            r = rad * rad;
+        }
+    };
+    // wave `sl % W` adds the W rows of slice sl in wave order and writes its visitors' slot
+    const auto fold = [&](int sl) {
+        if (wave != sl % W) return;
+        const int vj = t.y * B + sl * WAVE + lane;
+        if (vj >= n) return;
+        const int buf = sl & 1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float sum = row[buf][0][k][lane];
+#pragma unroll
+            for (int w = 1; w < W; ++w) sum += row[buf][w][k][lane];
+            s.partial[(size_t(t.x) * 3 + k) * size_t(n) + vj] = sum;
+        }
+    };
+    if (wave == 0) {
+        float4 q;
+        float r;
+        load_visitor(0, q, r);
+        vis[0][lane] = q;
+        vr2[0][lane] = r;
+    }
+    for (int sl = 0; sl < slices; ++sl) {
+        const int buf = sl & 1;
+        __syncthreads();  // slice sl's visitors are in; every wave's rows of slice sl - 1 too
+        if (!diag && sl > 0) fold(sl - 1);
+        float4 qn = make_float4(0.f, 0.f, 0.f, 0.f);
+        float rn = 0.f;
+        if (wave == 0 && sl + 1 < slices) load_visitor(sl + 1, qn, rn);  // in flight during the slice
+        float* rw = &row[buf][wave][0][0];
+        // Step k: lane l meets visitor v = (l - k) mod 64.  Its row slot was written by lane
+        // l - 1 in step k - 1 (at step 0 each lane starts its own slot); the compiler sees no
+        // dependence across lanes, so a compiler-only fence keeps each step's row accesses
+        // after the previous step's, and LDS runs a wave's accesses in program order.  The
+        // next visitor and this step's row sums are read before the arithmetic.
+        int v = lane;
+        float4 q = vis[buf][v];
+        float rv2 = CUT ? vr2[buf][v] : 0.f;
+#pragma unroll 2
+        for (int k = 0; k < WAVE; ++k) {
+            const int vn = (lane - k - 1) & (WAVE - 1);
+            const float4 qn_step = vis[buf][vn];
+            const float rn_step = CUT ? vr2[buf][vn] : 0.f;
+            float ox = 0.f, oy = 0.f, oz = 0.f;
+            if (!diag && k > 0) {
+                ox = rw[0 * WAVE + v];
+                oy = rw[1 * WAVE + v];
+                oz = rw[2 * WAVE + v];
+            }
+            float vx = 0.f, vy = 0.f, vz = 0.f;
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const float dx = q.x - x[p].x;
+                const float dy = q.y - x[p].y;
+                const float dz = q.z - x[p].z;
+                float wh, wv;
+                if (CUT) {
+                    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    const float inv = __builtin_amdgcn_rsqf(d2 + s.eps2);
+                    const float i3 = inv * inv * inv;
+                    wh = d2 > r2[p] ? q.w * i3 : 0.f;
+                    wv = d2 > rv2 ? x[p].w * i3 : 0.f;
+                } else {
+                    const float e2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, s.eps2)));
+                    const float inv = __builtin_amdgcn_rsqf(e2);
+                    const float i3 = inv * inv * inv;
+                    wh = q.w * i3;
+                    wv = x[p].w * i3;
+                }
+                T[p][0] = fmaf(wh, dx, T[p][0]);
+                T[p][1] = fmaf(wh, dy, T[p][1]);
+                T[p][2] = fmaf(wh, dz, T[p][2]);
+                vx = fmaf(-wv, dx, vx);
+                vy = fmaf(-wv, dy, vy);
+                vz = fmaf(-wv, dz, vz);
+            }
+            if (!diag) {
+                rw[0 * WAVE + v] = ox + vx;
+                rw[1 * WAVE + v] = oy + vy;
+                rw[2 * WAVE + v] = oz + vz;
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            v = vn;
+            q = qn_step;
+            rv2 = rn_step;
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                two_sum(S[p][k], C[p][k], T[p][k]);
+                T[p][k] = 0.f;
+            }
+        if (wave == 0 && sl + 1 < slices) {
+            vis[buf ^ 1][lane] = qn;  // last read in slice sl - 1, before this slice's barrier
+            vr2[buf ^ 1][lane] = rn;
+        }
+    }
+    if (!diag) {
+        __syncthreads();
+        fold(slices - 1);
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int h = t.x * B + (wave * P + p) * WAVE + lane;
+        if (h >= n) continue;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s.partial[(size_t(t.y) * 3 + k) * size_t(n) + h] = S[p][k] + C[p][k];
+    }
+}
+
+// Adds body i's M partial sums in slot order (compensated), eight loads in flight at a
+// time, then finishes it as OUT_ACC / OUT_STEP.
+__global__ __launch_bounds__(256) void k_reduce_slots(ForceArgs a, int slots)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const size_t n = a.n;
+    float s[3] = { 0.f, 0.f, 0.f }, c[3] = { 0.f, 0.f, 0.f };
+    const float4 xi = a.pm[i];
+    for (int y0 = 0; y0 < slots; y0 += 8) {
+        float v[8][3];
+#pragma unroll
+        for (int y = 0; y < 8; ++y)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                v[y][k] = y0 + y < slots ? a.partial[(size_t(y0 + y) * 3 + k) * n + i] : 0.f;
+#pragma unroll
+        for (int y = 0; y < 8; ++y)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) two_sum(s[k], c[k], v[y][k]);
+    }
+    finish_body(a, i, xi, a.G * (s[0] + c[0]), a.G * (s[1] + c[1]), a.G * (s[2] + c[2]));
+}
+
 // Combine j-split partials in a fixed order, then finish as OUT_ACC / OUT_STEP.  MAXJ >=
 // jsplit: every partial load is issued before the first add (at small N the pass is one
 // memory round trip, not jsplit of them); a missing chunk adds an exact zero.
@@ -316,6 +514,27 @@ ForceKernel pick_cut(bool cut)
     return cut ? (ForceKernel)k_force<P, W, true> : (ForceKernel)k_force<P, W, false>;
 }
 
+using SymKernel = void (*)(SymArgs);
+
+template <int P, int W>
+SymKernel pick_sym_cut(bool cut)
+{
+    return cut ? (SymKernel)k_force_sym<P, W, true> : (SymKernel)k_force_sym<P, W, false>;
+}
+
+SymKernel pick_sym(int P, int W, bool cut)
+{
+    if (W == 4) {
+        if (P == 2) return pick_sym_cut<2, 4>(cut);
+        if (P == 4) return pick_sym_cut<4, 4>(cut);
+        if (P == 8) return pick_sym_cut<8, 4>(cut);
+    } else if (W == 2) {
+        if (P == 4) return pick_sym_cut<4, 2>(cut);
+        if (P == 8) return pick_sym_cut<8, 2>(cut);
+    }
+    return nullptr;
+}
+
 ForceKernel pick_force(int P, int W, bool cut)
 {
 #define NBHIP_CASE(p, w) \
@@ -438,11 +657,60 @@ void launch_reduce(hipStream_t stream, const ForceArgs& a, int jsplit)
 
 }  // namespace
 
+size_t sym_blocks(size_t n, const DirectConfig& cfg)
+{
+    const size_t B = size_t(WAVE) * cfg.P * cfg.W;
+    return (n + B - 1) / B;
+}
+
+// The tile table (a, b), a <= b, row by row, for M blocks; uploaded when M or the block
+// size changes (never under stream capture: direct_prepare runs first).
+void ensure_sym_tiles(hipStream_t stream, SymTiles& t, size_t M, int B)
+{
+    if (t.m == M && t.B == B && t.dev.ptr) return;
+    std::vector<int2> h;
+    h.reserve(M * (M + 1) / 2);
+    for (size_t a = 0; a < M; ++a)
+        for (size_t b = a; b < M; ++b) h.push_back(make_int2(int(a), int(b)));
+    t.dev.grow(h.size(), 0, stream);
+    NBHIP_CHECK(hipMemcpyAsync(t.dev.ptr, h.data(), h.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
+    NBHIP_CHECK(hipStreamSynchronize(stream));
+    t.m = M;
+    t.B = B;
+}
+
+void direct_prepare(hipStream_t stream, size_t n, const DirectConfig& cfg, DevBuf<float>& partial, SymTiles* tiles)
+{
+    if (cfg.sym) {
+        if (!tiles) fail(NBHIP_ERR_INVALID, "the pair-symmetric pass needs a tile table");
+        const size_t M = sym_blocks(n, cfg);
+        ensure_sym_tiles(stream, *tiles, M, WAVE * cfg.P * cfg.W);
+        partial.grow(M * 3 * n, 0, stream);
+    } else if (cfg.jsplit > 1) {
+        partial.grow(size_t(cfg.jsplit) * 3 * n, 0, stream);
+    }
+}
+
 void direct_pass(hipStream_t stream, const SysView& v, const nbhip_params& p, bool fuse,
-                 DevBuf<float>& partial, const DirectConfig& cfg)
+                 DevBuf<float>& partial, const DirectConfig& cfg, SymTiles* tiles)
 {
     const size_t n = v.n;
     const bool cut = v.any_radius || p.eps == 0.f;
+    if (cfg.sym) {
+        const SymKernel k = pick_sym(cfg.P, cfg.W, cut);
+        if (!k) fail(NBHIP_ERR_INVALID, "no pair-symmetric kernel for P=%d W=%d", cfg.P, cfg.W);
+        direct_prepare(stream, n, cfg, partial, tiles);
+        const size_t M = tiles->m;
+        const SymArgs sa{ v.pm, v.vr, int(n), p.eps * p.eps, tiles->dev.ptr, partial.ptr };
+        hipLaunchKernelGGL(k, dim3(unsigned(M * (M + 1) / 2)), dim3(WAVE * cfg.W), 0, stream, sa);
+        NBHIP_CHECK(hipGetLastError());
+        ForceArgs a = force_args(v, p, 0);
+        a.partial = partial.ptr;
+        a.out_mode = fuse ? OUT_STEP : OUT_ACC;
+        hipLaunchKernelGGL(k_reduce_slots, dim3(grid1(n)), dim3(256), 0, stream, a, int(M));
+        NBHIP_CHECK(hipGetLastError());
+        return;
+    }
     size_t jchunk = 0;
     const int jsplit = split_chunks(n, cfg, jchunk);
     ForceArgs a = force_args(v, p, jchunk);
@@ -563,21 +831,59 @@ void accelerate_noalloc(nbhip_ctx* c, const float4* pm, const float4* vr, float4
 
 std::atomic<unsigned long long> g_buffer_epoch{ 0 };
 
+// The pair-symmetric pass by default where it fills the GPU (at least 2048 tiles) and its
+// partial sums (12 bytes per body per block) stay within SYM_PARTIAL_MAX: blocks of 1024
+// bodies (P = 4, W = 4: 14.4 ms against 17.5 for the best one-sided shape at N = 262,144),
+// or of 2048 where that keeps the partial sums within bounds.
+constexpr size_t SYM_PARTIAL_MAX = size_t(8) << 30;
+
+DirectConfig default_sym_config(size_t n)
+{
+    DirectConfig cfg;
+    for (const int P : { 4, 8 }) {
+        cfg.P = P;
+        cfg.W = 4;
+        cfg.sym = 1;
+        const size_t M = sym_blocks(n, cfg);
+        const size_t bytes = M * 3 * n * sizeof(float);
+        if (M * (M + 1) / 2 < 2048) break;  // too few tiles; a larger block only has fewer
+        if (P == 4 && bytes <= (size_t(2) << 30)) return cfg;
+        if (P == 8 && bytes <= SYM_PARTIAL_MAX) return cfg;
+    }
+    cfg.sym = 0;
+    return cfg;
+}
+
 // Launch shape for a context: an explicit override in the environment wins, then the
 // autotuner's choice for this body count, then the default heuristic.
 DirectConfig context_config(const nbhip_ctx* c)
 {
     const bool forced = std::getenv("NBHIP_DIRECT_P") || std::getenv("NBHIP_DIRECT_W") ||
-                        std::getenv("NBHIP_DIRECT_JSPLIT");
+                        std::getenv("NBHIP_DIRECT_JSPLIT") || std::getenv("NBHIP_DIRECT_SYM");
     if (!forced && c->tuned_n == c->count && c->tuned_n != 0) return c->tuned;
-    return choose_direct_config(c->count);
+    DirectConfig cfg = choose_direct_config(c->count);
+    const int sym = env_int("NBHIP_DIRECT_SYM", -1);
+    if (sym > 0) {
+        cfg.sym = 1;
+        cfg.P = env_int("NBHIP_DIRECT_P", 4);
+        cfg.W = env_int("NBHIP_DIRECT_W", 4);
+    } else if (sym < 0 && !forced) {
+        const DirectConfig s = default_sym_config(c->count);
+        if (s.sym) cfg = s;
+    }
+    return cfg;
+}
+
+void direct_prepare_context(nbhip_ctx* c, const DirectConfig& cfg)
+{
+    direct_prepare(c->stream, c->count, cfg, c->partial, &c->sym_tiles);
 }
 
 void direct_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse)
 {
     ensure_radius_flag(c);
     SysView v{ c->pm[c->cur].ptr, c->pm[1 - c->cur].ptr, c->vr.ptr, c->acc.ptr, c->count, c->any_radius };
-    direct_pass(c->stream, v, p, fuse, c->partial, context_config(c));
+    direct_pass(c->stream, v, p, fuse, c->partial, context_config(c), &c->sym_tiles);
     c->launches += 1;
     if (fuse) c->cur = 1 - c->cur;
 }
@@ -592,6 +898,31 @@ void tune_direct(nbhip_ctx* c, const nbhip_params& p, std::vector<nbhip_tune_ent
                      c->any_radius };
     const size_t n = c->count;
     best = nbhip_tune_entry{ 0, 0, 0, 1e30f };
+    const auto timed = [&](const DirectConfig& cfg) {
+        direct_pass(c->stream, v, p, false, c->partial, cfg, &c->sym_tiles);  // warm
+        NBHIP_CHECK(hipEventRecord(c->ev_start, c->stream));
+        for (int r = 0; r < 2; ++r) direct_pass(c->stream, v, p, false, c->partial, cfg, &c->sym_tiles);
+        NBHIP_CHECK(hipEventRecord(c->ev_stop, c->stream));
+        NBHIP_CHECK(hipEventSynchronize(c->ev_stop));
+        float ms = 0.f;
+        NBHIP_CHECK(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+        // the pair-symmetric pass is reported with j-split 0
+        const nbhip_tune_entry e{ cfg.P, cfg.W, cfg.sym ? 0 : cfg.jsplit, ms / 2 };
+        all.push_back(e);
+        if (e.ms < best.ms) best = e;
+    };
+    // the pair-symmetric tiles, where there are at least 64 tiles and the partial sums (one
+    // per body per block) stay within SYM_PARTIAL_MAX
+    static const int sym_shapes[][2] = { { 2, 4 }, { 4, 4 }, { 8, 4 }, { 4, 2 }, { 8, 2 } };
+    for (const auto& sh : sym_shapes) {
+        DirectConfig cfg;
+        cfg.P = sh[0];
+        cfg.W = sh[1];
+        cfg.sym = 1;
+        const size_t M = sym_blocks(n, cfg);
+        if (M * (M + 1) / 2 < 64 || M * 3 * n * sizeof(float) > SYM_PARTIAL_MAX) continue;
+        timed(cfg);
+    }
     for (const auto& sh : shapes)
         for (int js : { 1, 2, 4, 8, 16, 32 }) {
             DirectConfig cfg;
@@ -603,21 +934,13 @@ void tune_direct(nbhip_ctx* c, const nbhip_params& p, std::vector<nbhip_tune_ent
             const size_t iblocks = (n + size_t(WAVE) * cfg.P - 1) / (size_t(WAVE) * cfg.P);
             if (js > 8 && iblocks * size_t(js) > 16384) continue;
             cfg.jsplit = js;
-            direct_pass(c->stream, v, p, false, c->partial, cfg);  // warm
-            NBHIP_CHECK(hipEventRecord(c->ev_start, c->stream));
-            for (int r = 0; r < 2; ++r) direct_pass(c->stream, v, p, false, c->partial, cfg);
-            NBHIP_CHECK(hipEventRecord(c->ev_stop, c->stream));
-            NBHIP_CHECK(hipEventSynchronize(c->ev_stop));
-            float ms = 0.f;
-            NBHIP_CHECK(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
-            const nbhip_tune_entry e{ cfg.P, cfg.W, js, ms / 2 };
-            all.push_back(e);
-            if (e.ms < best.ms) best = e;
+            timed(cfg);
         }
     c->tuned_n = n;
     c->tuned.P = best.P;
     c->tuned.W = best.W;
-    c->tuned.jsplit = best.jsplit;
+    c->tuned.sym = best.jsplit == 0 ? 1 : 0;
+    c->tuned.jsplit = best.jsplit == 0 ? 1 : best.jsplit;
 }
 
 void integrate(nbhip_ctx* c, const nbhip_params& p)

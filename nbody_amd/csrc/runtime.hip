@@ -177,6 +177,7 @@ void nbhip_destroy(nbhip_ctx* c)
     c->vr.release();
     c->acc.release();
     c->partial.release();
+    c->sym_tiles.dev.release();
     c->aos.release();
     c->dred.release();
     c->fscratch.release();
@@ -442,13 +443,13 @@ void step_graph(nbhip_ctx* c, const nbhip_params& p, int steps)
     nbhip::ensure_radius_flag(c);
     const nbhip::DirectConfig cfg = nbhip::context_config(c);
     if (p.mode == NBHIP_MODE_N2) {
-        if (cfg.jsplit > 1) c->partial.grow(size_t(cfg.jsplit) * 3 * c->count, 0, c->stream);
+        nbhip::direct_prepare_context(c, cfg);
     } else {
         nbhip::bh_reserve(c);
     }
     char key[256];
-    std::snprintf(key, sizeof key, "%d/%d/%zu/%d/%d/%d/%d/%d/%d/%llu/", p.mode, steps, c->count, c->cur,
-                  cfg.P, cfg.W, cfg.jsplit, c->bh_split, int(c->any_radius), nbhip::g_buffer_epoch.load());
+    std::snprintf(key, sizeof key, "%d/%d/%zu/%d/%d/%d/%d/%d/%d/%d/%llu/", p.mode, steps, c->count, c->cur,
+                  cfg.P, cfg.W, cfg.jsplit, cfg.sym, c->bh_split, int(c->any_radius), nbhip::g_buffer_epoch.load());
     std::string k(key);
     k.append(reinterpret_cast<const char*>(&p), sizeof p);
     const int cur_after = p.mode == NBHIP_MODE_N2 ? (c->cur ^ (steps & 1)) : c->cur;

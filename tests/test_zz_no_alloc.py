@@ -1,11 +1,15 @@
 """nbhip_accelerate_noalloc: the direct sum over the caller's device arrays, with its
 scratch in a static device buffer (no allocation), through the C ABI.
 
-Bar: bit-identical to nbhip_accelerate on a context holding the same bodies (the same
-kernels and launch shape), on consecutive calls of different sizes -- the second smaller,
-so stale partial sums or a stale radius flag in the shared scratch would show -- with and
-without radius cutoffs and softening; and refusals that leave the GPU untouched.
+Bar: bit-identical to nbhip_accelerate's tiled pass on a context holding the same bodies
+(the same kernels and launch shape), on consecutive calls of different sizes -- the second
+smaller, so stale partial sums or a stale radius flag in the shared scratch would show --
+with and without radius cutoffs and softening; within 2e-6 of nbhip_accelerate's default,
+which above 65,536 bodies is the pair-symmetric pass; and refusals that leave the GPU
+untouched.
 """
+import os
+
 import ctypes as C
 
 import numpy as np
@@ -59,12 +63,22 @@ def split(b):
     return pm, vr
 
 
-def reference(b, p):
-    # nbhip_accelerate on a fresh (untuned) context holding the same bodies
-    with _lib.Context(0) as c:
-        c.upload_bodies(b)
-        c.accelerate(p)
-        return c.download(_lib.RB_ACCELERATIONS)[2]
+def reference(b, p, tiled=True):
+    # nbhip_accelerate on a fresh (untuned) context holding the same bodies: its tiled
+    # pass (the kernels the no-allocation entry point shares), or its default choice
+    old = os.environ.get("NBHIP_DIRECT_SYM")
+    if tiled:
+        os.environ["NBHIP_DIRECT_SYM"] = "0"
+    try:
+        with _lib.Context(0) as c:
+            c.upload_bodies(b)
+            c.accelerate(p)
+            return c.download(_lib.RB_ACCELERATIONS)[2]
+    finally:
+        if old is None:
+            os.environ.pop("NBHIP_DIRECT_SYM", None)
+        else:
+            os.environ["NBHIP_DIRECT_SYM"] = old
 
 
 CASES = [
@@ -96,6 +110,10 @@ def test_noalloc_matches_accelerate_on_consecutive_calls():
                 got = acc_d.get((n, 4))
                 want = reference(b, p)
                 assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), k
+                if n >= 65536:
+                    dflt = reference(b, p, tiled=False)[:, :3].astype(np.float64)
+                    e = np.linalg.norm(got[:, :3] - dflt, axis=1) / np.linalg.norm(dflt, axis=1)
+                    assert e.max() <= 2e-6, (k, e.max())
     finally:
         for d in (pm_d, vr_d, acc_d):
             d.free()
