@@ -355,7 +355,8 @@ def run_batch(args) -> dict:
         "value": rate / 1e9, "unit": "G-interactions/s",
         "ms_per_step": ms / steps,
         "pct_fp32_peak": 100.0 * FLOPS_PER_INTERACTION * rate / (FP32_PEAK_TFLOPS * 1e12),
-        "launch": {"P": info.P, "W": info.W, "persistent": bool(info.persistent),
+        "launch": {"P": info.P, "W": info.W,
+                   "mode": {0: "grid", 1: "persistent", 2: "pairs"}.get(info.launch, info.launch),
                    "autotuned_ms_per_step": tuned_ms},
         "unbatched_value": rate_base / 1e9,
         "unbatched_ms_per_step": ms_base / base_steps,

@@ -409,8 +409,10 @@ int nbhip_generate(nbhip_ctx* ctx, const nbhip_ics* spec, size_t n);
  * No reference counterpart: the reference runs one simulation per Sim.  A batch lives on
  * a context (its stream, its device) and holds systems of any size, each with its own
  * parameters (dt, G, eps, size, wrap; mode must be N2).  One step advances every system
- * with one persistent launch over a work table of (system, block of bodies) items,
- * largest systems first; a body's force is summed in the same order whatever else is in
+ * with one launch over a work table, largest systems first: (system, block of bodies)
+ * items of the one-sided tile, or in pairs mode (system, a, b) tiles of the pair-symmetric
+ * tile, each pair of a system evaluated once, followed by a launch that adds each body's
+ * per-block partial sums.  A body's force is summed in the same order whatever else is in
  * the batch, so each system's result is the same alone and in any mix. */
 typedef struct nbhip_batch nbhip_batch;
 
@@ -418,13 +420,18 @@ typedef struct nbhip_batch nbhip_batch;
 #define NBHIP_BATCH_PER_SYSTEM 2u  /* one launch per system per step: the unbatched path   */
 #define NBHIP_BATCH_ACCELERATE 4u  /* accelerations only, no kick-drift                    */
 
+/* launch modes (nbhip_batch_configure) */
+#define NBHIP_BATCH_LAUNCH_GRID       0  /* one block per (system, i-block) item                */
+#define NBHIP_BATCH_LAUNCH_PERSISTENT 1  /* a resident grid taking items from a counter         */
+#define NBHIP_BATCH_LAUNCH_PAIRS      2  /* pair-symmetric tiles of 64*P*W-body blocks per system */
+
 typedef struct nbhip_batch_info {
     size_t systems;
     size_t bodies;
     size_t items;                  /* work-table entries */
     double interactions_per_step;  /* sum of N^2 */
     size_t device_bytes;           /* body storage held on the device */
-    int    P, W, persistent;       /* launch configuration in use */
+    int    P, W, launch;           /* launch configuration in use (NBHIP_BATCH_LAUNCH_*) */
 } nbhip_batch_info;
 
 int  nbhip_batch_create(nbhip_ctx* ctx, nbhip_batch** out);
@@ -449,11 +456,12 @@ int nbhip_batch_download(nbhip_batch* b, int id, void* bodies, size_t n);
 
 int nbhip_batch_query(nbhip_batch* b, nbhip_batch_info* out);
 
-/* Choose the launch configuration by hand: P bodies per lane, W waves per block,
- * persistent (1) or one block per item (0). */
-int nbhip_batch_configure(nbhip_batch* b, int P, int W, int persistent);
+/* Choose the launch configuration by hand: P bodies per lane, W waves per block, and the
+ * launch mode NBHIP_BATCH_LAUNCH_GRID, _PERSISTENT or _PAIRS (pairs mode: P in 2, 4, 8,
+ * W = 4). */
+int nbhip_batch_configure(nbhip_batch* b, int P, int W, int launch);
 
-/* Autotuner: time the compiled (P, W, persistent) candidates on the batch as it is
+/* Autotuner: time the compiled (P, W, launch mode) candidates on the batch as it is
  * (accelerations only, the state does not move), keep the fastest; *best_ms (may be
  * NULL) receives its time per step.  Synchronises. */
 int nbhip_batch_tune(nbhip_batch* b, double* best_ms);

@@ -58,6 +58,10 @@ class Params(C.Structure):
     ]
 
 
+BATCH_LAUNCH_GRID = 0
+BATCH_LAUNCH_PERSISTENT = 1
+BATCH_LAUNCH_PAIRS = 2
+
 BATCH_GRAPH = 1
 BATCH_PER_SYSTEM = 2
 BATCH_ACCELERATE = 4
@@ -73,7 +77,7 @@ class BatchInfo(C.Structure):
     _fields_ = [
         ("systems", C.c_size_t), ("bodies", C.c_size_t), ("items", C.c_size_t),
         ("interactions_per_step", C.c_double), ("device_bytes", C.c_size_t),
-        ("P", C.c_int), ("W", C.c_int), ("persistent", C.c_int),
+        ("P", C.c_int), ("W", C.c_int), ("launch", C.c_int),
     ]
 
 
@@ -480,8 +484,10 @@ class Batch:
         self.ctx._check(self.L.nbhip_batch_query(self.h, C.byref(out)))
         return out
 
-    def configure(self, P: int, W: int, persistent: bool = True) -> None:
-        self.ctx._check(self.L.nbhip_batch_configure(self.h, P, W, int(persistent)))
+    def configure(self, P: int, W: int, launch: int = BATCH_LAUNCH_PERSISTENT) -> None:
+        """P bodies per lane, W waves per block, launch mode BATCH_LAUNCH_GRID, _PERSISTENT
+        (True and False mean those two) or _PAIRS."""
+        self.ctx._check(self.L.nbhip_batch_configure(self.h, P, W, int(launch)))
 
     def tune(self) -> float:
         ms = C.c_double()

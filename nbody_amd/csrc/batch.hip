@@ -36,6 +36,8 @@ struct SysDesc {
     float size;
     int wrap;
     int cut;     // radius cutoff variant (some radius > 0, or eps == 0)
+    float* partial;  // pairs mode: [slots][3][n] per-block partial sums
+    int slots;       // pairs mode: blocks of 64*P*W bodies
 };
 
 template <int P, int W>
@@ -78,6 +80,50 @@ __global__ __launch_bounds__(WAVE* W) void k_batch(const SysDesc* __restrict__ s
         it = s_item;
         __syncthreads();
     }
+}
+
+// Pairs mode: one workgroup per tile (system, a, b) of the pair-symmetric tile, then one
+// thread per body adds its per-block partial sums in slot order and finishes it.
+template <int P, int W, bool CUT>
+__global__ __launch_bounds__(WAVE* W) void k_batch_sym(const SysDesc* __restrict__ sys, const int4* __restrict__ tiles,
+                                                       int parity)
+{
+    __shared__ SymShared<W> sh;
+    const int4 t = tiles[blockIdx.x];
+    const SysDesc& d = sys[t.x];
+    sym_tile<P, W, CUT>(sh, d.pm[d.cur ^ parity], d.vr, d.n, d.eps2, t.y, t.z, d.partial);
+}
+
+__global__ __launch_bounds__(256) void k_batch_sym_reduce(const SysDesc* __restrict__ sys, const int2* __restrict__ items,
+                                                          int parity, int fuse)
+{
+    const int2 w = items[blockIdx.x];
+    const SysDesc& d = sys[w.x];
+    const int i = w.y + int(threadIdx.x);
+    if (i >= d.n) return;
+    const int cur = d.cur ^ parity;
+    float tot[3];
+    sym_slot_sum(d.partial, d.slots, size_t(d.n), i, tot);
+    const float ax = d.G * tot[0], ay = d.G * tot[1], az = d.G * tot[2];
+    d.acc[i] = make_float4(ax, ay, az, 0.f);
+    if (fuse) {
+        float4 xi = d.pm[cur][i];
+        float4 v = d.vr[i];
+        kick_drift(xi, v, ax, ay, az, d.dt, d.size, d.wrap);
+        d.pm[cur ^ 1][i] = xi;
+        d.vr[i] = v;
+    }
+}
+
+using SymBatchKernel = void (*)(const SysDesc*, const int4*, int);
+
+SymBatchKernel pick_batch_sym(int P, int W, bool cut)
+{
+#define NBHIP_CASE(p, w) \
+    if (P == p && W == w) return cut ? (SymBatchKernel)k_batch_sym<p, w, true> : (SymBatchKernel)k_batch_sym<p, w, false>;
+    NBHIP_CASE(2, 4) NBHIP_CASE(4, 4) NBHIP_CASE(8, 4)
+#undef NBHIP_CASE
+    return nullptr;
 }
 
 __global__ void k_fill_u32(unsigned* p, int n, unsigned v)
@@ -132,9 +178,17 @@ struct nbhip_batch {
     size_t bodies = 0;
     size_t device_bytes = 0;
 
-    // launch configuration
+    // launch configuration: NBHIP_BATCH_LAUNCH_GRID, _PERSISTENT or _PAIRS
     int P = 2, W = 8;
-    bool persistent = true;
+    int launch = NBHIP_BATCH_LAUNCH_PERSISTENT;
+
+    // pairs mode: tiles (system, a, b, 0) of the cutoff and plain variants, reduction items
+    // (system, first body) of 256 bodies, and every system's partial sums
+    std::vector<int4> h_tiles[2];
+    std::vector<int2> h_red;
+    DevBuf<int4> d_tiles[2];
+    DevBuf<int2> d_red;
+    DevBuf<float> sym_partial;
 
     // captured step sequence
     hipGraph_t graph = nullptr;
@@ -142,7 +196,7 @@ struct nbhip_batch {
     int g_steps = -1, g_parity = -1, g_flags = -1;
     long g_version = -1;
     int g_P = 0, g_W = 0;
-    bool g_persistent = false;
+    int g_launch = -1;
     const unsigned* g_counters = nullptr;  // the step counters the graph was captured with
 
     std::map<int, int> grid_cache;  // persistent grid per (P, W)
@@ -156,6 +210,10 @@ struct nbhip_batch {
         d_sys.release();
         d_items.release();
         d_counters.release();
+        d_tiles[0].release();
+        d_tiles[1].release();
+        d_red.release();
+        sym_partial.release();
         partial.release();
         aos.release();
         for (auto& kv : systems) release(*kv.second);
@@ -211,6 +269,36 @@ void rebuild_tables(nbhip_batch* b)
     }
     std::stable_sort(order.begin(), order.end(), [](auto& x, auto& y) { return x.first > y.first; });
     const int per = WAVE * b->P;
+    b->h_tiles[0].clear();
+    b->h_tiles[1].clear();
+    b->h_red.clear();
+    if (b->launch == NBHIP_BATCH_LAUNCH_PAIRS) {
+        // per system: blocks of B bodies, its tiles (largest systems first) and partial sums
+        const size_t B = size_t(WAVE) * b->P * b->W;
+        size_t off = 0;
+        for (auto& o : order) {
+            SysDesc& d = b->h_sys[size_t(o.second)];
+            const int M = int((o.first + B - 1) / B);
+            d.slots = M;
+            d.partial = reinterpret_cast<float*>(off);  // offset until the buffer exists
+            off += size_t(M) * 3 * o.first;
+            for (int a = 0; a < M; ++a)
+                for (int c = a; c < M; ++c) b->h_tiles[d.cut ? 1 : 0].push_back(make_int4(o.second, a, c, 0));
+            for (int ib = 0; ib < int(o.first); ib += 256) b->h_red.push_back(make_int2(o.second, ib));
+        }
+        b->sym_partial.grow(std::max<size_t>(1, off), 0, s);
+        for (auto& d : b->h_sys) d.partial = b->sym_partial.ptr + reinterpret_cast<size_t>(d.partial);
+        for (int k = 0; k < 2; ++k) {
+            b->d_tiles[k].grow(std::max<size_t>(1, b->h_tiles[k].size()), 0, s);
+            if (!b->h_tiles[k].empty())
+                NBHIP_CHECK(hipMemcpyAsync(b->d_tiles[k].ptr, b->h_tiles[k].data(), b->h_tiles[k].size() * sizeof(int4),
+                                           hipMemcpyHostToDevice, s));
+        }
+        b->d_red.grow(std::max<size_t>(1, b->h_red.size()), 0, s);
+        if (!b->h_red.empty())
+            NBHIP_CHECK(hipMemcpyAsync(b->d_red.ptr, b->h_red.data(), b->h_red.size() * sizeof(int2),
+                                       hipMemcpyHostToDevice, s));
+    }
     for (auto& o : order)
         for (int ib = 0; ib < int(o.first); ib += per) b->h_items.push_back(make_int2(o.second, ib));
     b->d_sys.grow(std::max<size_t>(1, b->h_sys.size()), 0, s);
@@ -253,24 +341,53 @@ int persistent_grid(nbhip_batch* b)
 // device query, so it can run under stream capture once prepare_steps() has run.
 void prepare_steps(nbhip_batch* b, int steps)
 {
+    if (b->launch == NBHIP_BATCH_LAUNCH_PAIRS) {
+        if (!pick_batch_sym(b->P, b->W, false))
+            fail(NBHIP_ERR_INVALID, "no pairs-mode batch kernel for P=%d W=%d", b->P, b->W);
+        return;
+    }
     if (!pick_batch(b->P, b->W)) fail(NBHIP_ERR_INVALID, "no batch kernel for P=%d W=%d", b->P, b->W);
     b->d_counters.grow(size_t(std::max(steps, 1)), 0, b->ctx->stream);
     (void)persistent_grid(b);
+}
+
+void enqueue_steps_pairs(nbhip_batch* b, int steps, int parity, bool fuse)
+{
+    hipStream_t s = b->ctx->stream;
+    for (int k_ = 0; k_ < steps; ++k_) {
+        const int par = fuse ? (parity ^ (k_ & 1)) : parity;
+        for (int cut = 0; cut < 2; ++cut) {
+            const size_t nt = b->h_tiles[cut].size();
+            if (nt == 0) continue;
+            hipLaunchKernelGGL(pick_batch_sym(b->P, b->W, cut != 0), dim3(unsigned(nt)), dim3(WAVE * b->W), 0, s,
+                               b->d_sys.ptr, b->d_tiles[cut].ptr, par);
+            NBHIP_CHECK(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_batch_sym_reduce, dim3(unsigned(b->h_red.size())), dim3(256), 0, s, b->d_sys.ptr,
+                           b->d_red.ptr, par, fuse ? 1 : 0);
+        NBHIP_CHECK(hipGetLastError());
+        b->ctx->launches += 1;
+    }
 }
 
 void enqueue_steps(nbhip_batch* b, int steps, int parity, bool fuse)
 {
     const int nitems = int(b->h_items.size());
     if (nitems == 0 || steps == 0) return;
+    if (b->launch == NBHIP_BATCH_LAUNCH_PAIRS) {
+        enqueue_steps_pairs(b, steps, parity, fuse);
+        return;
+    }
     hipStream_t s = b->ctx->stream;
     BatchKernel k = pick_batch(b->P, b->W);
     hipLaunchKernelGGL(k_fill_u32, dim3((steps + 255) / 256), dim3(256), 0, s, b->d_counters.ptr, steps, 0u);
     NBHIP_CHECK(hipGetLastError());
-    const int grid = b->persistent ? std::min(nitems, b->grid_cache.at(b->P * 16 + b->W)) : nitems;
+    const bool persistent = b->launch == NBHIP_BATCH_LAUNCH_PERSISTENT;
+    const int grid = persistent ? std::min(nitems, b->grid_cache.at(b->P * 16 + b->W)) : nitems;
     for (int k_ = 0; k_ < steps; ++k_) {
         const int par = fuse ? (parity ^ (k_ & 1)) : parity;
         hipLaunchKernelGGL(k, dim3(grid), dim3(WAVE * b->W), 0, s, b->d_sys.ptr, b->d_items.ptr, nitems,
-                           b->d_counters.ptr + k_, par, b->persistent ? 1 : 0, fuse ? 1 : 0);
+                           b->d_counters.ptr + k_, par, persistent ? 1 : 0, fuse ? 1 : 0);
         NBHIP_CHECK(hipGetLastError());
         b->ctx->launches += 1;
     }
@@ -432,14 +549,18 @@ int nbhip_batch_remove(nbhip_batch* b, int id)
     });
 }
 
-int nbhip_batch_configure(nbhip_batch* b, int P, int W, int persistent)
+int nbhip_batch_configure(nbhip_batch* b, int P, int W, int launch)
 {
     return batch_guarded(b, [&] {
-        if (!nbhip::pick_batch(P, W)) fail(NBHIP_ERR_INVALID, "no batch kernel for P=%d W=%d", P, W);
-        if (P != b->P) b->dirty = true;  // the item table depends on P
+        if (launch != NBHIP_BATCH_LAUNCH_GRID && launch != NBHIP_BATCH_LAUNCH_PERSISTENT &&
+            launch != NBHIP_BATCH_LAUNCH_PAIRS)
+            fail(NBHIP_ERR_INVALID, "unknown batch launch mode %d", launch);
+        if (launch == NBHIP_BATCH_LAUNCH_PAIRS ? !nbhip::pick_batch_sym(P, W, false) : !nbhip::pick_batch(P, W))
+            fail(NBHIP_ERR_INVALID, "no batch kernel for P=%d W=%d in launch mode %d", P, W, launch);
+        if (P != b->P || W != b->W || launch != b->launch) b->dirty = true;  // the tables depend on them
         b->P = P;
         b->W = W;
-        b->persistent = persistent != 0;
+        b->launch = launch;
     });
 }
 
@@ -458,7 +579,7 @@ int nbhip_batch_step(nbhip_batch* b, int steps, unsigned flags)
         if (flags & NBHIP_BATCH_GRAPH) {
             const bool reuse = b->exec && b->g_steps == steps && b->g_parity == b->parity &&
                                b->g_flags == int(flags) && b->g_version == b->version && b->g_P == b->P &&
-                               b->g_W == b->W && b->g_persistent == b->persistent &&
+                               b->g_W == b->W && b->g_launch == b->launch &&
                                b->g_counters == b->d_counters.ptr && size_t(steps) <= b->d_counters.cap;
             if (!reuse) {
                 b->drop_graph();
@@ -480,7 +601,7 @@ int nbhip_batch_step(nbhip_batch* b, int steps, unsigned flags)
                 b->g_version = b->version;
                 b->g_P = b->P;
                 b->g_W = b->W;
-                b->g_persistent = b->persistent;
+                b->g_launch = b->launch;
                 b->g_counters = b->d_counters.ptr;
             }
             NBHIP_CHECK(hipGraphLaunch(b->exec, s));
@@ -522,7 +643,7 @@ int nbhip_batch_query(nbhip_batch* b, nbhip_batch_info* out)
         out->device_bytes = b->device_bytes;
         out->P = b->P;
         out->W = b->W;
-        out->persistent = b->persistent ? 1 : 0;
+        out->launch = b->launch;
     });
 }
 
@@ -530,17 +651,20 @@ int nbhip_batch_tune(nbhip_batch* b, double* best_ms)
 {
     return batch_guarded(b, [&] {
         if (b->systems.empty()) return;
-        static const int cand[][2] = { { 1, 8 }, { 2, 4 }, { 2, 8 }, { 4, 4 }, { 4, 8 }, { 8, 4 } };
+        // (P, W, launch): the one-sided tile per item, one block each or persistent, and the
+        // pair-symmetric tiles
+        static const int cand[][3] = { { 1, 8, 0 }, { 2, 4, 0 }, { 2, 8, 0 }, { 4, 4, 0 }, { 4, 8, 0 }, { 8, 4, 0 },
+                                       { 1, 8, 1 }, { 2, 4, 1 }, { 2, 8, 1 }, { 4, 4, 1 }, { 4, 8, 1 }, { 8, 4, 1 },
+                                       { 2, 4, 2 }, { 4, 4, 2 }, { 8, 4, 2 } };
         hipStream_t s = b->ctx->stream;
         double best = 1e300;
-        int bp = b->P, bw = b->W;
-        bool bpers = b->persistent;
-        for (const auto& c : cand)
-            for (int pers = 0; pers < 2; ++pers) {
-                if (c[0] != b->P) b->dirty = true;
+        int bp = b->P, bw = b->W, bl = b->launch;
+        for (const auto& c : cand) {
+            {
+                if (c[0] != b->P || c[1] != b->W || c[2] != b->launch) b->dirty = true;
                 b->P = c[0];
                 b->W = c[1];
-                b->persistent = pers != 0;
+                b->launch = c[2];
                 nbhip::rebuild_tables(b);
                 nbhip::prepare_steps(b, 2);
                 // accelerations only: the state does not move while tuning
@@ -555,13 +679,14 @@ int nbhip_batch_tune(nbhip_batch* b, double* best_ms)
                     best = ms / 2;
                     bp = c[0];
                     bw = c[1];
-                    bpers = pers != 0;
+                    bl = c[2];
                 }
             }
-        if (bp != b->P) b->dirty = true;
+        }
+        if (bp != b->P || bw != b->W || bl != b->launch) b->dirty = true;
         b->P = bp;
         b->W = bw;
-        b->persistent = bpers;
+        b->launch = bl;
         nbhip::rebuild_tables(b);
         if (best_ms) *best_ms = best;
     });

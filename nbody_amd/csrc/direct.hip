@@ -75,21 +75,7 @@ __global__ __launch_bounds__(WAVE* W) void k_force(ForceArgs a)
                           });
 }
 
-// ---- the pair-symmetric pass -----------------------------------------------------------
-// Bodies fall into blocks of B = 64*P*W; tile (a, b), a <= b, is one workgroup.  Its W waves
-// hold block a's bodies ("homes", P per lane, in registers); block b's bodies ("visitors")
-// pass through LDS 64 at a time.  Within such a slice, at step k lane l meets visitor
-// (l - k) mod 64, so over 64 steps every lane meets every visitor once, and no two lanes of
-// a wave meet the same visitor in the same step.  Each pair is evaluated once: one
-// |d|^2 + eps^2, one v_rsq_f32, and the force goes both ways -- onto the home (its
-// registers) and, with the opposite sign, onto the visitor (the wave's LDS row for that
-// visitor, a plain read-modify-write: one lane per slot per step, in step order).  The W
-// waves' rows for a slice are then added in wave order.  A diagonal tile (a == b) takes
-// the home side only, every pair within the block met from both ends.
-// Every body thus receives one partial sum per other block (slot b for its home side in
-// tile (block, b), slot a for its visitor side in tile (a, block)), written exactly once
-// into partial[slot][3][n]; k_reduce_slots adds a body's M slots in slot order.  Each
-// cutoff applies to its receiving body (home: its radius; visitor: its radius).
+// ---- the pair-symmetric pass (sym_tile, force_tile.hpp) ---------------------------------
 struct SymArgs {
     const float4* pm;
     const float4* vr;
@@ -102,151 +88,9 @@ struct SymArgs {
 template <int P, int W, bool CUT>
 __global__ __launch_bounds__(WAVE* W) void k_force_sym(SymArgs s)
 {
-    constexpr int B = WAVE * P * W;
-    constexpr int SLICES = P * W;
-    // visitors of the slice in use and of the next one; each wave's rows of visitor sums for
-    // the slice in use and for the previous one, whose fold overlaps the next slice
-    __shared__ float4 vis[2][WAVE];
-    __shared__ float vr2[2][WAVE];
-    __shared__ float row[2][W][3][WAVE];
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / WAVE);
+    __shared__ SymShared<W> sh;
     const int2 t = s.tiles[blockIdx.x];
-    const bool diag = t.x == t.y;
-    const int n = s.n;
-    const int slices = min(SLICES, (n - t.y * B + WAVE - 1) / WAVE);  // block b may be short
-
-    float4 x[P];
-    float r2[P];
-    float S[P][3], C[P][3], T[P][3];
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        const int h = t.x * B + (wave * P + p) * WAVE + lane;
-        x[p] = h < n ? s.pm[h] : make_float4(0.f, 0.f, 0.f, 0.f);  // padding: mass 0, adds 0 both ways
-        if (CUT) {
-            const float r = h < n ? s.vr[h].w : 0.f;
-            r2[p] = r * r;
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) S[p][k] = C[p][k] = T[p][k] = 0.f;
-    }
-    const auto load_visitor = [&](int sl, float4& q, float& r) {
-        const int vj = t.y * B + sl * WAVE + lane;
-        q = vj < n ? s.pm[vj] : make_float4(0.f, 0.f, 0.f, 0.f);
-        r = 0.f;
-        if (CUT && vj < n) {
-            const float rad = s.vr[vj].w;
-            r = rad * rad;
-        }
-    };
-    // wave `sl % W` adds the W rows of slice sl in wave order and writes its visitors' slot
-    const auto fold = [&](int sl) {
-        if (wave != sl % W) return;
-        const int vj = t.y * B + sl * WAVE + lane;
-        if (vj >= n) return;
-        const int buf = sl & 1;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            float sum = row[buf][0][k][lane];
-#pragma unroll
-            for (int w = 1; w < W; ++w) sum += row[buf][w][k][lane];
-            s.partial[(size_t(t.x) * 3 + k) * size_t(n) + vj] = sum;
-        }
-    };
-    if (wave == 0) {
-        float4 q;
-        float r;
-        load_visitor(0, q, r);
-        vis[0][lane] = q;
-        vr2[0][lane] = r;
-    }
-    for (int sl = 0; sl < slices; ++sl) {
-        const int buf = sl & 1;
-        __syncthreads();  // slice sl's visitors are in; every wave's rows of slice sl - 1 too
-        if (!diag && sl > 0) fold(sl - 1);
-        float4 qn = make_float4(0.f, 0.f, 0.f, 0.f);
-        float rn = 0.f;
-        if (wave == 0 && sl + 1 < slices) load_visitor(sl + 1, qn, rn);  // in flight during the slice
-        float* rw = &row[buf][wave][0][0];
-        // Step k: lane l meets visitor v = (l - k) mod 64.  Its row slot was written by lane
-        // l - 1 in step k - 1 (at step 0 each lane starts its own slot); the compiler sees no
-        // dependence across lanes, so a compiler-only fence keeps each step's row accesses
-        // after the previous step's, and LDS runs a wave's accesses in program order.  The
-        // next visitor and this step's row sums are read before the arithmetic.
-        int v = lane;
-        float4 q = vis[buf][v];
-        float rv2 = CUT ? vr2[buf][v] : 0.f;
-#pragma unroll 2
-        for (int k = 0; k < WAVE; ++k) {
-            const int vn = (lane - k - 1) & (WAVE - 1);
-            const float4 qn_step = vis[buf][vn];
-            const float rn_step = CUT ? vr2[buf][vn] : 0.f;
-            float ox = 0.f, oy = 0.f, oz = 0.f;
-            if (!diag && k > 0) {
-                ox = rw[0 * WAVE + v];
-                oy = rw[1 * WAVE + v];
-                oz = rw[2 * WAVE + v];
-            }
-            float vx = 0.f, vy = 0.f, vz = 0.f;
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                const float dx = q.x - x[p].x;
-                const float dy = q.y - x[p].y;
-                const float dz = q.z - x[p].z;
-                float wh, wv;
-                if (CUT) {
-                    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    const float inv = __builtin_amdgcn_rsqf(d2 + s.eps2);
-                    const float i3 = inv * inv * inv;
-                    wh = d2 > r2[p] ? q.w * i3 : 0.f;
-                    wv = d2 > rv2 ? x[p].w * i3 : 0.f;
-                } else {
-                    const float e2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, s.eps2)));
-                    const float inv = __builtin_amdgcn_rsqf(e2);
-                    const float i3 = inv * inv * inv;
-                    wh = q.w * i3;
-                    wv = x[p].w * i3;
-                }
-                T[p][0] = fmaf(wh, dx, T[p][0]);
-                T[p][1] = fmaf(wh, dy, T[p][1]);
-                T[p][2] = fmaf(wh, dz, T[p][2]);
-                vx = fmaf(-wv, dx, vx);
-                vy = fmaf(-wv, dy, vy);
-                vz = fmaf(-wv, dz, vz);
-            }
-            if (!diag) {
-                rw[0 * WAVE + v] = ox + vx;
-                rw[1 * WAVE + v] = oy + vy;
-                rw[2 * WAVE + v] = oz + vz;
-            }
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            v = vn;
-            q = qn_step;
-            rv2 = rn_step;
-        }
-#pragma unroll
-        for (int p = 0; p < P; ++p)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                two_sum(S[p][k], C[p][k], T[p][k]);
-                T[p][k] = 0.f;
-            }
-        if (wave == 0 && sl + 1 < slices) {
-            vis[buf ^ 1][lane] = qn;  // last read in slice sl - 1, before this slice's barrier
-            vr2[buf ^ 1][lane] = rn;
-        }
-    }
-    if (!diag) {
-        __syncthreads();
-        fold(slices - 1);
-    }
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        const int h = t.x * B + (wave * P + p) * WAVE + lane;
-        if (h >= n) continue;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) s.partial[(size_t(t.y) * 3 + k) * size_t(n) + h] = S[p][k] + C[p][k];
-    }
+    sym_tile<P, W, CUT>(sh, s.pm, s.vr, s.n, s.eps2, t.x, t.y, s.partial);
 }
 
 // Adds body i's M partial sums in slot order (compensated), eight loads in flight at a
@@ -255,22 +99,10 @@ __global__ __launch_bounds__(256) void k_reduce_slots(ForceArgs a, int slots)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= a.n) return;
-    const size_t n = a.n;
-    float s[3] = { 0.f, 0.f, 0.f }, c[3] = { 0.f, 0.f, 0.f };
     const float4 xi = a.pm[i];
-    for (int y0 = 0; y0 < slots; y0 += 8) {
-        float v[8][3];
-#pragma unroll
-        for (int y = 0; y < 8; ++y)
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-                v[y][k] = y0 + y < slots ? a.partial[(size_t(y0 + y) * 3 + k) * n + i] : 0.f;
-#pragma unroll
-        for (int y = 0; y < 8; ++y)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) two_sum(s[k], c[k], v[y][k]);
-    }
-    finish_body(a, i, xi, a.G * (s[0] + c[0]), a.G * (s[1] + c[1]), a.G * (s[2] + c[2]));
+    float tot[3];
+    sym_slot_sum(a.partial, slots, size_t(a.n), i, tot);
+    finish_body(a, i, xi, a.G * tot[0], a.G * tot[1], a.G * tot[2]);
 }
 
 // Combine j-split partials in a fixed order, then finish as OUT_ACC / OUT_STEP.  MAXJ >=

@@ -1,5 +1,6 @@
-// force_tile.hpp -- the direct-summation tile shared by the single-system pass
-// (direct.hip) and the batched pass (batch.hip), plus the kick-drift step.
+// force_tile.hpp -- the direct-summation tiles shared by the single-system pass
+// (direct.hip) and the batched pass (batch.hip): the one-sided tile (force_tile), the
+// pair-symmetric tile (sym_tile, at the end of this file), and the kick-drift step.
 //
 // Force law (reference source/detail/physics.h:21-42, GLSL shaders/include/common.glsl:41-56,
 // with a softening extension; eps = 0 is the reference law):
@@ -183,6 +184,197 @@ __device__ __forceinline__ void force_tile(ForceShared<P, W>& sh, const float4* 
         }
         epi(p, i, xi[p], tot[0], tot[1], tot[2]);
     }
+}
+
+
+// ---- the pair-symmetric tile ------------------------------------------------------------
+// Bodies fall into blocks of B = 64*P*W; tile (a, b), a <= b, is one workgroup.  Its W waves
+// hold block a's bodies ("homes", P per lane, in registers); block b's bodies ("visitors")
+// pass through LDS 64 at a time.  Within such a slice, at step k lane l meets visitor
+// (l - k) mod 64, so over 64 steps every lane meets every visitor once, and no two lanes of
+// a wave meet the same visitor in the same step.  Each pair is evaluated once: one
+// |d|^2 + eps^2, one v_rsq_f32, and the force goes both ways -- onto the home (its
+// registers) and, with the opposite sign, onto the visitor (the wave's LDS row for that
+// visitor, a plain read-modify-write: one lane per slot per step, in step order).  The W
+// waves' rows for a slice are then added in wave order.  A diagonal tile (a == b) takes
+// the home side only, every pair within the block met from both ends.
+// Every body thus receives one partial sum per other block (slot b for its home side in
+// tile (block, b), slot a for its visitor side in tile (a, block)), written exactly once
+// into partial[slot][3][n]; sym_slot_sum adds a body's M slots in slot order.  Each cutoff
+// applies to its receiving body (home: its radius; visitor: its radius).
+template <int W>
+struct SymShared {
+    float4 vis[2][WAVE];       // visitors of the slice in use and of the next one
+    float vr2[2][WAVE];        // their squared radii (cutoff variant)
+    float row[2][W][3][WAVE];  // each wave's visitor sums: the slice in use, and the one before
+};
+
+// Tile (ta, tb) of one system: bodies pm[0, n), blocks of 64*P*W.  Must be called by every
+// thread of the workgroup (it synchronises).
+template <int P, int W, bool CUT>
+__device__ __forceinline__ void sym_tile(SymShared<W>& sh, const float4* __restrict__ pm,
+                                         const float4* __restrict__ vr, int n, float eps2, int ta, int tb,
+                                         float* __restrict__ partial)
+{
+    constexpr int B = WAVE * P * W;
+    constexpr int SLICES = P * W;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / WAVE);
+    const bool diag = ta == tb;
+    const int slices = min(SLICES, (n - tb * B + WAVE - 1) / WAVE);  // block b may be short
+
+    float4 x[P];
+    float r2[P];
+    float S[P][3], C[P][3], T[P][3];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int h = ta * B + (wave * P + p) * WAVE + lane;
+        x[p] = h < n ? pm[h] : make_float4(0.f, 0.f, 0.f, 0.f);  // padding: mass 0, adds 0 both ways
+        if (CUT) {
+            const float r = h < n ? vr[h].w : 0.f;
+            r2[p] = r * r;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) S[p][k] = C[p][k] = T[p][k] = 0.f;
+    }
+    const auto load_visitor = [&](int sl, float4& q, float& r) {
+        const int vj = tb * B + sl * WAVE + lane;
+        q = vj < n ? pm[vj] : make_float4(0.f, 0.f, 0.f, 0.f);
+        r = 0.f;
+        if (CUT && vj < n) {
+            const float rad = vr[vj].w;
+            r = rad * rad;
+        }
+    };
+    // wave `sl % W` adds the W rows of slice sl in wave order and writes its visitors' slot
+    const auto fold = [&](int sl) {
+        if (wave != sl % W) return;
+        const int vj = tb * B + sl * WAVE + lane;
+        if (vj >= n) return;
+        const int buf = sl & 1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float sum = sh.row[buf][0][k][lane];
+#pragma unroll
+            for (int w = 1; w < W; ++w) sum += sh.row[buf][w][k][lane];
+            partial[(size_t(ta) * 3 + k) * size_t(n) + vj] = sum;
+        }
+    };
+    if (wave == 0) {
+        float4 q;
+        float r;
+        load_visitor(0, q, r);
+        sh.vis[0][lane] = q;
+        sh.vr2[0][lane] = r;
+    }
+    for (int sl = 0; sl < slices; ++sl) {
+        const int buf = sl & 1;
+        __syncthreads();  // slice sl's visitors are in; every wave's rows of slice sl - 1 too
+        if (!diag && sl > 0) fold(sl - 1);
+        float4 qn = make_float4(0.f, 0.f, 0.f, 0.f);
+        float rn = 0.f;
+        if (wave == 0 && sl + 1 < slices) load_visitor(sl + 1, qn, rn);  // in flight during the slice
+        float* rw = &sh.row[buf][wave][0][0];
+        // Step k: lane l meets visitor v = (l - k) mod 64.  Its row slot was written by lane
+        // l - 1 in step k - 1 (at step 0 each lane starts its own slot); the compiler sees no
+        // dependence across lanes, so a compiler-only fence keeps each step's row accesses
+        // after the previous step's, and LDS runs a wave's accesses in program order.  The
+        // next visitor and this step's row sums are read before the arithmetic.
+        int v = lane;
+        float4 q = sh.vis[buf][v];
+        float rv2 = CUT ? sh.vr2[buf][v] : 0.f;
+#pragma unroll 2
+        for (int k = 0; k < WAVE; ++k) {
+            const int vn = (lane - k - 1) & (WAVE - 1);
+            const float4 qn_step = sh.vis[buf][vn];
+            const float rn_step = CUT ? sh.vr2[buf][vn] : 0.f;
+            float ox = 0.f, oy = 0.f, oz = 0.f;
+            if (!diag && k > 0) {
+                ox = rw[0 * WAVE + v];
+                oy = rw[1 * WAVE + v];
+                oz = rw[2 * WAVE + v];
+            }
+            float vx = 0.f, vy = 0.f, vz = 0.f;
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const float dx = q.x - x[p].x;
+                const float dy = q.y - x[p].y;
+                const float dz = q.z - x[p].z;
+                float wh, wv;
+                if (CUT) {
+                    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
+                    const float i3 = inv * inv * inv;
+                    wh = d2 > r2[p] ? q.w * i3 : 0.f;
+                    wv = d2 > rv2 ? x[p].w * i3 : 0.f;
+                } else {
+                    const float e2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+                    const float inv = __builtin_amdgcn_rsqf(e2);
+                    const float i3 = inv * inv * inv;
+                    wh = q.w * i3;
+                    wv = x[p].w * i3;
+                }
+                T[p][0] = fmaf(wh, dx, T[p][0]);
+                T[p][1] = fmaf(wh, dy, T[p][1]);
+                T[p][2] = fmaf(wh, dz, T[p][2]);
+                vx = fmaf(-wv, dx, vx);
+                vy = fmaf(-wv, dy, vy);
+                vz = fmaf(-wv, dz, vz);
+            }
+            if (!diag) {
+                rw[0 * WAVE + v] = ox + vx;
+                rw[1 * WAVE + v] = oy + vy;
+                rw[2 * WAVE + v] = oz + vz;
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            v = vn;
+            q = qn_step;
+            rv2 = rn_step;
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                two_sum(S[p][k], C[p][k], T[p][k]);
+                T[p][k] = 0.f;
+            }
+        if (wave == 0 && sl + 1 < slices) {
+            sh.vis[buf ^ 1][lane] = qn;  // last read in slice sl - 1, before this slice's barrier
+            sh.vr2[buf ^ 1][lane] = rn;
+        }
+    }
+    if (!diag) {
+        __syncthreads();
+        fold(slices - 1);
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int h = ta * B + (wave * P + p) * WAVE + lane;
+        if (h >= n) continue;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) partial[(size_t(tb) * 3 + k) * size_t(n) + h] = S[p][k] + C[p][k];
+    }
+}
+
+// Body i's M partial sums in slot order, compensated, eight loads in flight at a time.
+__device__ __forceinline__ void sym_slot_sum(const float* __restrict__ partial, int slots, size_t n, int i,
+                                             float out[3])
+{
+    float s[3] = { 0.f, 0.f, 0.f }, c[3] = { 0.f, 0.f, 0.f };
+    for (int y0 = 0; y0 < slots; y0 += 8) {
+        float v[8][3];
+#pragma unroll
+        for (int y = 0; y < 8; ++y)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                v[y][k] = y0 + y < slots ? partial[(size_t(y0 + y) * 3 + k) * n + i] : 0.f;
+#pragma unroll
+        for (int y = 0; y < 8; ++y)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) two_sum(s[k], c[k], v[y][k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) out[k] = s[k] + c[k];
 }
 
 }  // namespace nbhip

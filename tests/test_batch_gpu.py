@@ -132,7 +132,7 @@ def test_tuned_batch_matches_f64(ctx, orc):
         ids = [batch.add(b, p) for b, p in sy]
         ms = batch.tune()
         info = batch.info()
-        print(f"tuned: P={info.P} W={info.W} persistent={info.persistent} {ms:.3f} ms/step")
+        print(f"tuned: P={info.P} W={info.W} launch={info.launch} {ms:.3f} ms/step")
         assert info.systems == len(sy) and info.interactions_per_step == sum(float(b.shape[0]) ** 2 for b, _ in sy)
         batch.step(1, _lib.BATCH_ACCELERATE)
         for (b, _), i in zip(sy[:6], ids[:6]):
@@ -211,9 +211,64 @@ def test_c4_workload_50_step_graph(ctx, orc):
         assert e.max() <= 1e-5, (k, g.shape[0], e.max())
     for k in alone_pick:
         with _lib.Batch(ctx) as one:
-            one.configure(info.P, info.W, bool(info.persistent))
+            one.configure(info.P, info.W, info.launch)
             i = one.add(systems[k], p)
             one.step(50, _lib.BATCH_GRAPH)
             one.step(1, _lib.BATCH_ACCELERATE)
             solo = one.download(i)
         assert np.array_equal(bits(solo), bits(got[k])), (k, systems[k].shape[0])
+
+
+@pytest.mark.parametrize("P", [2, 4, 8])
+def test_pairs_mode_equals_symmetric_pass_alone(ctx, monkeypatch, P):
+    # pairs mode runs each system through the pair-symmetric tile with blocks of 64*P*4:
+    # bit-identical to the single-system pair-symmetric pass of the same shape, fused steps
+    # and radius cutoffs included, and the same alone and in a mix
+    sy = systems() + [(ics.plummer(9000, seed=77), _lib.params(dt=1e-3, eps=0.01))]
+    with _lib.Batch(ctx) as batch:
+        batch.configure(P, 4, _lib.BATCH_LAUNCH_PAIRS)
+        ids = [batch.add(b, p) for b, p in sy]
+        assert batch.info().launch == _lib.BATCH_LAUNCH_PAIRS
+        batch.step(3)
+        got = [batch.download(i) for i in ids]
+    monkeypatch.setenv("NBHIP_DIRECT_SYM", "1")
+    monkeypatch.setenv("NBHIP_DIRECT_P", str(P))
+    monkeypatch.setenv("NBHIP_DIRECT_W", "4")
+    for (b, p), g in zip(sy, got):
+        ctx.upload_bodies(b)
+        ctx.step(p, 3)
+        ref = ctx.download_bodies()
+        assert np.array_equal(bits(g), bits(ref)), b.shape[0]
+    with _lib.Batch(ctx) as one:
+        one.configure(P, 4, _lib.BATCH_LAUNCH_PAIRS)
+        i = one.add(*sy[-1])
+        one.step(3)
+        assert np.array_equal(bits(one.download(i)), bits(got[-1]))
+
+
+def test_pairs_mode_graph_and_f64(ctx, orc):
+    sy = systems()[1:]
+    res = []
+    for flags in (0, _lib.BATCH_GRAPH):
+        with _lib.Batch(ctx) as batch:
+            batch.configure(4, 4, _lib.BATCH_LAUNCH_PAIRS)
+            ids = [batch.add(b, p) for b, p in sy]
+            for _ in range(3):
+                batch.step(4, flags)
+                _ = batch.download(ids[1])
+                ctx.synchronize()
+            batch.step(1, _lib.BATCH_ACCELERATE)
+            res.append([batch.download(i) for i in ids])
+    for a, g in zip(*res):
+        assert np.array_equal(bits(a), bits(g))
+    for (b, p), g in zip(sy, res[0]):
+        a64 = orc.direct_f64(g, G=float(p.G), eps=float(p.eps))
+        assert rel_err(g[:, 8:11], a64).max() <= 1e-5
+
+
+def test_rejects_bad_launch_modes(ctx):
+    with _lib.Batch(ctx) as batch:
+        with pytest.raises(_lib.HipError):
+            batch.configure(2, 4, 3)
+        with pytest.raises(_lib.HipError):
+            batch.configure(1, 8, _lib.BATCH_LAUNCH_PAIRS)   # no pairs kernel for this shape
