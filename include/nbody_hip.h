@@ -71,6 +71,11 @@ typedef struct nbhip_ctx nbhip_ctx;
  *              cutoff, source/detail/physics.h:28-41).  eps = 0 is the reference law.
  *   opening    NBHIP_OPEN_REFERENCE or NBHIP_OPEN_TEXTBOOK (NLogN only).
  *   leaf_size  maximum bodies per Barnes-Hut leaf; 1 reproduces the reference tree.
+ * In NLogN mode `size` is also the edge of the octree's root cube, centred at the origin
+ * (the reference's State::size, source/detail/tree.h:20).  size == 0 there asks for the
+ * root to be found from the bodies on the device: the smallest power of two at least twice
+ * the largest |x|, |y| or |z| (an FP64 reduction; 1 when every body is at the origin), so
+ * the cube holds every body; torus wrap needs size > 0 and is then off.
  */
 typedef struct nbhip_params {
     float dt;
@@ -265,6 +270,11 @@ int nbhip_bh_configure(nbhip_ctx* ctx, int walk_split);
  * stream around each step's tree build and its traversal; *build_ms and *walk_ms receive
  * the means.  Synchronises. */
 int nbhip_bh_profile(nbhip_ctx* ctx, const nbhip_params* p, int steps, double* build_ms, double* walk_ms);
+
+/* The root-cube edge of this context's last Barnes-Hut build: the params' size, or the one
+ * found from the bodies when that was 0.  NBHIP_ERR_INVALID when there is no build for the
+ * current body count.  Synchronises. */
+int nbhip_bh_root(nbhip_ctx* ctx, float* size);
 
 /* ---- diagnostics (FP64 reductions on the device) --------------------------------- */
 

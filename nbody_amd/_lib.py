@@ -125,6 +125,7 @@ SIGNATURES = {
     "nbhip_tune": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, C.POINTER(_sz), C.POINTER(TuneEntry)]),
     "nbhip_bh_tune": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, _vp, C.POINTER(C.c_int)]),
     "nbhip_bh_configure": (C.c_int, [_vp, C.c_int]),
+    "nbhip_bh_root": (C.c_int, [_vp, C.POINTER(C.c_float)]),
     "nbhip_bh_profile": (C.c_int, [_vp, C.POINTER(Params), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "nbhip_diagnostics": (C.c_int, [_vp, C.POINTER(Params), _vp]),
     "nbhip_force_error_f64": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, _vp, _vp]),
@@ -369,6 +370,13 @@ class Context:
         b, w = C.c_double(), C.c_double()
         self._check(self.L.nbhip_bh_profile(self.h, C.byref(p), steps, C.byref(b), C.byref(w)))
         return b.value, w.value
+
+    def bh_root(self) -> float:
+        """The root-cube edge of the last Barnes-Hut build (found from the bodies when the
+        params' size was 0)."""
+        v = C.c_float()
+        self._check(self.L.nbhip_bh_root(self.h, C.byref(v)))
+        return v.value
 
     def bh_configure(self, walk_split: int = 0) -> None:
         """Waves per 64-body group in the Barnes-Hut walk: 1, 2, 4, 8, or 0 for by body count."""

@@ -79,14 +79,15 @@ __device__ __forceinline__ Dbl4 operator+(const Dbl4& a, const Dbl4& b)
 }
 
 // ---- 1. keys ------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_keys(const float4* pm, int n, float size,
+// The root edge is `size`, or *size_dev when that is given (the edge found on the device).
+__global__ __launch_bounds__(256) void k_keys(const float4* pm, int n, float size, const float* size_dev,
                                               unsigned long long* keys, unsigned* vals)
 {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const float4 p = pm[i];
-    float cx = 0.f, cy = 0.f, cz = 0.f, s = size;
+    float cx = 0.f, cy = 0.f, cz = 0.f, s = size_dev ? *size_dev : size;
     unsigned long long key = 0;
     for (int l = 0; l < LEVELS; ++l) {
         const unsigned q = unsigned(p.x < cx) | (unsigned(p.y < cy) << 1) | (unsigned(p.z < cz) << 2);
@@ -99,6 +100,50 @@ __global__ __launch_bounds__(256) void k_keys(const float4* pm, int n, float siz
     }
     keys[i] = key;
     vals[i] = unsigned(i);
+}
+
+// ---- the root cube found from the bodies (params.size == 0) --------------------------------
+// FP64 maximum of |x|, |y|, |z| over the bodies (exact: a maximum does not round), then the
+// smallest power of two at least twice it: the reference's origin-centred root cube, just
+// large enough to hold every body.  Per-block maxima, then one block.
+constexpr int ROOT_BLOCKS_MAX = 1024;
+
+__global__ __launch_bounds__(256) void k_root_extent(const float4* pm, int n, double* part)
+{
+    __shared__ double sh[256];
+    double m = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const float4 p = pm[i];
+        m = fmax(m, fmax(fabs(double(p.x)), fmax(fabs(double(p.y)), fabs(double(p.z)))));
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_root_size(const double* part, int nb, float* root)
+{
+    __shared__ double sh[256];
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) m = fmax(m, part[b]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double t = 2.0 * sh[0];
+        int e = 0;
+        const double f = t > 0.0 ? frexp(t, &e) : 0.0;  // t = f 2^e, f in [0.5, 1)
+        double size = t > 0.0 ? ldexp(1.0, f == 0.5 ? e - 1 : e) : 1.0;
+        if (!(size <= 0x1p127)) size = 0x1p127;  // beyond FP32 range, or NaN-only positions
+        *root = float(size);
+    }
 }
 
 // ---- scans (two phases: tile totals, then tile scans that add the totals before them) -----
@@ -1095,6 +1140,7 @@ struct EmitArgs {
     const Dbl4* pref;      // n + 1 entries
     const float4* spm;
     float size;
+    const float* size_dev; // the root edge found on the device, or null: `size`
     float theta;
     int opening;
     BhNode* nodes;
@@ -1129,7 +1175,7 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
                                 float(M));
         else
             nd.cm = make_float4(0.f, 0.f, 0.f, 0.f);
-        float s = e.size;  // the cell edge at this level, halved as the reference does
+        float s = e.size_dev ? *e.size_dev : e.size;  // the cell edge at this level, halved as the reference does
         for (int l = 0; l < lvl; ++l) s = s * .5f;
         const float s2 = s * s;
         nd.thr = e.opening == NBHIP_OPEN_TEXTBOOK ? s2 / theta_sq : s2 * theta_sq;
@@ -1636,6 +1682,24 @@ void bh_reserve(nbhip_ctx* c)
     rs_reserve(c, n);
     c->bh_uscan.grow(std::max(size_t(grid(n)), size_t(std::max(1, (n + SC_TILE - 1) / SC_TILE))), 0, s);
     c->bh_dscan.grow(size_t(std::max(1, (n + SC_TILE - 1) / SC_TILE)), 0, s);
+    c->bh_root.grow(1, 0, s);
+    c->bh_root_part.grow(ROOT_BLOCKS_MAX, 0, s);
+}
+
+// The root edge of a build: p.size, or (p.size == 0) found from the bodies on the device;
+// returns the device word holding it in that case, else null.
+const float* bh_root(nbhip_ctx* c, const nbhip_params& p, int n)
+{
+    if (p.size > 0.f) return nullptr;
+    hipStream_t s = c->stream;
+    c->bh_root.grow(1, 0, s);
+    c->bh_root_part.grow(ROOT_BLOCKS_MAX, 0, s);
+    const int nb = std::max(1, std::min(ROOT_BLOCKS_MAX, int(grid(n))));
+    hipLaunchKernelGGL(k_root_extent, dim3(nb), dim3(256), 0, s, c->pm[c->cur].ptr, n, c->bh_root_part.ptr);
+    NBHIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_root_size, dim3(1), dim3(256), 0, s, c->bh_root_part.ptr, nb, c->bh_root.ptr);
+    NBHIP_CHECK(hipGetLastError());
+    return c->bh_root.ptr;
 }
 
 void bh_check_fault(nbhip_ctx* c)
@@ -1679,7 +1743,8 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     c->bh_kept.grow(n, 0, s);
     c->bh_top.grow(8, 0, s);
 
-    hipLaunchKernelGGL(k_keys, dim3(grid(n)), dim3(256), 0, s, c->pm[c->cur].ptr, n, p.size,
+    const float* root = bh_root(c, p, n);
+    hipLaunchKernelGGL(k_keys, dim3(grid(n)), dim3(256), 0, s, c->pm[c->cur].ptr, n, p.size, root,
                        c->bh_keys[0].ptr, c->bh_vals[0].ptr);
     NBHIP_CHECK(hipGetLastError());
     const int sorted = radix_sort(c, n, 0);
@@ -1733,12 +1798,12 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
                                        SC_TILE / 256);
     }
 
-    EmitArgs e{ keys, n, leaf, c->bh_base.ptr, c->bh_kept.ptr, c->bh_pref.ptr, c->bh_spm.ptr, p.size,
+    EmitArgs e{ keys, n, leaf, c->bh_base.ptr, c->bh_kept.ptr, c->bh_pref.ptr, c->bh_spm.ptr, p.size, root,
                 p.theta, p.opening, c->bh_nodes.ptr, unsigned(c->bh_nodes.cap), dv, ns, vc, c->bh_top.ptr };
     hipLaunchKernelGGL(k_emit, dim3(grid(n)), dim3(256), 0, s, e);
     NBHIP_CHECK(hipGetLastError());
     c->bh_built = true;
-    c->bh_size = p.size;
+    c->bh_size = p.size;  // 0: the edge sits in bh_root on the device
     ++c->bh_builds;
 }
 
@@ -1848,7 +1913,8 @@ void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm
         c->bh_keys[i].grow(n, 0, s);
         c->bh_vals[i].grow(n, 0, s);
     }
-    hipLaunchKernelGGL(k_keys, dim3(grid(n)), dim3(256), 0, s, c->pm[c->cur].ptr, n, p.size,
+    const float* root = bh_root(c, p, n);
+    hipLaunchKernelGGL(k_keys, dim3(grid(n)), dim3(256), 0, s, c->pm[c->cur].ptr, n, p.size, root,
                        c->bh_keys[0].ptr, c->bh_vals[0].ptr);
     NBHIP_CHECK(hipGetLastError());
     if (keys) {
@@ -1886,13 +1952,29 @@ size_t bh_tree(nbhip_ctx* c, const nbhip_params* p)
     const size_t total = 1 + 8 * size_t(internal);
     if (total > 0xffffffffull) fail(NBHIP_ERR_UNSUPPORTED, "tree of %zu nodes exceeds 32-bit indices", total);
     c->bh_tree.grow(total, 0, s);
-    TreeArgs t{ keys, n, c->bh_icount.ptr, c->bh_pref.ptr, c->bh_spm.ptr, c->bh_size, c->bh_tree.ptr,
+    float root = c->bh_size;
+    if (!(root > 0.f)) {
+        NBHIP_CHECK(hipMemcpyAsync(&root, c->bh_root.ptr, sizeof root, hipMemcpyDeviceToHost, s));
+        NBHIP_CHECK(hipStreamSynchronize(s));
+    }
+    TreeArgs t{ keys, n, c->bh_icount.ptr, c->bh_pref.ptr, c->bh_spm.ptr, root, c->bh_tree.ptr,
                 unsigned(c->bh_tree.cap < 0xffffffffull ? c->bh_tree.cap : 0xffffffffull) };
     hipLaunchKernelGGL(k_tree_emit, dim3(grid(n)), dim3(256), 0, s, t);
     NBHIP_CHECK(hipGetLastError());
     c->bh_tree_total = total;
     c->bh_tree_build = c->bh_builds;
     return total;
+}
+
+float bh_root_size(nbhip_ctx* c)
+{
+    if (!c->bh_built) fail(NBHIP_ERR_INVALID, "no Barnes-Hut tree has been built for the current bodies");
+    float root = c->bh_size;
+    if (!(root > 0.f)) {
+        NBHIP_CHECK(hipMemcpyAsync(&root, c->bh_root.ptr, sizeof root, hipMemcpyDeviceToHost, c->stream));
+        NBHIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    return root;
 }
 
 void bh_release(nbhip_ctx* c)
@@ -1916,6 +1998,8 @@ void bh_release(nbhip_ctx* c)
     c->bh_tree.release();
     c->bh_levels.release();
     c->bh_rs_state.release();
+    c->bh_root.release();
+    c->bh_root_part.release();
     c->bh_built = false;
     c->bh_tree_build = ~0ull;
 }

@@ -175,6 +175,8 @@ struct nbhip_ctx {
     nbhip::DevBuf<unsigned> bh_uscan;
     nbhip::DevBuf<nbhip::Dbl4> bh_dscan;
     nbhip::DevBuf<unsigned> bh_visits;
+    nbhip::DevBuf<float> bh_root;         // the root edge found from the bodies (params.size == 0)
+    nbhip::DevBuf<double> bh_root_part;   // its per-block FP64 maxima
     int bh_sorted = 0;          // which keys/vals buffer holds the sorted result
     int bh_split = 0;           // waves per 64-body group in the walk (0: by body count)
     bool bh_built = false;      // a tree of the current count exists on the device
@@ -251,6 +253,7 @@ void bh_check_fault(nbhip_ctx* c);
 // Sets the fault word from the host, as a look-back that gave up waiting would (tests only).
 void bh_raise_sort_fault(nbhip_ctx* c);
 size_t bh_tree(nbhip_ctx* c, const nbhip_params* p);
+float bh_root_size(nbhip_ctx* c);
 void check_ics(const nbhip_ics* s, size_t n);
 bool ics_have_radius(const nbhip_ics& s);
 void generate(nbhip_ctx* c, const nbhip_ics& s, size_t n, float4* pm, float4* vr, float4* acc);

@@ -72,7 +72,7 @@ void check_params(const nbhip_params* p)
         if (p->opening != NBHIP_OPEN_REFERENCE && p->opening != NBHIP_OPEN_TEXTBOOK)
             fail(NBHIP_ERR_INVALID, "unknown opening criterion %d", p->opening);
         if (p->leaf_size < 1) fail(NBHIP_ERR_INVALID, "leaf_size must be >= 1");
-        if (!(p->size > 0.f)) fail(NBHIP_ERR_INVALID, "NLogN needs a positive root size");
+        if (!(p->size >= 0.f)) fail(NBHIP_ERR_INVALID, "NLogN needs a root size >= 0 (0: found from the bodies)");
         if (!(p->theta >= 0.f)) fail(NBHIP_ERR_INVALID, "theta must be >= 0");
     }
 }
@@ -638,7 +638,7 @@ int nbhip_bh_tree(nbhip_ctx* c, const nbhip_params* p, nbhip_tree_node* nodes, s
         check_ctx(c);
         if (p) {
             check_params(p);
-            if (!(p->size > 0.f)) fail(NBHIP_ERR_INVALID, "the tree needs a positive root size");
+            if (!(p->size >= 0.f)) fail(NBHIP_ERR_INVALID, "the tree needs a root size >= 0");
         }
         if (count) *count = 0;
         if (c->count == 0) {
@@ -647,7 +647,8 @@ int nbhip_bh_tree(nbhip_ctx* c, const nbhip_params* p, nbhip_tree_node* nodes, s
             if (count) *count = 1;
             if (cap >= 1) {
                 if (!nodes) fail(NBHIP_ERR_INVALID, "null nodes");
-                nodes[0] = nbhip_tree_node{ { 0.f, 0.f, 0.f }, p->size, { 0.f, 0.f, 0.f }, 0.f, 0u, 0u, 0u, 0u };
+                const float edge = p->size > 0.f ? p->size : 1.f;  // a root found from no bodies: edge 1
+                nodes[0] = nbhip_tree_node{ { 0.f, 0.f, 0.f }, edge, { 0.f, 0.f, 0.f }, 0.f, 0u, 0u, 0u, 0u };
             }
             return;
         }
@@ -700,6 +701,16 @@ int nbhip_bh_profile(nbhip_ctx* c, const nbhip_params* p, int steps, double* bui
         NBHIP_CHECK(hipSetDevice(c->device));
         nbhip::bh_profile_steps(c, *p, steps, build_ms, walk_ms);
         nbhip::bh_check_fault(c);
+    });
+}
+
+int nbhip_bh_root(nbhip_ctx* c, float* size)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        if (!size) fail(NBHIP_ERR_INVALID, "null output");
+        NBHIP_CHECK(hipSetDevice(c->device));
+        *size = nbhip::bh_root_size(c);
     });
 }
 

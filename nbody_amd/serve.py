@@ -335,7 +335,7 @@ def run_sequential(jobs: list[Job], clock=time.perf_counter) -> Report:
 def make_jobs(count: int, seed: int, n_range=(1024, 262144), steps_range=(10, 1000),
               direct_below: int = 16384, eps: float = 0.01, dt: float = 1e-3,
               theta: float = 0.5, leaf_size: int = 16, size: float = 128.0,
-              device_ics: bool = False) -> list[Job]:
+              device_ics: bool = False, bh_auto_root: bool = True) -> list[Job]:
     """The served workload: Plummer or disk+bulge systems, N log-uniform, steps uniform;
     direct sum below `direct_below` bodies, Barnes-Hut above.  Bodies are host arrays
     (numpy's generators, nbody_amd.ics) or, with device_ics, generator specs drawn on the
@@ -357,8 +357,10 @@ def make_jobs(count: int, seed: int, n_range=(1024, 262144), steps_range=(10, 10
         if n < direct_below:
             p = _lib.params(dt=dt, eps=eps, size=size, wrap=1)
         else:
-            p = _lib.params(mode=_lib.MODE_NLOGN, dt=dt, eps=eps, size=size, wrap=1, theta=theta,
-                            leaf_size=leaf_size)
+            # the octree's root cube is found from the job's own bodies each step (size 0: no
+            # torus wrap), so a system that spreads out is never clamped into edge octants
+            p = _lib.params(mode=_lib.MODE_NLOGN, dt=dt, eps=eps, size=0.0 if bh_auto_root else size,
+                            wrap=1, theta=theta, leaf_size=leaf_size)
         out.append(Job(bodies=bodies, steps=steps, params=p, tag="disk" if disk else "plummer",
                        ics=spec, n_bodies=n))
     return out

@@ -612,3 +612,52 @@ def test_profile_steps_equal_plain_steps(ctx):
     assert build_ms > 0 and walk_ms > 0
     with pytest.raises(_lib.HipError):
         ctx.bh_profile(_lib.params(size=128.0), 1)   # direct-sum params
+
+
+def auto_root(b):
+    # the root cube found from the bodies: smallest power of two >= 2 max |coordinate|
+    m = float(np.abs(b[:, 0:3].astype(np.float64)).max())
+    return 1.0 if m == 0.0 else float(2.0 ** np.ceil(np.log2(2.0 * m)))
+
+
+@pytest.mark.parametrize("scale", [1.0, 37.5, 1e-3])
+def test_root_found_from_the_bodies(ctx, orc, scale):
+    # size 0 in NLogN mode: the root edge comes from an FP64 reduction on the device; keys,
+    # permutation and forces equal those of an explicit root of that edge, bit for bit,
+    # and the forces match the bucket oracle there
+    b = ics.disk_bulge(30000, seed=int(scale * 10) + 3)
+    b[:, 0:3] *= np.float32(scale)
+    b[5, 0:3] = [0.0, -np.float32(24.0 * scale), 0.0]   # a far body sets the extent along -y
+    edge = auto_root(b)
+    ctx.upload_bodies(b)
+    p0 = bh_params(size=0.0, theta=0.5, eps=0.01 * scale, leaf_size=16)
+    pe = bh_params(size=edge, theta=0.5, eps=0.01 * scale, leaf_size=16)
+    k0, perm0 = ctx.bh_keys(p0)
+    ke, perme = ctx.bh_keys(pe)
+    assert np.array_equal(k0, ke) and np.array_equal(perm0, perme)
+    assert np.array_equal(k0, orc.morton_keys(b, edge, 21))
+    ctx.accelerate(p0)
+    assert ctx.bh_root() == edge
+    a0 = ctx.download_bodies()
+    tree0 = ctx.bh_tree()
+    ctx.accelerate(pe)
+    ae = ctx.download_bodies()
+    assert np.array_equal(a0.view(np.uint32), ae.view(np.uint32))
+    assert tree0["size"][0] == edge and np.array_equal(tree0, ctx.bh_tree())
+    x = b.copy()
+    _, xv = orc.bh_accelerate_buckets(x, edge, theta=0.5, leaf_size=16, eps=0.01 * scale)
+    visits, _ = ctx.bh_interactions(p0)
+    eq = visits == xv
+    e = rel_err(a0[:, 8:11], x[:, 8:11])
+    assert eq.mean() >= 0.9999 and np.quantile(e[eq], 0.999) <= 1e-5 and e[eq].max() <= 1e-4
+    # fused steps with size 0 find the root again each step and never wrap: the same as an
+    # explicit root of that edge without wrap, while the extent stays within it
+    dt = 1e-3 * scale ** 1.5   # the same fraction of a dynamical time at every scale
+    ctx.upload_bodies(b)
+    ctx.step(bh_params(size=0.0, theta=0.5, eps=0.01 * scale, dt=dt, wrap=1, leaf_size=16), 2)
+    s0 = ctx.download_bodies()
+    ctx.upload_bodies(b)
+    ctx.step(bh_params(size=edge, theta=0.5, eps=0.01 * scale, dt=dt, wrap=0, leaf_size=16), 2)
+    assert np.array_equal(s0.view(np.uint32), ctx.download_bodies().view(np.uint32))
+    with pytest.raises(_lib.HipError):
+        ctx.accelerate(bh_params(size=-1.0))

@@ -117,9 +117,10 @@ def test_served_mix_at_workload_scale_matches_oracles(orc):
             c.upload_bodies(res)
             visits, _ = c.bh_interactions(j.params)
             fresh = c.download_bodies()
+            root = c.bh_root()   # the jobs' roots are found from their bodies
             assert np.array_equal(fresh[:, 8:11].view(np.uint32), res[:, 8:11].view(np.uint32)), j.id
             x = res.copy()
-            _, xv = orc.bh_accelerate_buckets(x, float(j.params.size), theta=float(j.params.theta),
+            _, xv = orc.bh_accelerate_buckets(x, root, theta=float(j.params.theta),
                                               leaf_size=int(j.params.leaf_size), eps=eps)
             eq = visits == xv
             e = np.linalg.norm(fresh[:, 8:11] - x[:, 8:11], axis=1) / np.linalg.norm(x[:, 8:11], axis=1)
