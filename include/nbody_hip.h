@@ -476,6 +476,62 @@ int nbhip_batch_configure(nbhip_batch* b, int P, int W, int launch);
  * NULL) receives its time per step.  Synchronises. */
 int nbhip_batch_tune(nbhip_batch* b, double* best_ms);
 
+/* ---- Barnes-Hut batches: many independent Barnes-Hut systems, one build and one walk --
+ * Replaces the per-solver, per-step tree build and walk of the reference's split solver
+ * (source/solvers/gpu_solver_split.h:135-151) when many simulations run at once.  One
+ * step sorts every system's bodies in one radix sort (the key is the system's index above
+ * 18 levels of its own octree), builds every system's compressed octree in one pass of
+ * each build kernel (no node joins two systems), and walks all systems in one launch, each
+ * wave on 64 bodies of one system with that system's own parameters (mode must be NLOGN;
+ * size, theta, opening, leaf_size, G, eps, dt, wrap each per system; size 0 finds the root
+ * cube from the system's bodies).  Each system's prefix sums, node sequence and force sums
+ * are the ones it gets alone in a batch, so its result does not depend on what else the
+ * batch holds.  Against one context running the system (walk split 1), results are
+ * identical wherever no two of its bodies share an 18-level cell (the context descends
+ * 21).  The batch owns its own stream and device buffers on the context's device. */
+typedef struct nbhip_bh_batch nbhip_bh_batch;
+
+#define NBHIP_BH_BATCH_MAX_SYSTEMS 512
+#define NBHIP_BH_BATCH_LEVELS      18
+#define NBHIP_BATCH_COUNT          8u  /* with NBHIP_BATCH_ACCELERATE: per-body interaction counts */
+
+typedef struct nbhip_bh_batch_info {
+    size_t systems;
+    size_t bodies;
+    size_t device_bytes;   /* body storage and build buffers held on the device */
+    int    levels;         /* octree levels per system (NBHIP_BH_BATCH_LEVELS) */
+    int    max_systems;    /* NBHIP_BH_BATCH_MAX_SYSTEMS */
+} nbhip_bh_batch_info;
+
+int  nbhip_bh_batch_create(nbhip_ctx* ctx, nbhip_bh_batch** out);
+void nbhip_bh_batch_destroy(nbhip_bh_batch* b);
+
+/* Add a system of n reference `Body` records; *id receives its handle.  Uploads.  Errors
+ * go to nbhip_last_error of the batch's context. */
+int nbhip_bh_batch_add(nbhip_bh_batch* b, const void* bodies, size_t n, const nbhip_params* p, int* id);
+
+/* Add a system of n bodies drawn on the device from `spec`, without a host copy. */
+int nbhip_bh_batch_add_generated(nbhip_bh_batch* b, const nbhip_ics* spec, size_t n, const nbhip_params* p,
+                                 int* id);
+
+/* Remove a system; the systems after it move down in the batch's arrays. */
+int nbhip_bh_batch_remove(nbhip_bh_batch* b, int id);
+
+/* Advance every system `steps` steps (flags: NBHIP_BATCH_GRAPH, NBHIP_BATCH_ACCELERATE,
+ * NBHIP_BATCH_COUNT with ACCELERATE).  Enqueues only on the batch's stream. */
+int nbhip_bh_batch_step(nbhip_bh_batch* b, int steps, unsigned flags);
+
+/* Non-blocking: *idle = 1 when all work enqueued on the batch has finished, else 0. */
+int nbhip_bh_batch_idle(nbhip_bh_batch* b, int* idle);
+
+/* Download min(n, N) bodies of system `id`.  Synchronises; reports a radix-sort fault. */
+int nbhip_bh_batch_download(nbhip_bh_batch* b, int id, void* bodies, size_t n);
+
+/* The per-body interaction counts of system `id` from the last NBHIP_BATCH_COUNT step. */
+int nbhip_bh_batch_visits(nbhip_bh_batch* b, int id, uint32_t* visits, size_t n);
+
+int nbhip_bh_batch_query(nbhip_bh_batch* b, nbhip_bh_batch_info* out);
+
 /* ---- timing: HIP events on the context's stream ----------------------------------- */
 
 /* Record a start event on the context's stream. */

@@ -65,6 +65,10 @@ BATCH_LAUNCH_PAIRS = 2
 BATCH_GRAPH = 1
 BATCH_PER_SYSTEM = 2
 BATCH_ACCELERATE = 4
+BATCH_COUNT = 8
+
+BH_BATCH_MAX_SYSTEMS = 512
+BH_BATCH_LEVELS = 18
 
 
 class TuneEntry(C.Structure):
@@ -78,6 +82,14 @@ class BatchInfo(C.Structure):
         ("systems", C.c_size_t), ("bodies", C.c_size_t), ("items", C.c_size_t),
         ("interactions_per_step", C.c_double), ("device_bytes", C.c_size_t),
         ("P", C.c_int), ("W", C.c_int), ("launch", C.c_int),
+    ]
+
+
+class BhBatchInfo(C.Structure):
+    """nbhip_bh_batch_info (include/nbody_hip.h)."""
+    _fields_ = [
+        ("systems", C.c_size_t), ("bodies", C.c_size_t), ("device_bytes", C.c_size_t),
+        ("levels", C.c_int), ("max_systems", C.c_int),
     ]
 
 
@@ -145,6 +157,16 @@ SIGNATURES = {
     "nbhip_batch_query": (C.c_int, [_vp, C.POINTER(BatchInfo)]),
     "nbhip_batch_configure": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "nbhip_batch_tune": (C.c_int, [_vp, C.POINTER(C.c_double)]),
+    "nbhip_bh_batch_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "nbhip_bh_batch_destroy": (None, [_vp]),
+    "nbhip_bh_batch_add": (C.c_int, [_vp, _vp, _sz, C.POINTER(Params), C.POINTER(C.c_int)]),
+    "nbhip_bh_batch_add_generated": (C.c_int, [_vp, C.POINTER(Ics), _sz, C.POINTER(Params), C.POINTER(C.c_int)]),
+    "nbhip_bh_batch_remove": (C.c_int, [_vp, C.c_int]),
+    "nbhip_bh_batch_step": (C.c_int, [_vp, C.c_int, C.c_uint]),
+    "nbhip_bh_batch_idle": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "nbhip_bh_batch_download": (C.c_int, [_vp, C.c_int, _vp, _sz]),
+    "nbhip_bh_batch_visits": (C.c_int, [_vp, C.c_int, _vp, _sz]),
+    "nbhip_bh_batch_query": (C.c_int, [_vp, C.POINTER(BhBatchInfo)]),
     "nbhip_timer_start": (C.c_int, [_vp]),
     "nbhip_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_long)]),
 }
@@ -501,3 +523,70 @@ class Batch:
         ms = C.c_double()
         self.ctx._check(self.L.nbhip_batch_tune(self.h, C.byref(ms)))
         return ms.value
+
+
+class BhBatch:
+    """Many independent Barnes-Hut systems: one build and one walk per step for all of
+    them, on the batch's own stream (nbhip_bh_batch; errors report through `ctx`)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.L = ctx.L
+        h = C.c_void_p()
+        ctx._check(self.L.nbhip_bh_batch_create(ctx.h, C.byref(h)))
+        self.h = h
+        self.sizes: dict[int, int] = {}
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.nbhip_bh_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, bodies: np.ndarray, p: Params) -> int:
+        b = np.ascontiguousarray(bodies, np.float32)
+        if b.ndim != 2 or b.shape[1] != 12:
+            raise ValueError("bodies must have shape (n, 12)")
+        i = C.c_int()
+        self.ctx._check(self.L.nbhip_bh_batch_add(self.h, _ptr(b), b.shape[0], C.byref(p), C.byref(i)))
+        self.sizes[i.value] = b.shape[0]
+        return i.value
+
+    def add_generated(self, spec: Ics, n: int, p: Params) -> int:
+        i = C.c_int()
+        self.ctx._check(self.L.nbhip_bh_batch_add_generated(self.h, C.byref(spec), n, C.byref(p), C.byref(i)))
+        self.sizes[i.value] = n
+        return i.value
+
+    def remove(self, sid: int) -> None:
+        self.ctx._check(self.L.nbhip_bh_batch_remove(self.h, sid))
+        self.sizes.pop(sid, None)
+
+    def step(self, steps: int = 1, flags: int = 0) -> None:
+        self.ctx._check(self.L.nbhip_bh_batch_step(self.h, steps, flags))
+
+    def idle(self) -> bool:
+        out = C.c_int()
+        self.ctx._check(self.L.nbhip_bh_batch_idle(self.h, C.byref(out)))
+        return bool(out.value)
+
+    def download(self, sid: int) -> np.ndarray:
+        b = np.zeros((self.sizes[sid], 12), np.float32)
+        self.ctx._check(self.L.nbhip_bh_batch_download(self.h, sid, _ptr(b), b.shape[0]))
+        return b
+
+    def visits(self, sid: int) -> np.ndarray:
+        """Per-body interaction counts of the last BATCH_COUNT step."""
+        v = np.zeros(self.sizes[sid], np.uint32)
+        self.ctx._check(self.L.nbhip_bh_batch_visits(self.h, sid, _ptr(v), v.shape[0]))
+        return v
+
+    def info(self) -> BhBatchInfo:
+        out = BhBatchInfo()
+        self.ctx._check(self.L.nbhip_bh_batch_query(self.h, C.byref(out)))
+        return out

@@ -79,17 +79,15 @@ __device__ __forceinline__ Dbl4 operator+(const Dbl4& a, const Dbl4& b)
 }
 
 // ---- 1. keys ------------------------------------------------------------------------------
-// The root edge is `size`, or *size_dev when that is given (the edge found on the device).
-__global__ __launch_bounds__(256) void k_keys(const float4* pm, int n, float size, const float* size_dev,
-                                              unsigned long long* keys, unsigned* vals)
+// L octant digits of the reference's own float descent from a root cube of edge s centred
+// at the origin, most significant first.
+template <int L>
+__device__ __forceinline__ unsigned long long descent_key(const float4 p, float s)
 {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const float4 p = pm[i];
-    float cx = 0.f, cy = 0.f, cz = 0.f, s = size_dev ? *size_dev : size;
+    float cx = 0.f, cy = 0.f, cz = 0.f;
     unsigned long long key = 0;
-    for (int l = 0; l < LEVELS; ++l) {
+    for (int l = 0; l < L; ++l) {
         const unsigned q = unsigned(p.x < cx) | (unsigned(p.y < cy) << 1) | (unsigned(p.z < cz) << 2);
         key = (key << 3) | q;
         const float quart = s * .25f;
@@ -98,7 +96,16 @@ __global__ __launch_bounds__(256) void k_keys(const float4* pm, int n, float siz
         cz = (q & 4u) ? (cz - quart) : (cz + quart);
         s = s * .5f;
     }
-    keys[i] = key;
+    return key;
+}
+
+// The root edge is `size`, or *size_dev when that is given (the edge found on the device).
+__global__ __launch_bounds__(256) void k_keys(const float4* pm, int n, float size, const float* size_dev,
+                                              unsigned long long* keys, unsigned* vals)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = descent_key<LEVELS>(pm[i], size_dev ? *size_dev : size);
     vals[i] = unsigned(i);
 }
 
@@ -108,21 +115,44 @@ __global__ __launch_bounds__(256) void k_keys(const float4* pm, int n, float siz
 // large enough to hold every body.  Per-block maxima, then one block.
 constexpr int ROOT_BLOCKS_MAX = 1024;
 
-__global__ __launch_bounds__(256) void k_root_extent(const float4* pm, int n, double* part)
+// The smallest power of two at least 2 m, for the largest coordinate magnitude m.
+__device__ float root_edge(double m)
 {
-    __shared__ double sh[256];
-    double m = 0.0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const float4 p = pm[i];
-        m = fmax(m, fmax(fabs(double(p.x)), fmax(fabs(double(p.y)), fabs(double(p.z)))));
-    }
-    sh[threadIdx.x] = m;
+    const double t = 2.0 * m;
+    int e = 0;
+    const double f = t > 0.0 ? frexp(t, &e) : 0.0;  // t = f 2^e, f in [0.5, 1)
+    double size = t > 0.0 ? ldexp(1.0, f == 0.5 ? e - 1 : e) : 1.0;
+    if (!(size <= 0x1p127)) size = 0x1p127;  // beyond FP32 range, or NaN-only positions
+    return float(size);
+}
+
+// FP64 maximum of |x|, |y|, |z| of one body (exact: no rounding).
+__device__ __forceinline__ double body_extent(const float4 p)
+{
+    return fmax(fabs(double(p.x)), fmax(fabs(double(p.y)), fabs(double(p.z))));
+}
+
+// Maximum of v over a block of 256 threads (every thread gets it).
+__device__ double block_max256(double v, double* sh)
+{
+    sh[threadIdx.x] = v;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+    const double m = sh[0];
+    __syncthreads();
+    return m;
+}
+
+__global__ __launch_bounds__(256) void k_root_extent(const float4* pm, int n, double* part)
+{
+    __shared__ double sh[256];
+    double m = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) m = fmax(m, body_extent(pm[i]));
+    m = block_max256(m, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
 __global__ __launch_bounds__(256) void k_root_size(const double* part, int nb, float* root)
@@ -130,26 +160,57 @@ __global__ __launch_bounds__(256) void k_root_size(const double* part, int nb, f
     __shared__ double sh[256];
     double m = 0.0;
     for (int b = threadIdx.x; b < nb; b += 256) m = fmax(m, part[b]);
-    sh[threadIdx.x] = m;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const double t = 2.0 * sh[0];
-        int e = 0;
-        const double f = t > 0.0 ? frexp(t, &e) : 0.0;  // t = f 2^e, f in [0.5, 1)
-        double size = t > 0.0 ? ldexp(1.0, f == 0.5 ? e - 1 : e) : 1.0;
-        if (!(size <= 0x1p127)) size = 0x1p127;  // beyond FP32 range, or NaN-only positions
-        *root = float(size);
+    m = block_max256(m, sh);
+    if (threadIdx.x == 0) *root = root_edge(m);
+}
+
+// Scan tiles: SC_IPT positions per thread of a SC_BLOCK-thread block.
+constexpr int SC_BLOCK = 256;
+constexpr int SC_IPT = 4;
+constexpr int SC_TILE = SC_BLOCK * SC_IPT;
+
+// ---- the same for a batch of systems: one block per item (a scan tile of one system), then
+// one block per system.  A system's root depends on its own bodies only (a maximum does not
+// round).
+__global__ __launch_bounds__(256) void k_bextent(const float4* pm, const BhSys* sys, const int2* items, double* ext)
+{
+    __shared__ double sh[256];
+    const int2 it = items[blockIdx.x];
+    const BhSys sy = sys[it.x];
+    const int lo = sy.off + it.y * SC_TILE, hi = sy.off + sy.n;
+    double m = 0.0;
+    for (int k = lo + int(threadIdx.x); k < min(hi, lo + SC_TILE); k += 256) m = fmax(m, body_extent(pm[k]));
+    m = block_max256(m, sh);
+    if (threadIdx.x == 0) ext[blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(256) void k_broot(const BhSys* sys, const double* ext, float* roots)
+{
+    __shared__ double sh[256];
+    const BhSys sy = sys[blockIdx.x];
+    double m = 0.0;
+    for (int t = threadIdx.x; t < sy.tiles; t += 256) m = fmax(m, ext[sy.first_item + t]);
+    m = block_max256(m, sh);
+    if (threadIdx.x == 0) roots[blockIdx.x] = root_edge(m);
+}
+
+// Batch keys: the system's index above 18 levels of its own descent (the first 18 digits of
+// the key the system alone gets).  The value is the position.
+__global__ __launch_bounds__(256) void k_bkeys(const float4* pm, const BhSys* sys, const int2* items,
+                                               const float* roots, unsigned long long* keys, unsigned* vals)
+{
+    const int2 it = items[blockIdx.x];
+    const BhSys sy = sys[it.x];
+    const float size = sy.size > 0.f ? sy.size : roots[it.x];
+    const int lo = sy.off + it.y * SC_TILE, hi = min(sy.off + sy.n, lo + SC_TILE);
+    const unsigned long long top = (unsigned long long)it.x << (3 * BH_BATCH_LEVELS);
+    for (int k = lo + int(threadIdx.x); k < hi; k += 256) {
+        keys[k] = top | descent_key<BH_BATCH_LEVELS>(pm[k], size);
+        vals[k] = unsigned(k);
     }
 }
 
 // ---- scans (two phases: tile totals, then tile scans that add the totals before them) -----
-constexpr int SC_BLOCK = 256;
-constexpr int SC_IPT = 4;
-constexpr int SC_TILE = SC_BLOCK * SC_IPT;
 
 template <typename T>
 __device__ __forceinline__ T shfl_up_item(const T& v, int off)
@@ -210,17 +271,6 @@ struct ArrayOf {
     __device__ T operator()(int k) const { return in[k]; }
 };
 
-// FP64 moments {m, m x, m y, m z} of a body in key order.
-struct MomentOf {
-    const float4* spm;
-    __device__ Dbl4 operator()(int k) const
-    {
-        const float4 p = spm[k];
-        const double m = p.w;
-        return Dbl4{ m, m * double(p.x), m * double(p.y), m * double(p.z) };
-    }
-};
-
 // part holds the totals of consecutive runs of SC_TILE / parts_per_tile items.
 template <typename T, typename Src>
 __global__ __launch_bounds__(SC_BLOCK) void k_scan_apply(Src in, int n, const T* part, T* out, int parts_per_tile)
@@ -246,27 +296,11 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_apply(Src in, int n, const T*
     }
     T total;
     T run = before + block_exclusive_scan(sum, sh, total);
-    if constexpr (sizeof(T) >= 16) {
-        // wide items leave through LDS, so that each store instruction covers consecutive
-        // items across the wave instead of one thread's run of SC_IPT
-        __shared__ T stage[SC_TILE];
-        for (int k = 0; k < SC_IPT; ++k) {
-            stage[threadIdx.x * SC_IPT + k] = run;
-            run = run + v[k];
-        }
-        if (base <= n - 1 && n - 1 < base + SC_IPT) out[n] = run;  // items past n - 1 are zero
-        __syncthreads();
-        for (int k = 0; k < SC_IPT; ++k) {
-            const int i = k * SC_BLOCK + threadIdx.x;
-            if (blockIdx.x * SC_TILE + i < n) out[blockIdx.x * SC_TILE + i] = stage[i];
-        }
-    } else {
-        for (int k = 0; k < SC_IPT; ++k) {
-            if (base + k < n) out[base + k] = run;
-            run = run + v[k];
-        }
-        if (base <= n - 1 && n - 1 < base + SC_IPT) out[n] = run;  // the thread holding the last item
+    for (int k = 0; k < SC_IPT; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run = run + v[k];
     }
+    if (base <= n - 1 && n - 1 < base + SC_IPT) out[n] = run;  // the thread holding the last item
 }
 
 // The whole scan in one block, for short arrays where launches cost more than bytes:
@@ -670,17 +704,56 @@ int radix_sort(nbhip_ctx* c, int n, int src)
 // Each block gathers one scan tile (coalesced: position tile + i * SC_BLOCK + thread) and
 // writes the tile's moment total, so the prefix scan that follows needs only its apply
 // phase, which computes the moments again from the gathered bodies (the same doubles).
+//
+// A tile covers positions [origin, origin + SC_TILE) of a run of positions ending at `end`
+// (one system); its total goes to part[slot], the run's tiles having slots first .. slot.
+// The run's prefix sums go to out[position + shift]: n + 1 of them, the last its total.
+// One system alone is one run from position 0; a batch has one run per system, its tiles
+// counted from the system's first position, so each system's sums are the ones it gets
+// alone.
+struct ScanTile {
+    int origin, end, slot, first, shift;
+};
+
+struct WholeRun {
+    int n;
+    __device__ ScanTile tile() const
+    {
+        const int t = int(blockIdx.x);
+        return ScanTile{ t * SC_TILE, n, t, 0, 0 };
+    }
+};
+
+struct SystemRuns {
+    const BhSys* sys;
+    const int2* items;
+    __device__ ScanTile tile() const
+    {
+        const int2 it = items[blockIdx.x];
+        const BhSys sy = sys[it.x];
+        return ScanTile{ sy.off + it.y * SC_TILE, sy.off + sy.n, sy.first_item + it.y, sy.first_item, it.x };
+    }
+};
+
+__device__ __forceinline__ Dbl4 moment(const float4 p)
+{
+    const double m = p.w;
+    return Dbl4{ m, m * double(p.x), m * double(p.y), m * double(p.z) };
+}
+
+template <typename Runs>
 __global__ __launch_bounds__(SC_BLOCK) void k_gather(const unsigned* perm, const float4* pm, int n, float4* spm,
-                                                     Dbl4* part)
+                                                     Dbl4* part, Runs runs)
 {
     __shared__ Dbl4 sh[SC_BLOCK];
-    const int base = blockIdx.x * SC_TILE + threadIdx.x;
+    const ScanTile tt = runs.tile();
+    const int base = tt.origin + int(threadIdx.x);
     // every index load, then every body load, in flight together
     unsigned j[SC_IPT];
 #pragma unroll
     for (int i = 0; i < SC_IPT; ++i) {
         const int k = base + i * SC_BLOCK;
-        j[i] = k < n ? perm[k] : unsigned(n);
+        j[i] = k < tt.end ? perm[k] : unsigned(n);
     }
     float4 p[SC_IPT];
 #pragma unroll
@@ -689,15 +762,49 @@ __global__ __launch_bounds__(SC_BLOCK) void k_gather(const unsigned* perm, const
 #pragma unroll
     for (int i = 0; i < SC_IPT; ++i) {
         const int k = base + i * SC_BLOCK;
-        if (k < n) {
+        if (k < tt.end) {
             spm[k] = p[i];
-            const double m = p[i].w;
-            sum = sum + Dbl4{ m, m * double(p[i].x), m * double(p[i].y), m * double(p[i].z) };
+            sum = sum + moment(p[i]);
         }
     }
     Dbl4 total;
     (void)block_exclusive_scan(sum, sh, total);
-    if (threadIdx.x == 0) part[blockIdx.x] = total;
+    if (threadIdx.x == 0) part[tt.slot] = total;
+}
+
+// The apply phase over the gathered bodies: each block first sums the totals of its run's
+// tiles before it, in a fixed order, then scans its own tile.  The 32-byte sums leave
+// through LDS, so each store instruction covers consecutive positions across the wave.
+template <typename Runs>
+__global__ __launch_bounds__(SC_BLOCK) void k_moment_scan(const float4* spm, const Dbl4* part, Dbl4* out, Runs runs)
+{
+    __shared__ Dbl4 sh[SC_BLOCK];
+    __shared__ Dbl4 stage[SC_TILE];
+    const ScanTile tt = runs.tile();
+    Dbl4 pre{};
+    for (int b = int(threadIdx.x); b < tt.slot - tt.first; b += SC_BLOCK) pre = pre + part[tt.first + b];
+    Dbl4 before;
+    (void)block_exclusive_scan(pre, sh, before);
+    const int base = tt.origin + int(threadIdx.x) * SC_IPT;
+    Dbl4 v[SC_IPT];
+    Dbl4 sum{};
+    for (int k = 0; k < SC_IPT; ++k) {
+        v[k] = base + k < tt.end ? moment(spm[base + k]) : Dbl4{};
+        sum = sum + v[k];
+    }
+    Dbl4 total;
+    Dbl4 run = before + block_exclusive_scan(sum, sh, total);
+    for (int k = 0; k < SC_IPT; ++k) {
+        stage[threadIdx.x * SC_IPT + k] = run;
+        run = run + v[k];
+    }
+    // the thread holding the run's last position writes its total (items past it are zero)
+    if (base <= tt.end - 1 && tt.end - 1 < base + SC_IPT) out[tt.end + tt.shift] = run;
+    __syncthreads();
+    for (int k = 0; k < SC_IPT; ++k) {
+        const int i = k * SC_BLOCK + int(threadIdx.x);
+        if (tt.origin + i < tt.end) out[tt.origin + i + tt.shift] = stage[i];
+    }
 }
 
 // ---- 4. node structure from adjacent-key prefixes ---------------------------------------------
@@ -1035,9 +1142,11 @@ __global__ __launch_bounds__(256) void k_next_smaller(int n, DView dv, unsigned*
 // position a on its own) is kept, i.e. is not inside a bucket.  The deepest node at a has
 // level d(a) and ends where d first drops below d(a), at ns[a]; the next one has level
 // cpl(a, b + 1) = d(b) and ends where d first drops below that after b, at ns[b].
+// Cells above root_level are not nodes: a batch's keys start with root_level digits of
+// system index, so its systems' roots are the cells of that level, and nothing joins them.
 template <typename Visit>
 __device__ bool walk_position(const KeyView& kv, const DView& dv, const unsigned* ns, int a, int leaf_size,
-                              Visit&& visit)
+                              int root_level, Visit&& visit)
 {
     if (kv.n == 1) return true;
     // the no-node case asks whether a's cell holds more than leaf_size bodies: the window
@@ -1050,13 +1159,14 @@ __device__ bool walk_position(const KeyView& kv, const DView& dv, const unsigned
     int m = dv.d(a);
     if (m <= dl) {
         // no node starts at a: the deepest node holding a starts before it, at level d(a-1)
-        return dl < LEVELS && exceeds(dl);
+        // (above a system's root: a is a system of one body, a leaf)
+        return dl < root_level || (dl < LEVELS && exceeds(dl));
     }
     bool deepest_is_bucket = false, first = true;
     bool held = false;  // a bucket seen but not yet emitted (its parent decides)
     int held_l = 0, held_b = 0;
     int b = int(ns[a]);  // m > dl >= -1 puts a below n - 1
-    while (m > dl) {
+    while (m > dl && m >= root_level) {
         const bool bucket = (b - a + 1) <= leaf_size || m == LEVELS;
         if (first) {
             deepest_is_bucket = bucket;
@@ -1079,8 +1189,9 @@ __device__ bool walk_position(const KeyView& kv, const DView& dv, const unsigned
         b = nb;
     }
     if (held) {
-        // the shallowest node at a is a bucket; its parent starts before a, at level dl
-        if (dl < 0 || exceeds(dl)) visit(held_l, held_b, true);
+        // the shallowest node at a is a bucket; its parent starts before a, at level dl, or
+        // it is a system's root
+        if (dl < root_level || exceeds(dl)) visit(held_l, held_b, true);
     }
     return !deepest_is_bucket;
 }
@@ -1096,18 +1207,25 @@ struct VisitCache {
 
 // cnt[a]: entries position a adds to the pre-order (kept nodes + leaf); kept[a]: nodes only.
 // Block 0 also empties the root's child table, which k_emit fills.
+// The system a batch position belongs to: the top digits of its key.
+__device__ __forceinline__ int key_system(unsigned long long k) { return int(k >> (3 * BH_BATCH_LEVELS)); }
+
+// sys (a batch, else null) gives each position its system's bucket size; top (one system,
+// else null) is emptied for k_emit.
 __global__ __launch_bounds__(256) void k_count(const unsigned long long* keys, int n, int leaf_size,
                                                unsigned* cnt, unsigned char* kept, DView dv, const unsigned* ns,
-                                               VisitCache vc, unsigned* block_total, unsigned* top)
+                                               VisitCache vc, unsigned* block_total, unsigned* top,
+                                               const BhSys* sys, int root_level)
 {
     __shared__ unsigned wsum[4];
     const int a = blockIdx.x * 256 + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x < 8) top[threadIdx.x] = ~0u;
+    if (top && blockIdx.x == 0 && threadIdx.x < 8) top[threadIdx.x] = ~0u;
     unsigned entries = 0u;
     if (a < n) {
         unsigned nodes = 0;
         uint4 lo = make_uint4(0u, 0u, 0u, 0u), hi = lo;
-        const bool leaf = walk_position(KeyView{ keys, n }, dv, ns, a, leaf_size, [&](int lvl, int b, bool bucket) {
+        const int leaf_a = sys ? sys[key_system(keys[a])].leaf : leaf_size;
+        const bool leaf = walk_position(KeyView{ keys, n }, dv, ns, a, leaf_a, root_level, [&](int lvl, int b, bool bucket) {
             const unsigned m = VisitCache::meta(lvl, bucket);
             switch (nodes) {
             case 0: lo.x = unsigned(b); lo.y = m; break;
@@ -1137,7 +1255,7 @@ struct EmitArgs {
     int leaf_size;
     const unsigned* base;  // n + 1 entries
     const unsigned char* kept;
-    const Dbl4* pref;      // n + 1 entries
+    const Dbl4* pref;      // n + 1 entries (a batch: n + systems, each system's n + 1 in turn)
     const float4* spm;
     float size;
     const float* size_dev; // the root edge found on the device, or null: `size`
@@ -1148,7 +1266,10 @@ struct EmitArgs {
     DView dv;
     const unsigned* ns;
     VisitCache vc;
-    unsigned* top;         // the root's children, by their octant in the root
+    unsigned* top;         // the root's children, by their octant in the root (one system)
+    const BhSys* sys;      // a batch: each system's constants, or null (the fields above)
+    const float* roots;    // a batch: the root edges found from the bodies
+    int root_level;        // a batch: key digits above the systems' roots
 };
 
 __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
@@ -1159,13 +1280,24 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
     const unsigned base = e.base[a];
     const unsigned kept = e.kept[a];
     const bool leaf_kept = e.base[a + 1] - base > kept;
-    const float theta_sq = e.theta * e.theta;
+    // the constants of a's system; in a batch its prefix sums sit `shift` entries on
+    int leaf = e.leaf_size, opening = e.opening, shift = 0;
+    float theta = e.theta, root = e.size_dev ? *e.size_dev : e.size;
+    if (e.sys) {
+        shift = key_system(e.keys[a]);
+        const BhSys& sy = e.sys[shift];
+        leaf = sy.leaf;
+        opening = sy.opening;
+        theta = sy.theta;
+        root = sy.size > 0.f ? sy.size : e.roots[shift];
+    }
+    const float theta_sq = theta * theta;
     // The root (node 0) covers every key, which share its r leading digits; its children
     // are the nodes spanning exactly the keys of one value of digit r + 1, i.e. those that
     // start where d drops to r or below and end before the next such drop.  At most 8
-    // positions start such a run.
-    const int r = e.n > 1 ? KeyView{ e.keys, e.n }.cpl(0, e.n - 1) : LEVELS;
-    const bool run_start = e.n > 1 && e.dv.d(a - 1) <= r;
+    // positions start such a run.  (One system only: a batch walks without the table.)
+    const int r = e.top && e.n > 1 ? KeyView{ e.keys, e.n }.cpl(0, e.n - 1) : LEVELS;
+    const bool run_start = e.top && e.n > 1 && e.dv.d(a - 1) <= r;
     const auto make = [&](unsigned slot, int lvl, int b, bool bucket, const Dbl4& lo, const Dbl4& hi, unsigned next) {
         if (slot >= e.capacity) return;
         const double M = hi.m - lo.m;
@@ -1175,10 +1307,10 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
                                 float(M));
         else
             nd.cm = make_float4(0.f, 0.f, 0.f, 0.f);
-        float s = e.size_dev ? *e.size_dev : e.size;  // the cell edge at this level, halved as the reference does
-        for (int l = 0; l < lvl; ++l) s = s * .5f;
+        float s = root;  // the cell edge at this level, halved as the reference does
+        for (int l = e.root_level; l < lvl; ++l) s = s * .5f;
         const float s2 = s * s;
-        nd.thr = e.opening == NBHIP_OPEN_TEXTBOOK ? s2 / theta_sq : s2 * theta_sq;
+        nd.thr = opening == NBHIP_OPEN_TEXTBOOK ? s2 / theta_sq : s2 * theta_sq;
         nd.next = next;
         nd.first = unsigned(a);
         nd.count = bucket ? unsigned(b - a + 1) : 0u;
@@ -1189,22 +1321,22 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
     // shallow, so the j-th one met goes to base + kept - 1 - j
     if (kept > VC_MAX) {
         unsigned next_slot = base + kept;
-        const Dbl4 lo = e.pref[a];
-        walk_position(KeyView{ e.keys, e.n }, e.dv, e.ns, a, e.leaf_size, [&](int lvl, int b, bool bucket) {
-            make(--next_slot, lvl, b, bucket, lo, e.pref[b + 1], e.base[b + 1]);
+        const Dbl4 lo = e.pref[a + shift];
+        walk_position(KeyView{ e.keys, e.n }, e.dv, e.ns, a, leaf, e.root_level, [&](int lvl, int b, bool bucket) {
+            make(--next_slot, lvl, b, bucket, lo, e.pref[b + 1 + shift], e.base[b + 1]);
         });
     } else if (kept > 0u) {
         // every load first, then the nodes
         const uint4 v0 = e.vc.v[2 * size_t(a)];
         const uint4 v1 = kept > 2u ? e.vc.v[2 * size_t(a) + 1] : make_uint4(0u, 0u, 0u, 0u);
         const unsigned bs[VC_MAX] = { v0.x, v0.z, v1.x, v1.z }, ms[VC_MAX] = { v0.y, v0.w, v1.y, v1.w };
-        const Dbl4 lo = e.pref[a];
+        const Dbl4 lo = e.pref[a + shift];
         Dbl4 hi[VC_MAX];
         unsigned next[VC_MAX];
 #pragma unroll
         for (unsigned j = 0; j < VC_MAX; ++j) {
             if (j < kept) {
-                hi[j] = e.pref[bs[j] + 1];
+                hi[j] = e.pref[bs[j] + 1 + shift];
                 next[j] = e.base[bs[j] + 1];
             }
         }
@@ -1356,6 +1488,109 @@ __device__ __forceinline__ void store_stream(float4* p, const float4 v)
     __builtin_nontemporal_store(F4v{ v.x, v.y, v.z, v.w }, reinterpret_cast<F4v*>(p));
 }
 
+// One lane's reference traversal of the pre-order run [its start, end): idx is the lane's
+// first node (end for a lane with no body), and the wave visits the union of its lanes'
+// traversals in pre-order.  Interactions go to acc; COUNT also counts them in applied.
+template <bool CUT, bool COUNT>
+__device__ __forceinline__ void walk_run(const BhNode* __restrict__ nodes, const float4* __restrict__ spm, int n,
+                                         float eps2, const float4 x, const float r2, unsigned idx,
+                                         const unsigned end, Accum& acc, unsigned& applied)
+{
+    // After a step every lane's pointer is past cur, so when any lane sits on cur + 1 (an
+    // opened node's first child, next in pre-order) that is the new minimum and the
+    // wave-wide reduction is skipped.  The node is wave-uniform (scalar registers); the
+    // per-lane choices -- apply it, open it, apply a bucket's bodies -- are predicates on
+    // straight-line code rather than branches.  Partial sums fold into the compensated
+    // totals every 8 steps of the wave (at most 8 nodes or buckets per lane in between).
+    unsigned cur = __ockl_wfred_min_u32(idx);
+    unsigned steps = 0;
+    while (cur < end) {
+        const BhNode nd = nodes[cur];
+        const bool at = idx == cur;
+        // empty nodes are skipped (reference bhtree.cpp:108-113); a leaf is applied
+        // directly; otherwise the reference's inverted criterion decides
+        // (bhtree.cpp:125-128), on the distance the force then uses
+        unsigned mbits = __float_as_uint(nd.cm.w);
+        asm("" : "+s"(mbits));  // a scalar test (the node is wave-uniform), not a per-lane compare
+        const bool on = at && (mbits << 1) != 0u;
+        float dx, dy, dz, d2;
+        {
+#pragma clang fp contract(off)
+            dx = nd.cm.x - x.x;
+            dy = nd.cm.y - x.y;
+            dz = nd.cm.z - x.z;
+            d2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+        }
+        const bool agg = (nd.count == 1) | (d2 > nd.thr);
+        interact_d2<CUT>(on && agg, dx, dy, dz, d2, nd.cm.w, r2, eps2, acc.tx, acc.ty, acc.tz);
+        if (COUNT) applied += (on && agg) ? 1u : 0u;
+        const bool opened = on && !agg;
+        // open an internal node: its first child follows in pre-order
+        const unsigned nxt = (nd.count == 0 && opened) ? cur + 1 : nd.next;
+        // an opened bucket, body by body, four bodies' loads in flight at a time; the
+        // trip count is wave-uniform and zero unless some lane opened a bucket
+        const unsigned nb = (nd.count > 1 && __ballot(opened) != 0ull) ? min(nd.count, unsigned(n) - nd.first) : 0u;
+        unsigned j = 0;
+        for (; j + 4 <= nb; j += 4) {
+            const unsigned b = nd.first + j;
+            const float4 q0 = spm[b], q1 = spm[b + 1], q2 = spm[b + 2], q3 = spm[b + 3];
+            interact_if<CUT>(opened, x, r2, q0, eps2, acc.tx, acc.ty, acc.tz);
+            interact_if<CUT>(opened, x, r2, q1, eps2, acc.tx, acc.ty, acc.tz);
+            interact_if<CUT>(opened, x, r2, q2, eps2, acc.tx, acc.ty, acc.tz);
+            interact_if<CUT>(opened, x, r2, q3, eps2, acc.tx, acc.ty, acc.tz);
+        }
+        for (; j < nb; ++j) interact_if<CUT>(opened, x, r2, spm[nd.first + j], eps2, acc.tx, acc.ty, acc.tz);
+        if (COUNT) applied += (opened && nd.count > 1) ? nd.count : 0u;
+        if (at) idx = max(nxt, cur + 1);  // pre-order pointers only move forward
+        // every pointer is now > cur: if a lane is on cur + 1, that is the minimum.  If not,
+        // every lane that sat on cur went to the same place, past cur's subtree; when no
+        // other lane waits before that point, it is the minimum too.
+        const unsigned step = cur + 1;
+        if (__ballot(idx == step) != 0ull) {
+            cur = step;
+        } else {
+            const unsigned skip = max(nd.next, step);
+            if (__ballot(idx < skip) == 0ull)
+                cur = skip;
+            else
+                cur = max(wave_min(idx), step);  // the minimum is >= step; max keeps cur moving
+        }
+        if ((++steps & 7u) == 0u) acc.fold();
+    }
+}
+
+// A body's result: its acceleration (the compensated sum t times G) and, stepping, the
+// reference's semi-implicit Euler step with wrap (detail::integrate_euler, physics.h:60-70),
+// in place.
+template <bool STEP, bool COUNT>
+__device__ __forceinline__ void walk_store(float4* pm, float4* vr, float4* accel, unsigned* visits, unsigned body,
+                                           const float4 x, const float3 t, unsigned applied, float G, float dt,
+                                           float size, int wrap)
+{
+    const float ax = G * t.x, ay = G * t.y, az = G * t.z;
+    store_stream(accel + body, make_float4(ax, ay, az, 0.f));
+    if (COUNT) visits[body] = applied;
+    if (STEP) {
+#pragma clang fp contract(off)
+        float4 v = load_stream(vr + body);
+        float4 p = x;
+        v.x = v.x + ax * dt;
+        v.y = v.y + ay * dt;
+        v.z = v.z + az * dt;
+        p.x = p.x + v.x * dt;
+        p.y = p.y + v.y * dt;
+        p.z = p.z + v.z * dt;
+        if (wrap && size > 0.f) {
+            const float half = size * .5f;
+            p.x = fmodf(fmodf(p.x + half, size) + size, size) - half;
+            p.y = fmodf(fmodf(p.y + half, size) + size, size) - half;
+            p.z = fmodf(fmodf(p.z + half, size) + size, size) - half;
+        }
+        store_stream(pm + body, p);
+        store_stream(vr + body, v);
+    }
+}
+
 // K waves share each 64-body group: wave part p of the group walks the root's children in
 // its octants [8p/K, 8(p+1)/K), which are one contiguous run of the pre-order, and part 0
 // also the root itself.  An octant without an entry in the child table only moves its
@@ -1419,67 +1654,7 @@ __global__ __launch_bounds__(walk_threads<K>()) void k_walk(WalkArgs a)
     }
     Accum acc;
     unsigned applied = 0;
-    // After a step every lane's pointer is past cur, so when any lane sits on cur + 1 (an
-    // opened node's first child, next in pre-order) that is the new minimum and the
-    // wave-wide reduction is skipped.  The node is wave-uniform (scalar registers); the
-    // per-lane choices -- apply it, open it, apply a bucket's bodies -- are predicates on
-    // straight-line code rather than branches.  Partial sums fold into the compensated
-    // totals every 8 steps of the wave (at most 8 nodes or buckets per lane in between).
-    unsigned cur = __ockl_wfred_min_u32(idx);
-    unsigned steps = 0;
-    while (cur < end) {
-        const BhNode nd = a.nodes[cur];
-        const bool at = idx == cur;
-        // empty nodes are skipped (reference bhtree.cpp:108-113); a leaf is applied
-        // directly; otherwise the reference's inverted criterion decides
-        // (bhtree.cpp:125-128), on the distance the force then uses
-        unsigned mbits = __float_as_uint(nd.cm.w);
-        asm("" : "+s"(mbits));  // a scalar test (the node is wave-uniform), not a per-lane compare
-        const bool on = at && (mbits << 1) != 0u;
-        float dx, dy, dz, d2;
-        {
-#pragma clang fp contract(off)
-            dx = nd.cm.x - x.x;
-            dy = nd.cm.y - x.y;
-            dz = nd.cm.z - x.z;
-            d2 = ((dx * dx) + (dy * dy)) + (dz * dz);
-        }
-        const bool agg = (nd.count == 1) | (d2 > nd.thr);
-        interact_d2<CUT>(on && agg, dx, dy, dz, d2, nd.cm.w, r2, a.eps2, acc.tx, acc.ty, acc.tz);
-        if (COUNT) applied += (on && agg) ? 1u : 0u;
-        const bool opened = on && !agg;
-        // open an internal node: its first child follows in pre-order
-        const unsigned nxt = (nd.count == 0 && opened) ? cur + 1 : nd.next;
-        // an opened bucket, body by body, four bodies' loads in flight at a time; the
-        // trip count is wave-uniform and zero unless some lane opened a bucket
-        const unsigned nb = (nd.count > 1 && __ballot(opened) != 0ull) ? min(nd.count, unsigned(a.n) - nd.first) : 0u;
-        unsigned j = 0;
-        for (; j + 4 <= nb; j += 4) {
-            const unsigned b = nd.first + j;
-            const float4 q0 = a.spm[b], q1 = a.spm[b + 1], q2 = a.spm[b + 2], q3 = a.spm[b + 3];
-            interact_if<CUT>(opened, x, r2, q0, a.eps2, acc.tx, acc.ty, acc.tz);
-            interact_if<CUT>(opened, x, r2, q1, a.eps2, acc.tx, acc.ty, acc.tz);
-            interact_if<CUT>(opened, x, r2, q2, a.eps2, acc.tx, acc.ty, acc.tz);
-            interact_if<CUT>(opened, x, r2, q3, a.eps2, acc.tx, acc.ty, acc.tz);
-        }
-        for (; j < nb; ++j) interact_if<CUT>(opened, x, r2, a.spm[nd.first + j], a.eps2, acc.tx, acc.ty, acc.tz);
-        if (COUNT) applied += (opened && nd.count > 1) ? nd.count : 0u;
-        if (at) idx = max(nxt, cur + 1);  // pre-order pointers only move forward
-        // every pointer is now > cur: if a lane is on cur + 1, that is the minimum.  If not,
-        // every lane that sat on cur went to the same place, past cur's subtree; when no
-        // other lane waits before that point, it is the minimum too.
-        const unsigned step = cur + 1;
-        if (__ballot(idx == step) != 0ull) {
-            cur = step;
-        } else {
-            const unsigned skip = max(nd.next, step);
-            if (__ballot(idx < skip) == 0ull)
-                cur = skip;
-            else
-                cur = max(wave_min(idx), step);  // the minimum is >= step; max keeps cur moving
-        }
-        if ((++steps & 7u) == 0u) acc.fold();
-    }
+    walk_run<CUT, COUNT>(a.nodes, a.spm, a.n, a.eps2, x, r2, idx, end, acc, applied);
     if constexpr (K > 1) {
         // parts 1..K-1 hand their sums to part 0, which adds them in part order
         __shared__ float red[GROUPS][K - 1][7][64];
@@ -1509,29 +1684,59 @@ __global__ __launch_bounds__(walk_threads<K>()) void k_walk(WalkArgs a)
         }
     }
     if (!valid) return;
-    const float3 t = acc.total();
-    const float ax = a.G * t.x, ay = a.G * t.y, az = a.G * t.z;
-    store_stream(a.acc + body, make_float4(ax, ay, az, 0.f));
-    if (COUNT) a.visits[body] = applied;
-    if (STEP) {
-#pragma clang fp contract(off)
-        float4 v = load_stream(a.vr + body);
-        float4 p = x;
-        v.x = v.x + ax * a.dt;
-        v.y = v.y + ay * a.dt;
-        v.z = v.z + az * a.dt;
-        p.x = p.x + v.x * a.dt;
-        p.y = p.y + v.y * a.dt;
-        p.z = p.z + v.z * a.dt;
-        if (a.wrap && a.size > 0.f) {
-            const float half = a.size * .5f;
-            p.x = fmodf(fmodf(p.x + half, a.size) + a.size, a.size) - half;
-            p.y = fmodf(fmodf(p.y + half, a.size) + a.size, a.size) - half;
-            p.z = fmodf(fmodf(p.z + half, a.size) + a.size, a.size) - half;
+    walk_store<STEP, COUNT>(a.pm, a.vr, a.acc, a.visits, body, x, acc.total(), applied, a.G, a.dt, a.size, a.wrap);
+}
+
+// Barnes-Hut over many systems at once (one build, one walk): every wave walks 64 bodies of
+// one system, the same 64 a walk of that system alone gives one wave (its groups start at
+// the system's first position), over that system's own run of the pre-order, with its own
+// constants.  The wave's node sequence, and so every sum, is the system's own, whatever
+// else the batch holds.
+struct BatchWalkArgs {
+    const BhNode* nodes;
+    const float4* spm;
+    const unsigned* perm;
+    const unsigned* base;   // pre-order index of the first node at each sorted position
+    float4* pm;
+    float4* vr;
+    float4* acc;
+    unsigned* visits;
+    const BhSys* sys;
+    const int2* waves;      // (system, first position) per 64-body group
+    int nwaves;
+    int n;                  // bodies in the batch
+};
+
+template <bool STEP, bool COUNT>
+__global__ __launch_bounds__(256) void k_bwalk(BatchWalkArgs a)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int lane = int(threadIdx.x & 63);
+    const int w = int(xcd_block()) * 4 + wave;
+    if (w >= a.nwaves) return;
+    const int2 g = a.waves[w];
+    const BhSys sy = a.sys[g.x];
+    const int k = g.y + lane;
+    const bool valid = k < sy.off + sy.n;
+    const float4 x = valid ? a.spm[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const unsigned body = valid ? min(a.perm[k], unsigned(a.n - 1)) : 0u;
+    const unsigned begin = a.base[sy.off], end = a.base[sy.off + sy.n];
+    const unsigned idx = valid ? begin : end;
+    Accum acc;
+    unsigned applied = 0;
+    if (sy.cut) {
+        float r2 = 0.f;
+        if (valid) {
+            const float r = a.vr[body].w;
+            r2 = r * r;
         }
-        store_stream(a.pm + body, p);
-        store_stream(a.vr + body, v);
+        walk_run<true, COUNT>(a.nodes, a.spm, a.n, sy.eps2, x, r2, idx, end, acc, applied);
+    } else {
+        walk_run<false, COUNT>(a.nodes, a.spm, a.n, sy.eps2, x, 0.f, idx, end, acc, applied);
     }
+    if (!valid) return;
+    walk_store<STEP, COUNT>(a.pm, a.vr, a.acc, a.visits, body, x, acc.total(), applied, sy.G, sy.dt, sy.size,
+                            sy.wrap);
 }
 
 // ---- 6. export in the reference's node layout ----------------------------------------------
@@ -1660,12 +1865,11 @@ void check_nlogn_count(size_t n)
         fail(NBHIP_ERR_UNSUPPORTED, "a Barnes-Hut build takes at most %zu bodies, not %zu", MAX_NLOGN_BODIES, n);
 }
 
-}  // namespace
 
-void bh_reserve(nbhip_ctx* c)
+// The buffers of a build over n positions whose prefix sums take `extra` more entries (one
+// per system of a batch but the first), and the scans' and the sort's.
+void bh_grow(nbhip_ctx* c, int n, int extra)
 {
-    check_nlogn_count(c->count);
-    const int n = int(c->count);
     hipStream_t s = c->stream;
     c->bh_keys[0].grow(n, 0, s);
     c->bh_keys[1].grow(n, 0, s);
@@ -1673,7 +1877,7 @@ void bh_reserve(nbhip_ctx* c)
     c->bh_vals[1].grow(n, 0, s);
     c->bh_spm.grow(n, 0, s);
     c->bh_vcache.grow(2 * size_t(n), 0, s);
-    c->bh_pref.grow(size_t(n) + 1, 0, s);
+    c->bh_pref.grow(size_t(n) + size_t(extra) + 1, 0, s);
     c->bh_base.grow(size_t(n) + 1, 0, s);
     c->bh_nodes.grow(2 * size_t(n) + 1, 0, s);
     c->bh_kept.grow(n, 0, s);
@@ -1682,8 +1886,74 @@ void bh_reserve(nbhip_ctx* c)
     rs_reserve(c, n);
     c->bh_uscan.grow(std::max(size_t(grid(n)), size_t(std::max(1, (n + SC_TILE - 1) / SC_TILE))), 0, s);
     c->bh_dscan.grow(size_t(std::max(1, (n + SC_TILE - 1) / SC_TILE)), 0, s);
-    c->bh_root.grow(1, 0, s);
-    c->bh_root_part.grow(ROOT_BLOCKS_MAX, 0, s);
+}
+
+// From the sorted keys and the prefix sums on: the level arrays, the next-smaller
+// positions, the per-position node counts and their scan, and the nodes.  e carries the
+// laws (one system's fields, or a batch's tables); the buffers are filled in here.
+void build_nodes(nbhip_ctx* c, int n, const unsigned long long* keys, int sorted, EmitArgs e)
+{
+    hipStream_t s = c->stream;
+    const LevelSizes ls(n);
+    c->bh_levels.grow(ls.total, 0, s);
+    DView dv{};
+    LevelsTop lt{};
+    dv.levels = lt.levels = ls.levels;
+    for (int l = 0; l < ls.levels; ++l) {
+        dv.L[l] = lt.L[l] = c->bh_levels.ptr + ls.off[l];
+        dv.len[l] = lt.len[l] = ls.len[l];
+    }
+    hipLaunchKernelGGL(k_levels, dim3(ls.nb), dim3(256), 0, s, keys, n, c->bh_levels.ptr + ls.off[0],
+                       c->bh_levels.ptr + ls.off[1], c->bh_levels.ptr + ls.off[2], c->bh_levels.ptr + ls.off[3],
+                       ls.len[3]);
+    NBHIP_CHECK(hipGetLastError());
+    if (ls.levels > 4) {
+        hipLaunchKernelGGL(k_levels_top, dim3(1), dim3(256), 0, s, lt);
+        NBHIP_CHECK(hipGetLastError());
+    }
+
+    const VisitCache vc{ c->bh_vcache.ptr };
+    // the sort's other value buffer is free once the sort is done: it holds the next-smaller
+    // positions
+    unsigned* ns = c->bh_vals[1 - sorted].ptr;
+    if (n > 1) {
+        hipLaunchKernelGGL(k_next_smaller, dim3(grid(n - 1)), dim3(256), 0, s, n, dv, ns);
+        NBHIP_CHECK(hipGetLastError());
+    }
+    static_assert(SC_TILE % 256 == 0, "k_count blocks tile the scan tiles");
+    c->bh_uscan.grow(grid(n), 0, s);
+    hipLaunchKernelGGL(k_count, dim3(grid(n)), dim3(256), 0, s, keys, n, e.leaf_size, c->bh_base.ptr,
+                       c->bh_kept.ptr, dv, ns, vc, c->bh_uscan.ptr, e.top, e.sys, e.root_level);
+    NBHIP_CHECK(hipGetLastError());
+    if (n <= SC_SINGLE_MAX) {
+        exclusive_scan<unsigned>(c, c->bh_base.ptr, n, c->bh_base.ptr, c->bh_uscan);
+    } else {
+        exclusive_scan_apply<unsigned>(c, ArrayOf<unsigned>{ c->bh_base.ptr }, n, c->bh_base.ptr, c->bh_uscan.ptr,
+                                       SC_TILE / 256);
+    }
+
+    e.keys = keys;
+    e.n = n;
+    e.base = c->bh_base.ptr;
+    e.kept = c->bh_kept.ptr;
+    e.pref = c->bh_pref.ptr;
+    e.spm = c->bh_spm.ptr;
+    e.nodes = c->bh_nodes.ptr;
+    e.capacity = unsigned(c->bh_nodes.cap);
+    e.dv = dv;
+    e.ns = ns;
+    e.vc = vc;
+    hipLaunchKernelGGL(k_emit, dim3(grid(n)), dim3(256), 0, s, e);
+    NBHIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+void bh_reserve(nbhip_ctx* c)
+{
+    check_nlogn_count(c->count);
+    bh_grow(c, int(c->count), 0);
+    c->bh_root.grow(1, 0, c->stream);
+    c->bh_root_part.grow(ROOT_BLOCKS_MAX, 0, c->stream);
 }
 
 // The root edge of a build: p.size, or (p.size == 0) found from the bodies on the device;
@@ -1731,17 +2001,7 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     check_nlogn_count(c->count);
     const int n = int(c->count);
     hipStream_t s = c->stream;
-    c->bh_keys[0].grow(n, 0, s);
-    c->bh_keys[1].grow(n, 0, s);
-    c->bh_vals[0].grow(n, 0, s);
-    c->bh_vals[1].grow(n, 0, s);
-    c->bh_spm.grow(n, 0, s);
-    c->bh_vcache.grow(2 * size_t(n), 0, s);
-    c->bh_pref.grow(size_t(n) + 1, 0, s);
-    c->bh_base.grow(size_t(n) + 1, 0, s);
-    c->bh_nodes.grow(2 * size_t(n) + 1, 0, s);
-    c->bh_kept.grow(n, 0, s);
-    c->bh_top.grow(8, 0, s);
+    bh_grow(c, n, 0);
 
     const float* root = bh_root(c, p, n);
     hipLaunchKernelGGL(k_keys, dim3(grid(n)), dim3(256), 0, s, c->pm[c->cur].ptr, n, p.size, root,
@@ -1753,58 +2013,104 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     const unsigned* perm = c->bh_vals[sorted].ptr;
 
     const int tiles = std::max(1, (n + SC_TILE - 1) / SC_TILE);
-    c->bh_dscan.grow(size_t(tiles), 0, s);
-    hipLaunchKernelGGL(k_gather, dim3(tiles), dim3(SC_BLOCK), 0, s, perm, c->pm[c->cur].ptr, n, c->bh_spm.ptr,
-                       c->bh_dscan.ptr);
+    hipLaunchKernelGGL(k_gather<WholeRun>, dim3(tiles), dim3(SC_BLOCK), 0, s, perm, c->pm[c->cur].ptr, n,
+                       c->bh_spm.ptr, c->bh_dscan.ptr, WholeRun{ n });
     NBHIP_CHECK(hipGetLastError());
-    exclusive_scan_apply<Dbl4>(c, MomentOf{ c->bh_spm.ptr }, n, c->bh_pref.ptr, c->bh_dscan.ptr);
+    hipLaunchKernelGGL(k_moment_scan<WholeRun>, dim3(tiles), dim3(SC_BLOCK), 0, s, c->bh_spm.ptr, c->bh_dscan.ptr,
+                       c->bh_pref.ptr, WholeRun{ n });
+    NBHIP_CHECK(hipGetLastError());
 
-    const LevelSizes ls(n);
-    c->bh_levels.grow(ls.total, 0, s);
-    DView dv{};
-    LevelsTop lt{};
-    dv.levels = lt.levels = ls.levels;
-    for (int l = 0; l < ls.levels; ++l) {
-        dv.L[l] = lt.L[l] = c->bh_levels.ptr + ls.off[l];
-        dv.len[l] = lt.len[l] = ls.len[l];
-    }
-    hipLaunchKernelGGL(k_levels, dim3(ls.nb), dim3(256), 0, s, keys, n, c->bh_levels.ptr + ls.off[0],
-                       c->bh_levels.ptr + ls.off[1], c->bh_levels.ptr + ls.off[2], c->bh_levels.ptr + ls.off[3],
-                       ls.len[3]);
-    NBHIP_CHECK(hipGetLastError());
-    if (ls.levels > 4) {
-        hipLaunchKernelGGL(k_levels_top, dim3(1), dim3(256), 0, s, lt);
-        NBHIP_CHECK(hipGetLastError());
-    }
-
-    const int leaf = std::max(1, p.leaf_size);
-    const VisitCache vc{ c->bh_vcache.ptr };
-    // the sort's other value buffer is free once the sort is done: it holds the next-smaller
-    // positions
-    unsigned* ns = c->bh_vals[1 - sorted].ptr;
-    if (n > 1) {
-        hipLaunchKernelGGL(k_next_smaller, dim3(grid(n - 1)), dim3(256), 0, s, n, dv, ns);
-        NBHIP_CHECK(hipGetLastError());
-    }
-    static_assert(SC_TILE % 256 == 0, "k_count blocks tile the scan tiles");
-    c->bh_uscan.grow(grid(n), 0, s);
-    hipLaunchKernelGGL(k_count, dim3(grid(n)), dim3(256), 0, s, keys, n, leaf, c->bh_base.ptr,
-                       c->bh_kept.ptr, dv, ns, vc, c->bh_uscan.ptr, c->bh_top.ptr);
-    NBHIP_CHECK(hipGetLastError());
-    if (n <= SC_SINGLE_MAX) {
-        exclusive_scan<unsigned>(c, c->bh_base.ptr, n, c->bh_base.ptr, c->bh_uscan);
-    } else {
-        exclusive_scan_apply<unsigned>(c, ArrayOf<unsigned>{ c->bh_base.ptr }, n, c->bh_base.ptr, c->bh_uscan.ptr,
-                                       SC_TILE / 256);
-    }
-
-    EmitArgs e{ keys, n, leaf, c->bh_base.ptr, c->bh_kept.ptr, c->bh_pref.ptr, c->bh_spm.ptr, p.size, root,
-                p.theta, p.opening, c->bh_nodes.ptr, unsigned(c->bh_nodes.cap), dv, ns, vc, c->bh_top.ptr };
-    hipLaunchKernelGGL(k_emit, dim3(grid(n)), dim3(256), 0, s, e);
-    NBHIP_CHECK(hipGetLastError());
+    EmitArgs e{};
+    e.leaf_size = std::max(1, p.leaf_size);
+    e.size = p.size;
+    e.size_dev = root;
+    e.theta = p.theta;
+    e.opening = p.opening;
+    e.top = c->bh_top.ptr;
+    e.sys = nullptr;
+    e.roots = nullptr;
+    e.root_level = 0;
+    build_nodes(c, n, keys, sorted, e);
     c->bh_built = true;
     c->bh_size = p.size;  // 0: the edge sits in bh_root on the device
     ++c->bh_builds;
+}
+
+void bh_batch_reserve(nbhip_ctx* c, const BhBatchView& v)
+{
+    check_nlogn_count(c->count);
+    bh_grow(c, int(c->count), v.systems);
+    c->bh_dscan.grow(size_t(std::max(1, v.nitems)), 0, c->stream);
+}
+
+void bh_batch_build(nbhip_ctx* c, const BhBatchView& v)
+{
+    check_nlogn_count(c->count);
+    const int n = int(c->count);
+    if (n == 0 || v.systems == 0) return;
+    hipStream_t s = c->stream;
+    bh_grow(c, n, v.systems);
+    const float4* pm = c->pm[c->cur].ptr;
+    if (v.any_auto) {
+        hipLaunchKernelGGL(k_bextent, dim3(v.nitems), dim3(256), 0, s, pm, v.sys, v.items, v.ext);
+        NBHIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_broot, dim3(v.systems), dim3(256), 0, s, v.sys, v.ext, v.roots);
+        NBHIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_bkeys, dim3(v.nitems), dim3(256), 0, s, pm, v.sys, v.items, v.roots, c->bh_keys[0].ptr,
+                       c->bh_vals[0].ptr);
+    NBHIP_CHECK(hipGetLastError());
+    const int sorted = radix_sort(c, n, 0);
+    c->bh_sorted = sorted;
+    const unsigned long long* keys = c->bh_keys[sorted].ptr;
+    const unsigned* perm = c->bh_vals[sorted].ptr;
+
+    // every system's tiles counted from its own first position: its prefix sums are the
+    // ones it has alone
+    c->bh_dscan.grow(size_t(v.nitems), 0, s);
+    const SystemRuns runs{ v.sys, v.items };
+    hipLaunchKernelGGL(k_gather<SystemRuns>, dim3(v.nitems), dim3(SC_BLOCK), 0, s, perm, pm, n, c->bh_spm.ptr,
+                       c->bh_dscan.ptr, runs);
+    NBHIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_moment_scan<SystemRuns>, dim3(v.nitems), dim3(SC_BLOCK), 0, s, c->bh_spm.ptr,
+                       c->bh_dscan.ptr, c->bh_pref.ptr, runs);
+    NBHIP_CHECK(hipGetLastError());
+
+    EmitArgs e{};
+    e.leaf_size = 1;
+    e.top = nullptr;
+    e.sys = v.sys;
+    e.roots = v.roots;
+    e.root_level = BH_BATCH_ROOT_LEVEL;
+    build_nodes(c, n, keys, sorted, e);
+    c->bh_built = false;  // not a tree of one system: nothing exports it
+    ++c->bh_builds;
+}
+
+void bh_batch_walk(nbhip_ctx* c, const BhBatchView& v, bool fuse, bool count)
+{
+    const int n = int(c->count);
+    if (n == 0 || v.nwaves == 0) return;
+    if (count) c->bh_visits.grow(size_t(n), 0, c->stream);
+    BatchWalkArgs a{};
+    a.nodes = c->bh_nodes.ptr;
+    a.spm = c->bh_spm.ptr;
+    a.perm = c->bh_vals[c->bh_sorted].ptr;
+    a.base = c->bh_base.ptr;
+    a.pm = c->pm[c->cur].ptr;
+    a.vr = c->vr.ptr;
+    a.acc = c->acc.ptr;
+    a.visits = c->bh_visits.ptr;
+    a.sys = v.sys;
+    a.waves = v.waves;
+    a.nwaves = v.nwaves;
+    a.n = n;
+    using Walk = void (*)(BatchWalkArgs);
+    const Walk k = count ? (fuse ? (Walk)k_bwalk<true, true> : (Walk)k_bwalk<false, true>)
+                         : (fuse ? (Walk)k_bwalk<true, false> : (Walk)k_bwalk<false, false>);
+    hipLaunchKernelGGL(k, dim3(unsigned((v.nwaves + 3) / 4)), dim3(256), 0, c->stream, a);
+    NBHIP_CHECK(hipGetLastError());
+    c->launches += 1;
 }
 
 template <int K>

@@ -110,6 +110,45 @@ struct BhNode {
 };
 static_assert(sizeof(BhNode) == sizeof(nbhip_bh_node), "BhNode must match nbhip_bh_node");
 
+// Barnes-Hut over many systems in one build: a system's bodies occupy positions
+// [off, off + n) of the batch's arrays, in input order and, after the sort, in key order;
+// its key is its index in the table above 18 levels of its own octree.
+constexpr int BH_BATCH_SYS_BITS = 9;
+constexpr int BH_BATCH_MAX_SYSTEMS = 1 << BH_BATCH_SYS_BITS;
+constexpr int BH_BATCH_ROOT_LEVEL = BH_BATCH_SYS_BITS / 3;  // key digits above a system's root
+constexpr int BH_BATCH_LEVELS = 21 - BH_BATCH_ROOT_LEVEL;    // octree levels per system
+
+struct BhSys {
+    int off;         // first position
+    int n;           // bodies
+    int first_item;  // its first scan tile in the item table
+    int tiles;       // its scan tiles of 1024 positions, counted from off
+    float size;      // root edge; 0: found from the bodies each step (then no wrap)
+    float theta;
+    int opening;
+    int leaf;        // bucket size
+    float G;
+    float eps2;
+    float dt;
+    int wrap;
+    int cut;         // radius cutoff law (eps == 0 or some radius > 0)
+    int pad[3];
+};
+static_assert(sizeof(BhSys) == 64, "BhSys is 64 bytes");
+
+// The device tables of a Barnes-Hut batch.
+struct BhBatchView {
+    const BhSys* sys;
+    int systems;
+    const int2* items;  // (system, tile) per 1024 positions of a system
+    int nitems;
+    const int2* waves;  // (system, first position) per 64 positions of a system
+    int nwaves;
+    float* roots;       // per system: the root edge found from the bodies
+    double* ext;        // per item: the FP64 extent of its bodies
+    bool any_auto;      // some system finds its root from the bodies
+};
+
 // One node in the reference's 48-byte octree layout (nbhip_tree_node).
 struct TreeNode {
     float4 cs;         // centre xyz, edge
@@ -253,6 +292,11 @@ void bh_check_fault(nbhip_ctx* c);
 // Sets the fault word from the host, as a look-back that gave up waiting would (tests only).
 void bh_raise_sort_fault(nbhip_ctx* c);
 size_t bh_tree(nbhip_ctx* c, const nbhip_params* p);
+// Many systems in c's body arrays (c->count bodies in all): buffers for them (before a
+// capture), the build, and the walk (fused: kick-drift too; count: per-body visits).
+void bh_batch_reserve(nbhip_ctx* c, const BhBatchView& v);
+void bh_batch_build(nbhip_ctx* c, const BhBatchView& v);
+void bh_batch_walk(nbhip_ctx* c, const BhBatchView& v, bool fuse, bool count);
 float bh_root_size(nbhip_ctx* c);
 void check_ics(const nbhip_ics* s, size_t n);
 bool ics_have_radius(const nbhip_ics& s);

@@ -642,8 +642,9 @@ void orc_bh_accelerate_f64com(float* bodies, size_t n, const orc_node* nodes, si
 /* Barnes-Hut with leaf buckets, stated on the reference's own tree.  No reference
  * counterpart: the reference subdivides every cell of two or more bodies
  * (source/bhtree.cpp:14-95).  The library's leaf_size option stops subdividing a cell
- * once it holds at most leaf_size bodies, or once it sits 21 levels below the root (the
- * depth of a 63-bit key): such a cell is a bucket.  The traversal is the reference's
+ * once it holds at most leaf_size bodies, or once it sits max_level levels below the root
+ * (21, the depth of a 63-bit key, for one system; 18 in a batch of systems, whose keys
+ * carry a system index above the octree's digits): such a cell is a bucket.  The traversal is the reference's
  * Tree::apply (source/bhtree.cpp:97-138) with one change: a bucket cell is tested as any
  * cell is, and when the test opens it the walk does not descend into its subcells but
  * applies each of its bodies one by one (every one counted as a visit, the receiving
@@ -658,12 +659,12 @@ void orc_bh_accelerate_f64com(float* bodies, size_t n, const orc_node* nodes, si
  * FP64 centre rounded to FP32 with the reference's FP32 operation order and edge
  * (bhtree.cpp:125-128; the textbook criterion when `textbook` is set, size^2 / theta^2 as
  * the threshold); applied forces are summed in FP64.  With leaf_size 1 and no two bodies
- * in one 21-level cell this is orc_bh_accelerate_f64com.  The tree must come from
- * orc_bh_build on the same bodies with a root edge of at least 2^21 FLT_EPSILON (~0.25),
- * so that the reference keeps splitting to 21 levels.  Returns 0, or -1 when out of
+ * in one max_level cell this is orc_bh_accelerate_f64com.  The tree must come from
+ * orc_bh_build on the same bodies with a root edge of at least 2^max_level FLT_EPSILON
+ * (~0.25 at 21 levels), so that the reference keeps splitting to max_level levels.  Returns 0, or -1 when out of
  * memory. */
 typedef struct {
-    float* b; const orc_node* nodes; size_t nn; float theta; int textbook, leaf;
+    float* b; const orc_node* nodes; size_t nn; float theta; int textbook, leaf, max_level;
     double G, eps2;
     const uint32_t* cnt;      /* bodies per cell */
     const double* mom;        /* per cell: m, m x, m y, m z */
@@ -722,7 +723,7 @@ static void orc_bhb_block(void* p, size_t begin, size_t end)
             }
             /* a bucket cell opens: down its chain while one subcell holds all its bodies */
             int chain = 0;
-            if (a->level[idx] < 21 && nd->children != 0)
+            if (a->level[idx] < a->max_level && nd->children != 0)
                 for (uint32_t q = 0; q < 8; ++q)
                     if (a->cnt[nd->children + q] == c)
                         chain = 1;
@@ -754,10 +755,10 @@ static void orc_bhb_block(void* p, size_t begin, size_t end)
 }
 
 int orc_bh_accelerate_buckets(float* bodies, size_t n, const orc_node* nodes, size_t nn, float theta,
-                              int textbook, int leaf_size, double G, double eps, uint64_t* visits,
-                              int nthreads)
+                              int textbook, int leaf_size, int max_level, double G, double eps,
+                              uint64_t* visits, int nthreads)
 {
-    if (nn == 0 || leaf_size < 1)
+    if (nn == 0 || leaf_size < 1 || max_level < 1 || max_level > 21)
         return -1;
     uint32_t* cnt = (uint32_t*)calloc(nn, sizeof(uint32_t));
     double* mom = (double*)calloc(4 * nn, sizeof(double));
@@ -792,15 +793,15 @@ int orc_bh_accelerate_buckets(float* bodies, size_t n, const orc_node* nodes, si
         }
     }
     /* each body's bucket: the first cell on its descent that stops subdividing (at most
-     * leaf_size bodies, 21 levels deep, or a leaf of the reference), if it holds two or
-     * more bodies */
+     * leaf_size bodies, max_level levels deep, or a leaf of the reference), if it holds two
+     * or more bodies */
     for (size_t k = 0; k < nn; ++k)
         list_of[k] = UINT32_MAX;
     size_t lists = 0;
     for (size_t i = 0; i < n; ++i) {
         const float* p = bodies + i * BF;
         uint32_t idx = 0;
-        while (!(cnt[idx] <= (uint32_t)leaf_size || level[idx] >= 21 || nodes[idx].children == 0))
+        while (!(cnt[idx] <= (uint32_t)leaf_size || level[idx] >= max_level || nodes[idx].children == 0))
             idx = nodes[idx].children + orc_quadrant(&nodes[idx], p);
         root_of[i] = cnt[idx] >= 2 ? idx : UINT32_MAX;
         if (cnt[idx] >= 2 && list_of[idx] == UINT32_MAX)
@@ -829,7 +830,7 @@ int orc_bh_accelerate_buckets(float* bodies, size_t n, const orc_node* nodes, si
         if (list_of[k] == UINT32_MAX)
             continue;
         uint32_t idx = (uint32_t)k;
-        while (level[idx] < 21 && nodes[idx].children != 0) {
+        while (level[idx] < max_level && nodes[idx].children != 0) {
             uint32_t only = UINT32_MAX;
             for (uint32_t q = 0; q < 8; ++q)
                 if (cnt[nodes[idx].children + q] == cnt[idx])
@@ -841,8 +842,8 @@ int orc_bh_accelerate_buckets(float* bodies, size_t n, const orc_node* nodes, si
         }
     }
     {
-        orc_bhb_arg a = { bodies, nodes, nn, theta, textbook, leaf_size, G, eps * eps, cnt, mom, level,
-                          list_of, lstart, lbody, visits ? visits : own_visits };
+        orc_bhb_arg a = { bodies, nodes, nn, theta, textbook, leaf_size, max_level, G, eps * eps, cnt, mom,
+                          level, list_of, lstart, lbody, visits ? visits : own_visits };
         orc_parallel_blocks(n, nthreads, orc_bhb_block, &a);
     }
     rc = 0;

@@ -91,8 +91,8 @@ def lib():
     L.orc_bh_accelerate_f64com.argtypes = [_f32p, sz, _nodep, sz, C.c_float, C.c_double,
                                            C.c_double, C.c_void_p, C.c_int]
     L.orc_bh_accelerate_buckets.restype = C.c_int
-    L.orc_bh_accelerate_buckets.argtypes = [_f32p, sz, _nodep, sz, C.c_float, C.c_int, C.c_int, C.c_double,
-                                            C.c_double, _u64p, C.c_int]
+    L.orc_bh_accelerate_buckets.argtypes = [_f32p, sz, _nodep, sz, C.c_float, C.c_int, C.c_int, C.c_int,
+                                            C.c_double, C.c_double, _u64p, C.c_int]
     L.orc_node_size.restype = sz
     L.orc_philox4x64.argtypes = [_u64p, _u64p, _u64p]
     L.orc_generate.argtypes = [C.c_void_p, sz, _f32p, C.c_int]
@@ -245,17 +245,18 @@ def bh_accelerate_f64com(bodies: np.ndarray, size: float, theta=0.5, G=1.0, eps=
 
 
 def bh_accelerate_buckets(bodies: np.ndarray, size: float, theta=0.5, leaf_size=1, G=1.0, eps=0.0,
-                          textbook=False, threads=0, nodes=None):
+                          textbook=False, threads=0, nodes=None, max_level=21):
     """The reference tree and traversal with leaf buckets of at most `leaf_size` bodies
-    (and 21-level cells): a bucket is tested whole and, when opened, applied body by body.
-    FP64 centres and sums; the acceptance test in FP32.  Accelerations in place; returns
-    (bodies, per-body visit counts)."""
+    (and cells max_level levels deep: 21 for one system, 18 in a batch of systems): a
+    bucket is tested whole and, when opened, applied body by body.  FP64 centres and sums;
+    the acceptance test in FP32.  Accelerations in place; returns (bodies, per-body visit
+    counts)."""
     b = _bodies(bodies)
     if nodes is None:
         nodes = bh_build(b, size)
     v = np.zeros(b.shape[0], np.uint64)
     if lib().orc_bh_accelerate_buckets(b, b.shape[0], nodes, nodes.size, theta, int(bool(textbook)),
-                                       int(leaf_size), G, eps, v, _t(threads)) != 0:
+                                       int(leaf_size), int(max_level), G, eps, v, _t(threads)) != 0:
         raise MemoryError("orc_bh_accelerate_buckets")
     return b, v
 

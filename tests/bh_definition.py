@@ -9,12 +9,13 @@ prefixes, so an agreement between any two of them is a check, not a restatement.
 import numpy as np
 
 
-def bucket_tree(keys, spm, size, theta, leaf, textbook=False):
+def bucket_tree(keys, spm, size, theta, leaf, textbook=False, levels=21):
     """The leaf-bucket octree written from its definition, recursively over key prefixes:
     a cell of two or more bodies is one node at the deepest level its bodies share (chains
-    of one-child cells collapse); it is a bucket when it holds at most `leaf` bodies or sits
-    at the 21st level, else an internal node whose children follow in octant order, a
-    one-body child being a leaf entry.  Nodes in pre-order with skip pointers; `spm` holds
+    of one-child cells collapse), at most `levels` (21 for one system, 18 in a batch); it is
+    a bucket when it holds at most `leaf` bodies or sits at that deepest level, else an
+    internal node whose children follow in octant order, a one-body child being a leaf
+    entry.  Nodes in pre-order with skip pointers; `spm` holds
     the bodies in key order (x, y, z, mass).  Rows: first, count, next, open_sq, com xyz,
     mass (count 0 = internal node)."""
     K = [int(k) for k in keys]
@@ -37,12 +38,12 @@ def bucket_tree(keys, spm, size, theta, leaf, textbook=False):
         rows.append([a, 1, len(rows) + 1, np.float32(0.0), *spm[a, :3], spm[a, 3]])
 
     def cell(s, e):
-        m = cpl(s, e - 1)
+        m = min(cpl(s, e - 1), levels)
         me = len(rows)
         w = spm[s:e, 3].astype(np.float64)
         M = w.sum()
         com = (spm[s:e, :3].astype(np.float64) * w[:, None]).sum(0) / M if M != 0 else np.zeros(3)
-        bucket = e - s <= leaf or m == 21
+        bucket = e - s <= leaf or m == levels
         rows.append([s, e - s if bucket else 0, 0, open_sq(m), *com, M])
         if not bucket:
             shift = 60 - 3 * m

@@ -309,6 +309,51 @@ def test_bucket_oracle_matches_definition(orc, leaf, kind, textbook):
         assert not np.array_equal(v1, v)
 
 
+def _close_groups(n, size, seed, k=6):
+    # groups of k bodies 2^-21 .. 2^-18 root edges from the first of the group: many groups
+    # share a cell 18 levels deep and split further down by 21 levels
+    rng = np.random.default_rng(seed)
+    b = ics.plummer(n, seed=seed)
+    lead = (np.arange(n) // k) * k
+    d = rng.normal(size=(n, 3)) * (size * 2.0 ** rng.uniform(-21, -18, (n, 1)))
+    d[lead == np.arange(n)] = 0.0
+    b[:, 0:3] = (b[lead, 0:3] + d).astype(np.float32)
+    return b
+
+
+@pytest.mark.parametrize("leaf", [1, 2, 16])
+def test_bucket_oracle_18_levels_matches_definition(orc, leaf):
+    # the oracle's 18-level buckets (a batch of systems) against the definition's tree cut
+    # at 18 levels: same visits body for body, forces equal to FP64 rounding; and the input
+    # does put several bodies in 18-level cells that 21 levels would split, which changes
+    # the visits
+    from bh_definition import bucket_tree, walk
+    n, size, theta = 3000, 64.0, 0.7
+    b = _close_groups(n, size, 40 + leaf, k=6 if leaf < 4 else 3 * leaf)   # groups above the bucket size
+    keys = orc.morton_keys(b, size, 21)
+    k18 = keys >> np.uint64(9)
+    assert np.unique(k18).size < np.unique(keys).size - 100
+    x = b.copy()
+    _, v = orc.bh_accelerate_buckets(x, size, theta=theta, leaf_size=leaf, eps=0.01, max_level=18)
+    perm = orc.stable_sort(keys)
+    spm = b[perm][:, [0, 1, 2, 7]]
+    rows = bucket_tree(keys[perm], spm, size, theta, leaf, levels=18)
+    acc, dv = walk(rows, spm, eps=0.01)
+    assert np.array_equal(v[perm], dv)
+    a = x[perm, 8:11].astype(np.float64)
+    rel = np.linalg.norm(a - acc, axis=1) / np.linalg.norm(acc, axis=1)
+    assert rel.max() <= 2e-7
+    _, v21 = orc.bh_accelerate_buckets(b.copy(), size, theta=theta, leaf_size=leaf, eps=0.01)
+    assert not np.array_equal(v21, v)
+
+
+def test_bucket_oracle_refuses_bad_levels(orc):
+    b = ics.plummer(100, seed=1)
+    for lv in (0, 22):
+        with pytest.raises(MemoryError):
+            orc.bh_accelerate_buckets(b.copy(), 64.0, leaf_size=4, max_level=lv)
+
+
 def test_bucket_oracle_catches_a_shallow_threshold(orc):
     # a planted mistake in the definition's walk -- a bucket tested at the edge of the
     # shallowest cell of its chain instead of the deepest -- must change the visits on
