@@ -523,6 +523,16 @@ def run_served(args) -> dict:
     # the same arrivals under one first-come-first-served queue (the earlier policy)
     with serve.Server(hbm_budget=200e9, admission="fcfs") as srv:
         fcfs = srv.run(jobs).summary()
+    # the Barnes-Hut jobs each on a context of a pool of 8 instead of in the Barnes-Hut
+    # batch: saturated, then the same arrivals
+    arrivals = [j.arrival for j in jobs]
+    with serve.Server(hbm_budget=200e9, bh_mode="contexts") as srv:
+        for j in jobs:
+            j.arrival = 0.0
+        ctx_sat = srv.run(jobs)
+        for j, a in zip(jobs, arrivals):
+            j.arrival = a
+        ctx_rep = srv.run(jobs).summary()
     # the same jobs one at a time through the plain single-system path; the Barnes-Hut jobs
     # take the leaf sizes the server's autotuner picked
     for j in jobs:
@@ -534,9 +544,10 @@ def run_served(args) -> dict:
     return {
         "workload": f"{count} jobs: Plummer or disk+bulge, N log-uniform in [1024, 262144], steps "
                     "uniform in [10, 1000]; direct sum below 16384 bodies (batched), Barnes-Hut "
-                    "theta 0.5 above, leaf size autotuned per power-of-two size class (a context "
-                    "each, 8 concurrent, chunks of 16 steps replayed from captured graphs); bodies "
-                    "drawn on the device at admission; "
+                    "theta 0.5 above (batched: one sort, build and walk per step for all of them, "
+                    "on a stream of its own), leaf size autotuned per power-of-two size class, "
+                    "chunks of 16 steps replayed from captured graphs; bodies drawn on the device "
+                    "at admission; "
                     "200 GB budget; "
                     f"Poisson arrivals at 0.7x the measured capacity ({rate:.1f} jobs/s)",
         "jobs": s["jobs"], "direct_jobs": n_direct, "barnes_hut_jobs": count - n_direct,
@@ -553,6 +564,12 @@ def run_served(args) -> dict:
         "admission": "per-class queues (direct sum, Barnes-Hut), budget in arrival order",
         "fcfs_same_arrivals": {"steps_per_s": fcfs["steps_per_s"], "latency_p50_s": fcfs["latency_p50_s"],
                                "latency_p99_s": fcfs["latency_p99_s"]},
+        "bh_contexts_same_arrivals": {"steps_per_s": ctx_rep["steps_per_s"],
+                                      "latency_p50_s": ctx_rep["latency_p50_s"],
+                                      "latency_p99_s": ctx_rep["latency_p99_s"],
+                                      "saturated_steps_per_s": ctx_sat.job_steps / ctx_sat.wall_s,
+                                      "what": "Barnes-Hut jobs each on a context of a pool of 8 "
+                                              "(no Barnes-Hut batch), same jobs and arrivals"},
         "sequential_steps_per_s": seq.job_steps / seq.wall_s,
         "sequential_wall_s": seq.wall_s,
         "speedup_vs_sequential": (sat.job_steps / sat.wall_s) / (seq.job_steps / seq.wall_s),

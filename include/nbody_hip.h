@@ -532,6 +532,10 @@ int nbhip_bh_batch_visits(nbhip_bh_batch* b, int id, uint32_t* visits, size_t n)
 
 int nbhip_bh_batch_query(nbhip_bh_batch* b, nbhip_bh_batch_info* out);
 
+/* `steps` fused steps of every system, eager, with HIP events on the batch's stream around
+ * each step's build and its walk; *build_ms and *walk_ms receive the means.  Synchronises. */
+int nbhip_bh_batch_profile(nbhip_bh_batch* b, int steps, double* build_ms, double* walk_ms);
+
 /* ---- timing: HIP events on the context's stream ----------------------------------- */
 
 /* Record a start event on the context's stream. */

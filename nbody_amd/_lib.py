@@ -167,6 +167,7 @@ SIGNATURES = {
     "nbhip_bh_batch_download": (C.c_int, [_vp, C.c_int, _vp, _sz]),
     "nbhip_bh_batch_visits": (C.c_int, [_vp, C.c_int, _vp, _sz]),
     "nbhip_bh_batch_query": (C.c_int, [_vp, C.POINTER(BhBatchInfo)]),
+    "nbhip_bh_batch_profile": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "nbhip_timer_start": (C.c_int, [_vp]),
     "nbhip_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_long)]),
 }
@@ -590,3 +591,9 @@ class BhBatch:
         out = BhBatchInfo()
         self.ctx._check(self.L.nbhip_bh_batch_query(self.h, C.byref(out)))
         return out
+
+    def profile(self, steps: int) -> tuple[float, float]:
+        """`steps` fused steps timed by HIP events: mean (build_ms, walk_ms) per step."""
+        b, w = C.c_double(), C.c_double()
+        self.ctx._check(self.L.nbhip_bh_batch_profile(self.h, steps, C.byref(b), C.byref(w)))
+        return b.value, w.value

@@ -427,6 +427,49 @@ int nbhip_bh_batch_visits(nbhip_bh_batch* b, int id, uint32_t* visits, size_t n)
     });
 }
 
+int nbhip_bh_batch_profile(nbhip_bh_batch* b, int steps, double* build_ms, double* walk_ms)
+{
+    return nbhip::bh_batch_guarded(b, [&] {
+        if (steps < 1) nbhip::fail(NBHIP_ERR_INVALID, "at least one step");
+        if (!build_ms || !walk_ms) nbhip::fail(NBHIP_ERR_INVALID, "null output");
+        *build_ms = *walk_ms = 0.0;
+        if (b->systems.empty()) return;
+        nbhip_ctx* w = b->work;
+        hipStream_t s = w->stream;
+        nbhip::rebuild_tables(b);
+        const nbhip::BhBatchView v = nbhip::view(b);
+        nbhip::bh_batch_reserve(w, v);
+        std::vector<hipEvent_t> ev(3 * size_t(steps), nullptr);
+        try {
+            for (auto& e : ev) NBHIP_CHECK(hipEventCreate(&e));
+            for (int r = 0; r < steps; ++r) {
+                NBHIP_CHECK(hipEventRecord(ev[3 * r], s));
+                nbhip::bh_batch_build(w, v);
+                NBHIP_CHECK(hipEventRecord(ev[3 * r + 1], s));
+                nbhip::bh_batch_walk(w, v, true, false);
+                NBHIP_CHECK(hipEventRecord(ev[3 * r + 2], s));
+            }
+            NBHIP_CHECK(hipEventSynchronize(ev.back()));
+            double tb = 0, tw = 0;
+            for (int r = 0; r < steps; ++r) {
+                float t0 = 0.f, t1 = 0.f;
+                NBHIP_CHECK(hipEventElapsedTime(&t0, ev[3 * r], ev[3 * r + 1]));
+                NBHIP_CHECK(hipEventElapsedTime(&t1, ev[3 * r + 1], ev[3 * r + 2]));
+                tb += t0;
+                tw += t1;
+            }
+            *build_ms = tb / steps;
+            *walk_ms = tw / steps;
+        } catch (...) {
+            for (auto e : ev)
+                if (e) (void)hipEventDestroy(e);
+            throw;
+        }
+        for (auto e : ev) NBHIP_CHECK(hipEventDestroy(e));
+        nbhip::bh_check_fault(w);
+    });
+}
+
 int nbhip_bh_batch_query(nbhip_bh_batch* b, nbhip_bh_batch_info* out)
 {
     return nbhip::bh_batch_guarded(b, [&] {

@@ -4,8 +4,10 @@ Jobs arrive over time (each: bodies, a step count, a mode).  The server keeps ev
 admitted job's state on the device until its last step, and runs:
   * direct-sum jobs (N below `direct_below`) together in one nbhip_batch -- one launch per
     step for all of them, with jobs joining and leaving between chunks of steps;
-  * Barnes-Hut jobs each on their own context from a fixed pool, so their streams run
-    concurrently with the batch and with each other.
+  * Barnes-Hut jobs together in one nbhip_bh_batch (bh_mode "batch") -- one radix sort, one
+    pass of each build kernel and one walk per step for all of them, on the batch's own
+    stream, concurrent with the direct-sum batch, joining and leaving between chunks; or
+    (bh_mode "contexts") each on its own context from a fixed pool.
 Admission keeps one first-come-first-served queue per resource class (direct sum, Barnes-Hut):
 the device-memory budget (bytes per body estimated per mode) is granted in arrival order
 across both, but a class whose own resource is busy (the batch between chunk boundaries,
@@ -17,8 +19,9 @@ any job it holds, and is polled without blocking (nbhip_query), so a resource th
 finishes early is refilled while the others still run.
 
 A job's result does not depend on what else is running: a batched system sums each
-body's force in a fixed order whatever shares the batch, and a Barnes-Hut job owns its
-context.  Latency is measured from a job's arrival to its result being back on the host.
+body's force in a fixed order whatever shares the batch, and a Barnes-Hut system's sort
+order, prefix sums, nodes and walk are its own in the Barnes-Hut batch (a pooled job owns
+its context).  Latency is measured from a job's arrival to its result being back on the host.
 """
 from __future__ import annotations
 
@@ -32,7 +35,9 @@ from . import _lib
 
 # Device bytes per body, for admission: state arrays (pm x2, vr, acc: 64 B) for a
 # batched job; for a Barnes-Hut job also keys, sorted copies, prefix sums and up to 2N
-# nodes (about 170 B more) and the FP64 scratch of bodies drawn on the device (48 B).
+# nodes (about 170 B more) and the FP64 scratch of bodies drawn on the device (48 B) --
+# in the Barnes-Hut batch the spare arrays a removal compacts into take the place of the
+# second position buffer and the generator's scratch.
 BYTES_PER_BODY = {_lib.MODE_N2: 64, _lib.MODE_NLOGN: 288}
 
 
@@ -107,8 +112,12 @@ class Server:
     def __init__(self, hbm_budget: float = 200e9, bh_contexts: int = 8, chunk: int = 16,
                  direct_below: int = 16384, keep_results: bool = False, batch_shape=None,
                  bh_graph: bool = True, bh_leaf="auto", admission: str = "per_class",
-                 count_bh_interactions: bool = False, final_forces: bool = False):
+                 count_bh_interactions: bool = False, final_forces: bool = False,
+                 bh_mode: str = "batch"):
         self.budget = hbm_budget
+        if bh_mode not in ("batch", "contexts"):
+            raise ValueError(bh_mode)
+        self.bh_mode = bh_mode
         # end each job with one accelerate-only pass, so its result carries the forces at
         # its final positions (a fused step leaves those of the positions before it); for
         # checking served results, it costs a force pass per job
@@ -129,8 +138,9 @@ class Server:
         self.bh_leaf = bh_leaf
         self.leaf_by_class: dict[int, int] = {}
         self.chunk = chunk
-        # Barnes-Hut chunks replayed from a captured graph (nbhip_step_graph): one launch per
-        # chunk instead of ~20 per step; captured again when a chunk's length changes.
+        # Barnes-Hut chunks replayed from a captured graph (nbhip_step_graph, or the batch's
+        # NBHIP_BATCH_GRAPH): one launch per chunk instead of ~20 per step; captured again
+        # when a chunk's length or the batch's membership changes.
         self.bh_graph = bh_graph
         self.direct_below = direct_below
         self.keep_results = keep_results
@@ -138,11 +148,18 @@ class Server:
         self.batch = _lib.Batch(self.batch_ctx)
         if batch_shape is not None:
             self.batch.configure(*batch_shape)
-        self.slots = [_BhSlot() for _ in range(bh_contexts)]
+        self.slots = [_BhSlot() for _ in range(bh_contexts)] if bh_mode == "contexts" else []
+        # the Barnes-Hut batch (its own stream), and a context for the leaf tuner and the
+        # interaction counts, which run on one job's bodies
+        self.bh_ctx = _lib.Context(0)
+        self.bh = _lib.BhBatch(self.bh_ctx) if bh_mode == "batch" else None
 
     def close(self) -> None:
         self.batch.close()
         self.batch_ctx.close()
+        if self.bh is not None:
+            self.bh.close()
+        self.bh_ctx.close()
         for s in self.slots:
             s.ctx.close()
 
@@ -152,15 +169,34 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
-    def _leaf_for(self, job: Job, ctx: _lib.Context) -> int:
+    def _leaf_for(self, job: Job, ctx: _lib.Context | None) -> int:
+        """The job's bucket size; ctx holds its bodies (None: load them into the tuning
+        context when the tuner needs them)."""
         if self.bh_leaf != "auto":
             return int(self.bh_leaf)
         cls = max(0, int(job.n).bit_length() - 1)
         if cls not in self.leaf_by_class:
+            if ctx is None:
+                ctx = self.bh_ctx
+                self._load(ctx, job)
             # the accelerations the tuner leaves behind are recomputed by the job's first step
             _, _, best = ctx.bh_tune(job.params)
             self.leaf_by_class[cls] = best
         return self.leaf_by_class[cls]
+
+    @staticmethod
+    def _load(ctx: _lib.Context, job: Job) -> None:
+        if job.bodies is not None:
+            ctx.upload_bodies(job.bodies)
+        else:
+            ctx.generate(job.ics, job.n)
+
+    def _count(self, job: Job, result: np.ndarray) -> float:
+        """Body-node interactions of one force pass at the job's final positions."""
+        self.bh_ctx.upload_bodies(result)
+        _, inter = self.bh_ctx.bh_interactions(job.params)
+        job.interactions_per_step = float(inter)
+        return job.steps * job.interactions_per_step
 
     # ---- the loop ------------------------------------------------------------------------
     def run(self, jobs: list[Job], clock=time.perf_counter) -> Report:
@@ -174,6 +210,8 @@ class Server:
         waiting: deque[Job] = deque()
         direct: dict[int, tuple[Job, int]] = {}   # batch system id -> (job, steps remaining)
         batch_inflight = 0
+        bh_jobs: dict[int, tuple[Job, int]] = {}  # Barnes-Hut batch system id -> (job, remaining)
+        bh_inflight = 0
         in_use = 0
         peak = 0
         max_conc = 0
@@ -218,6 +256,26 @@ class Server:
                         direct[sid] = (job, rem)
                 batch_inflight = 0
 
+            # -- the Barnes-Hut batch: the same, on its own stream --------------------------
+            if bh_inflight and self.bh.idle():
+                if self.final_forces and any(rem <= bh_inflight for _, rem in bh_jobs.values()):
+                    self.bh.step(1, _lib.BATCH_ACCELERATE)
+                for sid in list(bh_jobs):
+                    job, rem = bh_jobs[sid]
+                    rem -= bh_inflight
+                    job_steps += bh_inflight
+                    body_steps += bh_inflight * job.n
+                    if rem <= 0:
+                        res = self.bh.download(sid)
+                        self.bh.remove(sid)
+                        del bh_jobs[sid]
+                        finish(job, res)
+                        if self.count_bh_interactions:
+                            interactions += self._count(job, res)
+                    else:
+                        bh_jobs[sid] = (job, rem)
+                bh_inflight = 0
+
             # -- admission: per-class queues, the budget granted in arrival order --------
             # A job whose class resource is busy keeps its place for the budget (the bytes
             # of such jobs ahead are held back) but does not stop later jobs of the other
@@ -233,9 +291,11 @@ class Server:
                     continue
                 if in_use + reserved + job.device_bytes() > self.budget:
                     break
+                slot = None
                 if job.mode == _lib.MODE_N2:
                     busy = batch_inflight > 0       # joins at the next chunk boundary
-                    slot = None
+                elif self.bh is not None:
+                    busy = bh_inflight > 0 or len(bh_jobs) >= _lib.BH_BATCH_MAX_SYSTEMS
                 else:
                     slot = next((s for s in self.slots if s.job is None), None)
                     busy = slot is None
@@ -251,11 +311,13 @@ class Server:
                            else self.batch.add_generated(job.ics, job.n, job.params))
                     direct[sid] = (job, job.steps)
                     job.interactions_per_step = float(job.n) ** 2
+                elif self.bh is not None:
+                    job.params.leaf_size = self._leaf_for(job, None)
+                    sid = (self.bh.add(job.bodies, job.params) if job.bodies is not None
+                           else self.bh.add_generated(job.ics, job.n, job.params))
+                    bh_jobs[sid] = (job, job.steps)
                 else:
-                    if job.bodies is not None:
-                        slot.ctx.upload_bodies(job.bodies)
-                    else:
-                        slot.ctx.generate(job.ics, job.n)
+                    self._load(slot.ctx, job)
                     job.params.leaf_size = self._leaf_for(job, slot.ctx)
                     slot.job, slot.remaining, slot.inflight = job, job.steps, 0
                 del waiting[i]
@@ -267,6 +329,10 @@ class Server:
                 k = min(self.chunk, min(rem for _, rem in direct.values()))
                 self.batch.step(k)
                 batch_inflight = k
+            if bh_jobs and not bh_inflight:
+                k = min(self.chunk, min(rem for _, rem in bh_jobs.values()))
+                self.bh.step(k, _lib.BATCH_GRAPH if self.bh_graph else 0)
+                bh_inflight = k
 
             # -- Barnes-Hut slots ---------------------------------------------------------
             for s in self.slots:
@@ -284,9 +350,7 @@ class Server:
                         job, s.job = s.job, None
                         finish(job, res)
                         if self.count_bh_interactions:
-                            _, inter = s.ctx.bh_interactions(job.params)   # at the final positions
-                            job.interactions_per_step = float(inter)
-                            interactions += job.steps * job.interactions_per_step
+                            interactions += self._count(job, res)   # at the final positions
                         continue
                 if not s.inflight:
                     k = min(self.chunk, s.remaining)
@@ -296,8 +360,8 @@ class Server:
                         s.ctx.step(s.job.params, k)
                     s.inflight = k
 
-            max_conc = max(max_conc, len(direct) + sum(1 for s in self.slots if s.job is not None))
-            if not waiting and pending and not direct and all(s.job is None for s in self.slots):
+            max_conc = max(max_conc, len(direct) + len(bh_jobs) + sum(1 for s in self.slots if s.job is not None))
+            if not waiting and pending and not direct and not bh_jobs and all(s.job is None for s in self.slots):
                 # nothing on the device: wait for the next arrival
                 time.sleep(max(0.0, min(pending[0].arrival - now(), 0.01)))
             else:

@@ -256,3 +256,15 @@ def test_errors(ctx):
         batch.remove(i)
         assert batch.info().systems == 0 and batch.info().bodies == 0
         batch.step(2)
+
+
+def test_profiled_steps_equal_plain_steps(ctx):
+    sys_ = systems()[:3]
+    with _lib.BhBatch(ctx) as a, _lib.BhBatch(ctx) as b:
+        ia = [a.add(x, p) for _, x, p in sys_]
+        ib = [b.add(x, p) for _, x, p in sys_]
+        build_ms, walk_ms = a.profile(3)
+        b.step(3)
+        assert build_ms > 0.0 and walk_ms > 0.0
+        for x, y in zip(ia, ib):
+            assert np.array_equal(a.download(x), b.download(y))

@@ -1,6 +1,7 @@
 """Host-side pieces of the server that need no device: the workload generator and the
 arrival process (nbody_amd/serve.py)."""
 import numpy as np
+import pytest
 
 from nbody_amd import _lib, serve
 
@@ -104,36 +105,59 @@ class _FakeBatch:
         pass
 
 
-def _admission_order(monkeypatch, policy):
+class _FakeBhBatch(_FakeBatch):
+    """The Barnes-Hut batch runs on its own stream: its chunk stays in flight for k polls."""
+
+    def __init__(self, ctx):
+        super().__init__(ctx)
+        self.busy = 0
+
+    def step(self, k, flags=0):
+        self.busy = k
+
+    def idle(self):
+        if self.busy > 0:
+            self.busy -= 1
+            return False
+        return True
+
+
+def _fake_device(monkeypatch):
     monkeypatch.setattr(_lib, "Context", _FakeCtx)
     monkeypatch.setattr(_lib, "Batch", _FakeBatch)
+    monkeypatch.setattr(_lib, "BhBatch", _FakeBhBatch)
+
+
+def _admission_order(monkeypatch, policy, bh_mode):
+    _fake_device(monkeypatch)
     jobs = serve.make_jobs(3, seed=1, n_range=(1024, 1025), steps_range=(64, 64), device_ics=True)
     d1, d2, b1 = jobs
     b1.params = _lib.params(mode=_lib.MODE_NLOGN, size=128.0, dt=1e-3, eps=0.01)
     b1.n_bodies = 20000
     d1.arrival, d2.arrival, b1.arrival = 0.0, 0.05, 0.06   # d2 and b1 arrive while d1's chunk runs
     ticks = iter(range(10 ** 7))
-    with serve.Server(chunk=64, bh_contexts=1, admission=policy) as srv:
+    with serve.Server(chunk=64, bh_contexts=1, admission=policy, bh_mode=bh_mode) as srv:
         rep = srv.run(jobs, clock=lambda: next(ticks) * 1e-3)
     assert rep.jobs == 3
     return d1, d2, b1
 
 
-def test_per_class_admission_does_not_block_across_classes(monkeypatch):
+@pytest.mark.parametrize("bh_mode", ["batch", "contexts"])
+def test_per_class_admission_does_not_block_across_classes(monkeypatch, bh_mode):
     # the batch is busy with d1 when d2 (direct) and b1 (Barnes-Hut) arrive: per class,
-    # b1 takes the free context at once while d2 waits for the chunk boundary; under one
-    # FCFS queue d2 at the head holds b1 back until then
-    d1, d2, b1 = _admission_order(monkeypatch, "per_class")
+    # b1 joins the idle Barnes-Hut batch (or takes the free context) at once while d2 waits
+    # for the chunk boundary; under one FCFS queue d2 at the head holds b1 back until then
+    d1, d2, b1 = _admission_order(monkeypatch, "per_class", bh_mode)
     assert b1.admitted < d2.admitted
-    d1, d2, b1 = _admission_order(monkeypatch, "fcfs")
+    d1, d2, b1 = _admission_order(monkeypatch, "fcfs", bh_mode)
     assert b1.admitted >= d2.admitted
 
 
-def test_budget_held_in_arrival_order(monkeypatch):
+@pytest.mark.parametrize("bh_mode", ["batch", "contexts"])
+def test_budget_held_in_arrival_order(monkeypatch, bh_mode):
     # a direct job blocked by the busy batch keeps its place for the budget: a later
     # Barnes-Hut job that would only fit by taking that place waits
-    monkeypatch.setattr(_lib, "Context", _FakeCtx)
-    monkeypatch.setattr(_lib, "Batch", _FakeBatch)
+    _fake_device(monkeypatch)
     jobs = serve.make_jobs(3, seed=1, n_range=(1024, 1025), steps_range=(64, 64), device_ics=True)
     d1, d2, b1 = jobs
     b1.params = _lib.params(mode=_lib.MODE_NLOGN, size=128.0, dt=1e-3, eps=0.01)
@@ -141,6 +165,6 @@ def test_budget_held_in_arrival_order(monkeypatch):
     d1.arrival, d2.arrival, b1.arrival = 0.0, 0.05, 0.06
     budget = d1.device_bytes() + d2.device_bytes() + b1.device_bytes() - 1
     ticks = iter(range(10 ** 7))
-    with serve.Server(hbm_budget=budget, chunk=64, bh_contexts=1) as srv:
+    with serve.Server(hbm_budget=budget, chunk=64, bh_contexts=1, bh_mode=bh_mode) as srv:
         srv.run(jobs, clock=lambda: next(ticks) * 1e-3)
     assert b1.admitted > d2.admitted
