@@ -381,6 +381,10 @@ constexpr unsigned LB_COUNT = 1u << 30, LB_TOTAL = 1u << 31, LB_VALUE = LB_COUNT
 // wrong but every block still finishes.
 constexpr unsigned LB_SPIN_LIMIT = 1u << 22;
 
+// The keys of one pass's digit, counted per block in LDS.  Consecutive lanes hold
+// consecutive keys, which after the first Barnes-Hut step are nearly sorted, so the high
+// digits come in long runs: the first lane of each run of equal digits adds the run's
+// length, instead of every lane adding one to the same counter.
 template <int RB>
 __global__ __launch_bounds__(RB) void k_rs_count(const unsigned long long* keys, int n, unsigned* counts,
                                                  unsigned* status, size_t status_words)
@@ -389,10 +393,24 @@ __global__ __launch_bounds__(RB) void k_rs_count(const unsigned long long* keys,
     for (int i = threadIdx.x; i < RS_PASSES * 256; i += RB) (&h[0][0])[i] = 0u;
     __syncthreads();
     const int stride = gridDim.x * RB;
-    for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += stride) {
-        const unsigned long long key = keys[k];
+    const int lane = int(threadIdx.x & 63);
+    // every lane of a wave takes the same trip count, so the shuffles see the whole wave
+    const int first = blockIdx.x * RB + int(threadIdx.x) - lane;
+    for (int w = first; w < n; w += stride) {
+        const int k = w + lane;
+        const bool valid = k < n;
+        const unsigned long long key = valid ? keys[k] : 0ull;
 #pragma unroll
-        for (int p = 0; p < RS_PASSES; ++p) atomicAdd(&h[p][unsigned(key >> (8 * p)) & 255u], 1u);
+        for (int p = 0; p < RS_PASSES; ++p) {
+            // invalid lanes (past n, at the wave's end) carry 256: never equal to a digit
+            const unsigned d = valid ? unsigned(key >> (8 * p)) & 255u : 256u;
+            const unsigned prev = unsigned(__shfl_up(int(d), 1));
+            const bool head = lane == 0 || d != prev;
+            const unsigned long long heads = __ballot(head);
+            const unsigned long long after = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+            const int next = after ? __ffsll((long long)after) - 1 : 64;
+            if (head && valid) atomicAdd(&h[p][d], unsigned(next - lane));
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < RS_PASSES * 256; i += RB) {
@@ -701,9 +719,12 @@ int radix_sort(nbhip_ctx* c, int n, int src)
 }
 
 // ---- 3. bodies in key order + FP64 moments -------------------------------------------------
-// Each block gathers one scan tile (coalesced: position tile + i * SC_BLOCK + thread) and
-// writes the tile's moment total, so the prefix scan that follows needs only its apply
-// phase, which computes the moments again from the gathered bodies (the same doubles).
+// Each block gathers one scan tile (coalesced: position tile + i * SC_BLOCK + thread) of
+// positions, velocities and the caller's indices, and writes the tile's moment total, so
+// the prefix scan that follows needs only its apply phase, which computes the moments again
+// from the gathered bodies (the same doubles).  The walk then writes every body back at its
+// key-order position, so from the second step on the bodies already sit near their sorted
+// places and this gather reads nearly sequentially.
 //
 // A tile covers positions [origin, origin + SC_TILE) of a run of positions ending at `end`
 // (one system); its total goes to part[slot], the run's tiles having slots first .. slot.
@@ -741,35 +762,55 @@ __device__ __forceinline__ Dbl4 moment(const float4 p)
     return Dbl4{ m, m * double(p.x), m * double(p.y), m * double(p.z) };
 }
 
+struct GatherArgs {
+    const unsigned* perm;   // key order -> position
+    const float4* pm;
+    const float4* vr;
+    const unsigned* order;  // the caller's index at each position
+    int n;
+    float4* spm;            // out, key order
+    float4* svr;
+    unsigned* sorder;
+    Dbl4* part;
+};
+
 template <typename Runs>
-__global__ __launch_bounds__(SC_BLOCK) void k_gather(const unsigned* perm, const float4* pm, int n, float4* spm,
-                                                     Dbl4* part, Runs runs)
+__global__ __launch_bounds__(SC_BLOCK) void k_gather(GatherArgs g, Runs runs)
 {
     __shared__ Dbl4 sh[SC_BLOCK];
     const ScanTile tt = runs.tile();
     const int base = tt.origin + int(threadIdx.x);
+    const int n = g.n;
     // every index load, then every body load, in flight together
     unsigned j[SC_IPT];
 #pragma unroll
     for (int i = 0; i < SC_IPT; ++i) {
         const int k = base + i * SC_BLOCK;
-        j[i] = k < tt.end ? perm[k] : unsigned(n);
+        j[i] = k < tt.end ? g.perm[k] : unsigned(n);
     }
-    float4 p[SC_IPT];
+    float4 p[SC_IPT], v[SC_IPT];
+    unsigned o[SC_IPT];
 #pragma unroll
-    for (int i = 0; i < SC_IPT; ++i) p[i] = j[i] < unsigned(n) ? pm[j[i]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int i = 0; i < SC_IPT; ++i) {
+        const bool in = j[i] < unsigned(n);
+        p[i] = in ? g.pm[j[i]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        v[i] = in ? g.vr[j[i]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        o[i] = in ? g.order[j[i]] : 0u;
+    }
     Dbl4 sum{};
 #pragma unroll
     for (int i = 0; i < SC_IPT; ++i) {
         const int k = base + i * SC_BLOCK;
         if (k < tt.end) {
-            spm[k] = p[i];
+            g.spm[k] = p[i];
+            g.svr[k] = v[i];
+            g.sorder[k] = o[i];
             sum = sum + moment(p[i]);
         }
     }
     Dbl4 total;
     (void)block_exclusive_scan(sum, sh, total);
-    if (threadIdx.x == 0) part[tt.slot] = total;
+    if (threadIdx.x == 0) g.part[tt.slot] = total;
 }
 
 // The apply phase over the gathered bodies: each block first sums the totals of its run's
@@ -1031,10 +1072,57 @@ struct DView {
     }
 };
 
+// A run of equal keys leaves the sort in the order of the positions it started from, which
+// after a walk is the last build's key order, not the caller's.  Puts the run perm[a .. b]
+// in the caller's order (order[] of each entry, all distinct), as the sort of the caller's
+// bodies leaves it: nothing to do when it already is (a run that was one before), an
+// insertion sort for a short run, a heapsort beyond.
+__device__ void order_run(unsigned* perm, const unsigned* order, int a, int b)
+{
+    const int len = b - a + 1;
+    bool sorted = true;
+    for (int i = a + 1; i <= b && sorted; ++i) sorted = order[perm[i - 1]] < order[perm[i]];
+    if (sorted) return;
+    if (len <= 32) {
+        for (int i = a + 1; i <= b; ++i) {
+            const unsigned pi = perm[i], oi = order[pi];
+            int j = i - 1;
+            while (j >= a && order[perm[j]] > oi) {
+                perm[j + 1] = perm[j];
+                --j;
+            }
+            perm[j + 1] = pi;
+        }
+        return;
+    }
+    const auto key = [&](int i) { return order[perm[a + i]]; };
+    const auto sift = [&](int root, int end) {
+        for (;;) {
+            int child = 2 * root + 1;
+            if (child >= end) return;
+            if (child + 1 < end && key(child) < key(child + 1)) ++child;
+            if (!(key(root) < key(child))) return;
+            const unsigned t = perm[a + root];
+            perm[a + root] = perm[a + child];
+            perm[a + child] = t;
+            root = child;
+        }
+    };
+    for (int r = len / 2 - 1; r >= 0; --r) sift(r, len);
+    for (int end = len - 1; end > 0; --end) {
+        const unsigned t = perm[a];
+        perm[a] = perm[a + end];
+        perm[a + end] = t;
+        sift(0, end);
+    }
+}
+
 // One block per 4096 positions: e for its positions (16 per thread), then the minima of its
-// 256 chunks of 16, of its 16 chunks of 256, and of all 4096.
+// 256 chunks of 16, of its 16 chunks of 256, and of all 4096.  The thread that meets the
+// first position of a run of equal keys puts the run in the caller's order (order_run).
 __global__ __launch_bounds__(256) void k_levels(const unsigned long long* K, int n, unsigned char* L0,
-                                                unsigned char* L1, unsigned char* L2, unsigned char* L3, int l3len)
+                                                unsigned char* L1, unsigned char* L2, unsigned char* L3, int l3len,
+                                                unsigned* perm, const unsigned* order)
 {
     __shared__ unsigned char m1[256];
     __shared__ unsigned char m2[16];
@@ -1043,6 +1131,8 @@ __global__ __launch_bounds__(256) void k_levels(const unsigned long long* K, int
     unsigned w[4] = { 0u, 0u, 0u, 0u };
     unsigned mn = 255u;
     unsigned long long kp = p0 < n ? K[p0] : 0ull;
+    // the key before this thread's first position (any other value at position 0)
+    unsigned long long kb = (p0 > 0 && p0 < n) ? K[p0 - 1] : ~kp;
     for (int i = 0; i < 16; ++i) {
         const int p = p0 + i;
         unsigned e;
@@ -1050,6 +1140,12 @@ __global__ __launch_bounds__(256) void k_levels(const unsigned long long* K, int
             const unsigned long long kq = K[p + 1];
             const unsigned long long x = kp ^ kq;
             e = unsigned(x == 0ull ? LEVELS : (__clzll((long long)x) - 1) / 3) + 1u;
+            if (x == 0ull && kb != kp) {
+                int b = p + 1;
+                while (b + 1 < n && K[b + 1] == kp) ++b;
+                order_run(perm, order, p, b);
+            }
+            kb = kp;
             kp = kq;
         } else {
             e = p == n - 1 ? 0u : 255u;
@@ -1362,16 +1458,21 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
 }
 
 // ---- 5. traversal ------------------------------------------------------------------------------
+// The walk reads each body from the build's key-order copies and writes it back at its
+// key-order position: positions, velocities, accelerations and the caller's index, in
+// coalesced stores, so the bodies stay in key order for the next build.
 struct WalkArgs {
     const BhNode* nodes;
     const unsigned* total;  // node count (= base[n])
     unsigned capacity;      // nodes allocated
     const float4* spm;      // bodies in key order
-    const unsigned* perm;   // key order -> body index
-    float4* pm;             // current positions (written in place when stepping)
-    float4* vr;
+    const float4* svr;      // their velocities + radii
+    const unsigned* sorder; // their caller's indices
+    float4* pm;             // out, key order: positions (stepped or not)
+    float4* vr;             // velocities
     float4* acc;
-    unsigned* visits;       // per-body applied-node counts (COUNT only)
+    unsigned* order;        // the caller's index at each position
+    unsigned* visits;       // per-body applied-node counts by the caller's index (COUNT only)
     const unsigned* top;    // node index of the root's child in each octant, ~0u if none
     int n;
     float G;
@@ -1559,21 +1660,21 @@ __device__ __forceinline__ void walk_run(const BhNode* __restrict__ nodes, const
     }
 }
 
-// A body's result: its acceleration (the compensated sum t times G) and, stepping, the
-// reference's semi-implicit Euler step with wrap (detail::integrate_euler, physics.h:60-70),
-// in place.
+// A body's result at key-order position k: its acceleration (the compensated sum t times G)
+// and, stepping, the reference's semi-implicit Euler step with wrap (detail::integrate_euler,
+// physics.h:60-70); its position, velocity and caller's index `body` move to position k.
 template <bool STEP, bool COUNT>
-__device__ __forceinline__ void walk_store(float4* pm, float4* vr, float4* accel, unsigned* visits, unsigned body,
-                                           const float4 x, const float3 t, unsigned applied, float G, float dt,
-                                           float size, int wrap)
+__device__ __forceinline__ void walk_store(float4* pm, float4* vr, float4* accel, unsigned* order, unsigned* visits,
+                                           int k, const float4 x, float4 v, unsigned body, const float3 t,
+                                           unsigned applied, float G, float dt, float size, int wrap)
 {
     const float ax = G * t.x, ay = G * t.y, az = G * t.z;
-    store_stream(accel + body, make_float4(ax, ay, az, 0.f));
+    store_stream(accel + k, make_float4(ax, ay, az, 0.f));
     if (COUNT) visits[body] = applied;
+    order[k] = body;
+    float4 p = x;
     if (STEP) {
 #pragma clang fp contract(off)
-        float4 v = load_stream(vr + body);
-        float4 p = x;
         v.x = v.x + ax * dt;
         v.y = v.y + ay * dt;
         v.z = v.z + az * dt;
@@ -1586,9 +1687,9 @@ __device__ __forceinline__ void walk_store(float4* pm, float4* vr, float4* accel
             p.y = fmodf(fmodf(p.y + half, size) + size, size) - half;
             p.z = fmodf(fmodf(p.z + half, size) + size, size) - half;
         }
-        store_stream(pm + body, p);
-        store_stream(vr + body, v);
     }
+    store_stream(pm + k, p);
+    store_stream(vr + k, v);
 }
 
 // K waves share each 64-body group: wave part p of the group walks the root's children in
@@ -1616,10 +1717,9 @@ __global__ __launch_bounds__(walk_threads<K>()) void k_walk(WalkArgs a)
     const bool valid = k < a.n;
     const unsigned total = min(*a.total, a.capacity);
     const float4 x = valid ? a.spm[k] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const unsigned body = valid ? min(a.perm[k], unsigned(a.n - 1)) : 0u;
     float r2 = 0.f;
     if (CUT && valid) {
-        const float r = a.vr[body].w;
+        const float r = a.svr[k].w;
         r2 = r * r;
     }
     // this part's run of the pre-order: [begin, end)
@@ -1684,7 +1784,9 @@ __global__ __launch_bounds__(walk_threads<K>()) void k_walk(WalkArgs a)
         }
     }
     if (!valid) return;
-    walk_store<STEP, COUNT>(a.pm, a.vr, a.acc, a.visits, body, x, acc.total(), applied, a.G, a.dt, a.size, a.wrap);
+    const unsigned body = min(a.sorder[k], unsigned(a.n - 1));
+    walk_store<STEP, COUNT>(a.pm, a.vr, a.acc, a.order, a.visits, k, x, load_stream(a.svr + k), body, acc.total(),
+                            applied, a.G, a.dt, a.size, a.wrap);
 }
 
 // Barnes-Hut over many systems at once (one build, one walk): every wave walks 64 bodies of
@@ -1695,12 +1797,14 @@ __global__ __launch_bounds__(walk_threads<K>()) void k_walk(WalkArgs a)
 struct BatchWalkArgs {
     const BhNode* nodes;
     const float4* spm;
-    const unsigned* perm;
+    const float4* svr;
+    const unsigned* sorder; // indices within each system
     const unsigned* base;   // pre-order index of the first node at each sorted position
     float4* pm;
     float4* vr;
     float4* acc;
-    unsigned* visits;
+    unsigned* order;
+    unsigned* visits;       // by position in the caller's order (system offset + index)
     const BhSys* sys;
     const int2* waves;      // (system, first position) per 64-body group
     int nwaves;
@@ -1719,7 +1823,6 @@ __global__ __launch_bounds__(256) void k_bwalk(BatchWalkArgs a)
     const int k = g.y + lane;
     const bool valid = k < sy.off + sy.n;
     const float4 x = valid ? a.spm[k] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const unsigned body = valid ? min(a.perm[k], unsigned(a.n - 1)) : 0u;
     const unsigned begin = a.base[sy.off], end = a.base[sy.off + sy.n];
     const unsigned idx = valid ? begin : end;
     Accum acc;
@@ -1727,7 +1830,7 @@ __global__ __launch_bounds__(256) void k_bwalk(BatchWalkArgs a)
     if (sy.cut) {
         float r2 = 0.f;
         if (valid) {
-            const float r = a.vr[body].w;
+            const float r = a.svr[k].w;
             r2 = r * r;
         }
         walk_run<true, COUNT>(a.nodes, a.spm, a.n, sy.eps2, x, r2, idx, end, acc, applied);
@@ -1735,8 +1838,28 @@ __global__ __launch_bounds__(256) void k_bwalk(BatchWalkArgs a)
         walk_run<false, COUNT>(a.nodes, a.spm, a.n, sy.eps2, x, 0.f, idx, end, acc, applied);
     }
     if (!valid) return;
-    walk_store<STEP, COUNT>(a.pm, a.vr, a.acc, a.visits, body, x, acc.total(), applied, sy.G, sy.dt, sy.size,
-                            sy.wrap);
+    const unsigned body = min(a.sorder[k], unsigned(sy.n - 1));
+    walk_store<STEP, COUNT>(a.pm, a.vr, a.acc, a.order, COUNT ? a.visits + sy.off : nullptr, k, x,
+                            load_stream(a.svr + k), body, acc.total(), applied, sy.G, sy.dt, sy.size, sy.wrap);
+}
+
+// ---- the caller's order ----------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_identity(unsigned* order, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) order[i] = unsigned(i);
+}
+
+// Body at position i back to position order[i] (a permutation of [0, n)), into the copies.
+__global__ __launch_bounds__(256) void k_unpermute(int n, const unsigned* order, const float4* pm, const float4* vr,
+                                                   const float4* acc, float4* pm_out, float4* vr_out, float4* acc_out)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const unsigned j = min(order[i], unsigned(n - 1));
+    pm_out[j] = pm[i];
+    vr_out[j] = vr[i];
+    acc_out[j] = acc[i];
 }
 
 // ---- 6. export in the reference's node layout ----------------------------------------------
@@ -1876,6 +1999,8 @@ void bh_grow(nbhip_ctx* c, int n, int extra)
     c->bh_vals[0].grow(n, 0, s);
     c->bh_vals[1].grow(n, 0, s);
     c->bh_spm.grow(n, 0, s);
+    c->bh_svr.grow(n, 0, s);
+    c->bh_sorder.grow(n, 0, s);
     c->bh_vcache.grow(2 * size_t(n), 0, s);
     c->bh_pref.grow(size_t(n) + size_t(extra) + 1, 0, s);
     c->bh_base.grow(size_t(n) + 1, 0, s);
@@ -1888,10 +2013,9 @@ void bh_grow(nbhip_ctx* c, int n, int extra)
     c->bh_dscan.grow(size_t(std::max(1, (n + SC_TILE - 1) / SC_TILE)), 0, s);
 }
 
-// From the sorted keys and the prefix sums on: the level arrays, the next-smaller
-// positions, the per-position node counts and their scan, and the nodes.  e carries the
-// laws (one system's fields, or a batch's tables); the buffers are filled in here.
-void build_nodes(nbhip_ctx* c, int n, const unsigned long long* keys, int sorted, EmitArgs e)
+// From the sorted keys: the level arrays, and runs of equal keys put in the caller's order
+// (perm, the sort's values, is rewritten there).  Runs before the gather reads perm.
+DView build_levels(nbhip_ctx* c, int n, const unsigned long long* keys, unsigned* perm)
 {
     hipStream_t s = c->stream;
     const LevelSizes ls(n);
@@ -1905,13 +2029,37 @@ void build_nodes(nbhip_ctx* c, int n, const unsigned long long* keys, int sorted
     }
     hipLaunchKernelGGL(k_levels, dim3(ls.nb), dim3(256), 0, s, keys, n, c->bh_levels.ptr + ls.off[0],
                        c->bh_levels.ptr + ls.off[1], c->bh_levels.ptr + ls.off[2], c->bh_levels.ptr + ls.off[3],
-                       ls.len[3]);
+                       ls.len[3], perm, static_cast<const unsigned*>(c->order.ptr));
     NBHIP_CHECK(hipGetLastError());
     if (ls.levels > 4) {
         hipLaunchKernelGGL(k_levels_top, dim3(1), dim3(256), 0, s, lt);
         NBHIP_CHECK(hipGetLastError());
     }
+    return dv;
+}
 
+// The gather of positions, velocities and caller's indices into key order.
+GatherArgs gather_args(nbhip_ctx* c, int n, const unsigned* perm)
+{
+    GatherArgs g{};
+    g.perm = perm;
+    g.pm = c->pm[c->cur].ptr;
+    g.vr = c->vr.ptr;
+    g.order = c->order.ptr;
+    g.n = n;
+    g.spm = c->bh_spm.ptr;
+    g.svr = c->bh_svr.ptr;
+    g.sorder = c->bh_sorder.ptr;
+    g.part = c->bh_dscan.ptr;
+    return g;
+}
+
+// From the levels and the prefix sums on: the next-smaller positions, the per-position node
+// counts and their scan, and the nodes.  e carries the laws (one system's fields, or a
+// batch's tables); the buffers are filled in here.
+void build_nodes(nbhip_ctx* c, int n, const unsigned long long* keys, int sorted, const DView& dv, EmitArgs e)
+{
+    hipStream_t s = c->stream;
     const VisitCache vc{ c->bh_vcache.ptr };
     // the sort's other value buffer is free once the sort is done: it holds the next-smaller
     // positions
@@ -2010,11 +2158,11 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     const int sorted = radix_sort(c, n, 0);
     c->bh_sorted = sorted;
     const unsigned long long* keys = c->bh_keys[sorted].ptr;
-    const unsigned* perm = c->bh_vals[sorted].ptr;
+    unsigned* perm = c->bh_vals[sorted].ptr;
+    const DView dv = build_levels(c, n, keys, perm);
 
     const int tiles = std::max(1, (n + SC_TILE - 1) / SC_TILE);
-    hipLaunchKernelGGL(k_gather<WholeRun>, dim3(tiles), dim3(SC_BLOCK), 0, s, perm, c->pm[c->cur].ptr, n,
-                       c->bh_spm.ptr, c->bh_dscan.ptr, WholeRun{ n });
+    hipLaunchKernelGGL(k_gather<WholeRun>, dim3(tiles), dim3(SC_BLOCK), 0, s, gather_args(c, n, perm), WholeRun{ n });
     NBHIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_moment_scan<WholeRun>, dim3(tiles), dim3(SC_BLOCK), 0, s, c->bh_spm.ptr, c->bh_dscan.ptr,
                        c->bh_pref.ptr, WholeRun{ n });
@@ -2030,7 +2178,7 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     e.sys = nullptr;
     e.roots = nullptr;
     e.root_level = 0;
-    build_nodes(c, n, keys, sorted, e);
+    build_nodes(c, n, keys, sorted, dv, e);
     c->bh_built = true;
     c->bh_size = p.size;  // 0: the edge sits in bh_root on the device
     ++c->bh_builds;
@@ -2063,14 +2211,14 @@ void bh_batch_build(nbhip_ctx* c, const BhBatchView& v)
     const int sorted = radix_sort(c, n, 0);
     c->bh_sorted = sorted;
     const unsigned long long* keys = c->bh_keys[sorted].ptr;
-    const unsigned* perm = c->bh_vals[sorted].ptr;
+    unsigned* perm = c->bh_vals[sorted].ptr;
+    const DView dv = build_levels(c, n, keys, perm);
 
     // every system's tiles counted from its own first position: its prefix sums are the
     // ones it has alone
     c->bh_dscan.grow(size_t(v.nitems), 0, s);
     const SystemRuns runs{ v.sys, v.items };
-    hipLaunchKernelGGL(k_gather<SystemRuns>, dim3(v.nitems), dim3(SC_BLOCK), 0, s, perm, pm, n, c->bh_spm.ptr,
-                       c->bh_dscan.ptr, runs);
+    hipLaunchKernelGGL(k_gather<SystemRuns>, dim3(v.nitems), dim3(SC_BLOCK), 0, s, gather_args(c, n, perm), runs);
     NBHIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_moment_scan<SystemRuns>, dim3(v.nitems), dim3(SC_BLOCK), 0, s, c->bh_spm.ptr,
                        c->bh_dscan.ptr, c->bh_pref.ptr, runs);
@@ -2082,7 +2230,7 @@ void bh_batch_build(nbhip_ctx* c, const BhBatchView& v)
     e.sys = v.sys;
     e.roots = v.roots;
     e.root_level = BH_BATCH_ROOT_LEVEL;
-    build_nodes(c, n, keys, sorted, e);
+    build_nodes(c, n, keys, sorted, dv, e);
     c->bh_built = false;  // not a tree of one system: nothing exports it
     ++c->bh_builds;
 }
@@ -2095,11 +2243,13 @@ void bh_batch_walk(nbhip_ctx* c, const BhBatchView& v, bool fuse, bool count)
     BatchWalkArgs a{};
     a.nodes = c->bh_nodes.ptr;
     a.spm = c->bh_spm.ptr;
-    a.perm = c->bh_vals[c->bh_sorted].ptr;
+    a.svr = c->bh_svr.ptr;
+    a.sorder = c->bh_sorder.ptr;
     a.base = c->bh_base.ptr;
     a.pm = c->pm[c->cur].ptr;
     a.vr = c->vr.ptr;
     a.acc = c->acc.ptr;
+    a.order = c->order.ptr;
     a.visits = c->bh_visits.ptr;
     a.sys = v.sys;
     a.waves = v.waves;
@@ -2146,10 +2296,12 @@ void bh_walk(nbhip_ctx* c, const nbhip_params& p, bool fuse, bool count)
     a.total = c->bh_base.ptr + n;
     a.capacity = unsigned(c->bh_nodes.cap);
     a.spm = c->bh_spm.ptr;
-    a.perm = c->bh_vals[c->bh_sorted].ptr;
+    a.svr = c->bh_svr.ptr;
+    a.sorder = c->bh_sorder.ptr;
     a.pm = c->pm[c->cur].ptr;
     a.vr = c->vr.ptr;
     a.acc = c->acc.ptr;
+    a.order = c->order.ptr;
     a.visits = c->bh_visits.ptr;
     a.n = int(n);
     a.G = p.G;
@@ -2169,6 +2321,7 @@ void bh_walk(nbhip_ctx* c, const nbhip_params& p, bool fuse, bool count)
     hipLaunchKernelGGL(k, dim3(unsigned((n + size_t(bodies) - 1) / size_t(bodies))), dim3(threads), 0, c->stream, a);
     NBHIP_CHECK(hipGetLastError());
     c->launches += 1;
+    if (n > 0) c->permuted = true;  // the bodies now sit in this build's key order
 }
 
 void bh_accelerate(nbhip_ctx* c, const nbhip_params& p, bool fuse, bool count)
@@ -2283,6 +2436,36 @@ float bh_root_size(nbhip_ctx* c)
     return root;
 }
 
+void identity_order(hipStream_t s, unsigned* order, size_t n)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_identity, dim3(grid(n)), dim3(256), 0, s, order, int(n));
+    NBHIP_CHECK(hipGetLastError());
+}
+
+void restore_order(nbhip_ctx* c)
+{
+    if (!c->permuted) return;
+    const size_t n = c->count;
+    hipStream_t s = c->stream;
+    if (n > 0) {
+        // the other position buffer and the visit cache are free between steps
+        c->bh_vcache.grow(2 * n, 0, s);
+        float4* pm_t = c->pm[c->cur ^ 1].ptr;
+        float4* vr_t = reinterpret_cast<float4*>(c->bh_vcache.ptr);
+        float4* acc_t = vr_t + n;
+        hipLaunchKernelGGL(k_unpermute, dim3(grid(n)), dim3(256), 0, s, int(n), c->order.ptr, c->pm[c->cur].ptr,
+                           c->vr.ptr, c->acc.ptr, pm_t, vr_t, acc_t);
+        NBHIP_CHECK(hipGetLastError());
+        const size_t bytes = n * sizeof(float4);
+        NBHIP_CHECK(hipMemcpyAsync(c->pm[c->cur].ptr, pm_t, bytes, hipMemcpyDeviceToDevice, s));
+        NBHIP_CHECK(hipMemcpyAsync(c->vr.ptr, vr_t, bytes, hipMemcpyDeviceToDevice, s));
+        NBHIP_CHECK(hipMemcpyAsync(c->acc.ptr, acc_t, bytes, hipMemcpyDeviceToDevice, s));
+        identity_order(s, c->order.ptr, n);
+    }
+    c->permuted = false;
+}
+
 void bh_release(nbhip_ctx* c)
 {
     for (int i = 0; i < 2; ++i) {
@@ -2290,6 +2473,8 @@ void bh_release(nbhip_ctx* c)
         c->bh_vals[i].release();
     }
     c->bh_spm.release();
+    c->bh_svr.release();
+    c->bh_sorder.release();
     c->bh_vcache.release();
     c->bh_pref.release();
     c->bh_base.release();

@@ -8,7 +8,9 @@
 // the end of the arrays; when one leaves, the ones after it move down (into the spare
 // arrays, which then become the current ones).  The device tables -- one entry per system,
 // per 1024-position scan tile of a system, per 64-body walk group of a system -- are made
-// again whenever the membership changes, and a captured step sequence with them.
+// again whenever the membership changes, and a captured step sequence with them.  A walk
+// leaves each system's bodies in the key order of its build; the batch's order array holds,
+// at each position, the body's index within its system, which a download puts back.
 #include <algorithm>
 #include <map>
 #include <new>
@@ -50,6 +52,7 @@ struct nbhip_bh_batch {
 
     // spare arrays that a removal compacts into (positions use work->pm[1])
     nbhip::DevBuf<float4> spare_vr, spare_acc;
+    nbhip::DevBuf<unsigned> spare_order;
 
     // captured step sequence
     hipGraph_t graph = nullptr;
@@ -77,6 +80,7 @@ struct nbhip_bh_batch {
         ext.release();
         spare_vr.release();
         spare_acc.release();
+        spare_order.release();
         nbhip_destroy(work);
     }
 };
@@ -121,8 +125,8 @@ void check_system(const nbhip_bh_batch* b, size_t n, const nbhip_params* p)
              total);
 }
 
-// Room for n more bodies at the end of the arrays (their contents kept); returns the first
-// new position.
+// Room for n more bodies at the end of the arrays (their contents kept), in the caller's
+// order; returns the first new position.
 size_t append_room(nbhip_bh_batch* b, size_t n)
 {
     nbhip_ctx* w = b->work;
@@ -130,6 +134,8 @@ size_t append_room(nbhip_bh_batch* b, size_t n)
     w->pm[0].grow(off + n, off, w->stream);
     w->vr.grow(off + n, off, w->stream);
     w->acc.grow(off + n, off, w->stream);
+    w->order.grow(off + n, off, w->stream);
+    identity_order(w->stream, w->order.ptr + off, n);
     return off;
 }
 
@@ -312,6 +318,7 @@ int nbhip_bh_batch_remove(nbhip_bh_batch* b, int id)
         w->pm[1].grow(std::max<size_t>(1, total), 0, s);
         b->spare_vr.grow(std::max<size_t>(1, total), 0, s);
         b->spare_acc.grow(std::max<size_t>(1, total), 0, s);
+        b->spare_order.grow(std::max<size_t>(1, total), 0, s);
         size_t off = 0;
         for (auto& kv : b->systems) {
             nbhip::BhBatchSystem& sy = kv.second;
@@ -319,6 +326,8 @@ int nbhip_bh_batch_remove(nbhip_bh_batch* b, int id)
             NBHIP_CHECK(hipMemcpyAsync(w->pm[1].ptr + off, w->pm[0].ptr + sy.off, bytes, hipMemcpyDeviceToDevice, s));
             NBHIP_CHECK(hipMemcpyAsync(b->spare_vr.ptr + off, w->vr.ptr + sy.off, bytes, hipMemcpyDeviceToDevice, s));
             NBHIP_CHECK(hipMemcpyAsync(b->spare_acc.ptr + off, w->acc.ptr + sy.off, bytes, hipMemcpyDeviceToDevice, s));
+            NBHIP_CHECK(hipMemcpyAsync(b->spare_order.ptr + off, w->order.ptr + sy.off, sy.n * sizeof(unsigned),
+                                       hipMemcpyDeviceToDevice, s));
             sy.off = off;
             off += sy.n;
         }
@@ -326,6 +335,7 @@ int nbhip_bh_batch_remove(nbhip_bh_batch* b, int id)
         std::swap(w->pm[0], w->pm[1]);
         std::swap(w->vr, b->spare_vr);
         std::swap(w->acc, b->spare_acc);
+        std::swap(w->order, b->spare_order);
         nbhip::g_buffer_epoch.fetch_add(1);
         w->count = total;
         b->dirty = true;
@@ -401,9 +411,10 @@ int nbhip_bh_batch_download(nbhip_bh_batch* b, int id, void* bodies, size_t n)
         if (k == 0) return;
         if (!bodies) nbhip::fail(NBHIP_ERR_INVALID, "null bodies");
         nbhip_ctx* w = b->work;
-        w->aos.grow(12 * k, 0, w->stream);
-        nbhip::split_to_aos_raw(w->stream, w->aos.ptr, k, w->pm[0].ptr + sy.off, w->vr.ptr + sy.off,
-                                w->acc.ptr + sy.off);
+        // the whole system, each body to its place in the caller's order, then the first k
+        w->aos.grow(12 * sy.n, 0, w->stream);
+        nbhip::split_to_aos_raw(w->stream, w->aos.ptr, sy.n, w->pm[0].ptr + sy.off, w->vr.ptr + sy.off,
+                                w->acc.ptr + sy.off, w->order.ptr + sy.off);
         NBHIP_CHECK(hipMemcpyAsync(bodies, w->aos.ptr, 48 * k, hipMemcpyDeviceToHost, w->stream));
         NBHIP_CHECK(hipStreamSynchronize(w->stream));
         nbhip::bh_check_fault(w);

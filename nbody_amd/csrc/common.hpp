@@ -177,6 +177,13 @@ struct nbhip_ctx {
     nbhip::DevBuf<float4> vr;
     nbhip::DevBuf<float4> acc;
 
+    // A Barnes-Hut walk leaves the bodies in the key order of its build: position k then
+    // holds the caller's body order[k] (`permuted`).  Everything that addresses bodies by
+    // the caller's index restores the caller's order first (restore_order); order is the
+    // identity otherwise.
+    nbhip::DevBuf<unsigned> order;
+    bool permuted = false;
+
     // launch shape chosen by nbhip_tune for one body count (tuned_n == 0: none)
     size_t tuned_n = 0;
     nbhip::DirectConfig tuned;
@@ -202,6 +209,8 @@ struct nbhip_ctx {
     nbhip::DevBuf<unsigned long long> bh_keys[2];
     nbhip::DevBuf<unsigned> bh_vals[2];
     nbhip::DevBuf<float4> bh_spm;
+    nbhip::DevBuf<float4> bh_svr;          // velocities + radii in key order
+    nbhip::DevBuf<unsigned> bh_sorder;     // the caller's index of each body in key order
     nbhip::DevBuf<uint4> bh_vcache;        // the nodes k_count met per sorted position, for k_emit
     nbhip::DevBuf<nbhip::Dbl4> bh_pref;
     nbhip::DevBuf<unsigned> bh_base;
@@ -267,8 +276,9 @@ void force_error_f64(nbhip_ctx* c, const nbhip_params& p, const uint32_t* sample
 void ensure_radius_flag(nbhip_ctx* c);
 void aos_to_split(nbhip_ctx* c, size_t n);
 void aos_to_split_raw(hipStream_t s, const float* aos, size_t n, float4* pm, float4* vr, float4* acc);
+// order (may be null): record order[i] receives body i (a system kept in key order)
 void split_to_aos_raw(hipStream_t s, float* aos, size_t n, const float4* pm, const float4* vr,
-                      const float4* acc);
+                      const float4* acc, const unsigned* order = nullptr);
 void split_to_aos(nbhip_ctx* c, size_t n);
 DirectConfig choose_direct_config(size_t n);
 // nbhip_accelerate_noalloc: takes the static scratch's address on `device` (current) once
@@ -287,6 +297,10 @@ void bh_profile_steps(nbhip_ctx* c, const nbhip_params& p, int reps, double* bui
 void bh_release(nbhip_ctx* c);
 void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm);
 void bh_reserve(nbhip_ctx* c);
+// Puts the bodies back in the caller's order (after Barnes-Hut walks), stream-ordered.
+void restore_order(nbhip_ctx* c);
+// order[off + i] = i for i < n, stream-ordered (bodies in the caller's order).
+void identity_order(hipStream_t s, unsigned* order, size_t n);
 // Reports (and clears) a radix-sort look-back that gave up waiting; call with the stream idle.
 void bh_check_fault(nbhip_ctx* c);
 // Sets the fault word from the host, as a look-back that gave up waiting would (tests only).

@@ -325,11 +325,12 @@ __global__ __launch_bounds__(256) void k_aos_to_split(const float* aos, int n, f
 }
 
 __global__ __launch_bounds__(256) void k_split_to_aos(float* aos, int n, const float4* pm,
-                                                      const float4* vr, const float4* acc)
+                                                      const float4* vr, const float4* acc,
+                                                      const unsigned* order)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    float* b = aos + size_t(i) * 12;
+    float* b = aos + size_t(order ? min(order[i], unsigned(n - 1)) : unsigned(i)) * 12;
     const float4 x = pm[i], v = vr[i], g = acc[i];
     b[0] = x.x; b[1] = x.y; b[2] = x.z; b[3] = v.w;
     b[4] = v.x; b[5] = v.y; b[6] = v.z; b[7] = x.w;
@@ -850,9 +851,9 @@ void aos_to_split_raw(hipStream_t s, const float* aos, size_t n, float4* pm, flo
 }
 
 void split_to_aos_raw(hipStream_t s, float* aos, size_t n, const float4* pm, const float4* vr,
-                      const float4* acc)
+                      const float4* acc, const unsigned* order)
 {
-    hipLaunchKernelGGL(k_split_to_aos, dim3(grid1(n)), dim3(256), 0, s, aos, int(n), pm, vr, acc);
+    hipLaunchKernelGGL(k_split_to_aos, dim3(grid1(n)), dim3(256), 0, s, aos, int(n), pm, vr, acc, order);
     NBHIP_CHECK(hipGetLastError());
 }
 
@@ -866,7 +867,7 @@ void aos_to_split(nbhip_ctx* c, size_t n)
 void split_to_aos(nbhip_ctx* c, size_t n)
 {
     hipLaunchKernelGGL(k_split_to_aos, dim3(grid1(n)), dim3(256), 0, c->stream, c->aos.ptr, int(n),
-                       c->pm[c->cur].ptr, c->vr.ptr, c->acc.ptr);
+                       c->pm[c->cur].ptr, c->vr.ptr, c->acc.ptr, static_cast<const unsigned*>(nullptr));
     NBHIP_CHECK(hipGetLastError());
 }
 

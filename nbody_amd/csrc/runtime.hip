@@ -176,6 +176,7 @@ void nbhip_destroy(nbhip_ctx* c)
     c->pm[1].release();
     c->vr.release();
     c->acc.release();
+    c->order.release();
     c->partial.release();
     c->sym_tiles.dev.release();
     c->aos.release();
@@ -206,11 +207,14 @@ int nbhip_reserve(nbhip_ctx* c, size_t n)
         check_ctx(c);
         ++c->body_version;  // the staged readback no longer matches
         NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::restore_order(c);  // the kept bodies stay where the caller put them
         const size_t keep = c->count;
         c->pm[0].grow(n, keep, c->stream);
         c->pm[1].grow(n, keep, c->stream);
         c->vr.grow(n, keep, c->stream);
         c->acc.grow(n, keep, c->stream);
+        c->order.grow(n, 0, c->stream);
+        nbhip::identity_order(c->stream, c->order.ptr, n);
         if (n != c->count) c->bh_built = false;
         c->count = n;
         c->radius_known = false;
@@ -231,6 +235,7 @@ int nbhip_upload(nbhip_ctx* c, const float* pos_mass, const float* vel_radius, c
                  offset + n, c->count);
         if (n == 0) return;
         NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::restore_order(c);
         const size_t bytes = n * sizeof(float4);
         if (pos_mass)
             NBHIP_CHECK(hipMemcpyAsync(c->pm[c->cur].ptr + offset, pos_mass, bytes,
@@ -257,6 +262,7 @@ int nbhip_download(nbhip_ctx* c, unsigned mask, float* pos_mass, float* vel_radi
             fail(NBHIP_ERR_INVALID, "download range [%zu, %zu) beyond body count %zu", offset,
                  offset + n, c->count);
         NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::restore_order(c);
         const size_t bytes = n * sizeof(float4);
         if (n && (mask & NBHIP_RB_POSITIONS)) {
             if (!pos_mass) fail(NBHIP_ERR_INVALID, "null pos_mass with POSITIONS requested");
@@ -280,6 +286,7 @@ int nbhip_download(nbhip_ctx* c, unsigned mask, float* pos_mass, float* vel_radi
 
 int nbhip_upload_bodies(nbhip_ctx* c, const void* bodies, size_t n)
 {
+    if (c) c->permuted = false;  // every body is replaced: no order to put back
     int st = nbhip_reserve(c, n);
     if (st != NBHIP_OK) return st;
     return guarded(c, [&] {
@@ -299,7 +306,10 @@ int nbhip_generate(nbhip_ctx* c, const nbhip_ics* spec, size_t n)
         nbhip::check_ics(spec, n);
         check_ctx(c);
     });
-    if (st == NBHIP_OK) st = nbhip_reserve(c, n);
+    if (st == NBHIP_OK) {
+        c->permuted = false;  // every body is replaced: no order to put back
+        st = nbhip_reserve(c, n);
+    }
     if (st != NBHIP_OK) return st;
     return guarded(c, [&] {
         NBHIP_CHECK(hipSetDevice(c->device));
@@ -331,6 +341,7 @@ int nbhip_readback(nbhip_ctx* c)
             c->staging_cap = want;
         }
         if (n) {
+            nbhip::restore_order(c);
             c->aos.grow(12 * n, 0, c->stream);
             nbhip::split_to_aos(c, n);
             NBHIP_CHECK(hipMemcpyAsync(c->staging, c->aos.ptr, 48 * n, hipMemcpyDeviceToHost, c->stream));
@@ -367,6 +378,7 @@ int nbhip_download_bodies(nbhip_ctx* c, void* bodies, size_t n)
         if (k == 0) return;
         if (!bodies) fail(NBHIP_ERR_INVALID, "null bodies");
         NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::restore_order(c);
         c->aos.grow(12 * c->count, 0, c->stream);
         nbhip::split_to_aos(c, k);
         NBHIP_CHECK(hipMemcpyAsync(bodies, c->aos.ptr, 48 * k, hipMemcpyDeviceToHost, c->stream));
@@ -477,7 +489,10 @@ void step_graph(nbhip_ctx* c, const nbhip_params& p, int steps)
     NBHIP_CHECK(hipGraphLaunch(c->graph_exec, c->stream));
     c->cur = cur_after;
     c->launches += steps;
-    if (p.mode != NBHIP_MODE_N2) c->bh_builds += steps;  // the replay rebuilt the tree
+    if (p.mode != NBHIP_MODE_N2) {
+        c->bh_builds += steps;  // the replay rebuilt the tree
+        c->permuted = true;     // and left the bodies in its key order
+    }
 }
 
 }  // namespace
@@ -574,6 +589,7 @@ int nbhip_diagnostics(nbhip_ctx* c, const nbhip_params* p, double out[9])
         for (int k = 0; k < 9; ++k) out[k] = 0;
         if (c->count == 0) return;
         NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::restore_order(c);  // the same sums in the same order as before any walk
         nbhip::diagnostics(c, *p, out);
     });
 }
@@ -591,6 +607,7 @@ int nbhip_force_error_f64(nbhip_ctx* c, const nbhip_params* p, const uint32_t* s
                 fail(NBHIP_ERR_INVALID, "sample index %u beyond body count %zu", sample[k],
                      c->count);
         NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::restore_order(c);  // the sample names the caller's bodies
         nbhip::force_error_f64(c, *p, sample, ns, acc64, rel_err);
     });
 }
@@ -602,6 +619,7 @@ int nbhip_bh_keys(nbhip_ctx* c, const nbhip_params* p, uint64_t* keys, uint32_t*
         check_params(p);
         if (c->count == 0) return;
         NBHIP_CHECK(hipSetDevice(c->device));
+        nbhip::restore_order(c);  // keys and permutation by the caller's index
         nbhip::bh_keys(c, *p, keys, perm);
     });
 }

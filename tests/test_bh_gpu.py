@@ -661,3 +661,77 @@ def test_root_found_from_the_bodies(ctx, orc, scale):
     assert np.array_equal(s0.view(np.uint32), ctx.download_bodies().view(np.uint32))
     with pytest.raises(_lib.HipError):
         ctx.accelerate(bh_params(size=-1.0))
+
+
+def test_key_order_between_steps_is_invisible(ctx):
+    # Steps leave the bodies in key order on the device.  A dense cluster under the
+    # reference's root edge (10000: 21-level cells 0.005 wide) puts many bodies in shared
+    # finest cells, opened and applied body by body, and bodies change cells between steps,
+    # so the last build's key order often disagrees with the caller's order among equal
+    # keys; every build puts those in the caller's order.  Stepping on therefore equals
+    # re-uploading in the caller's order before every step, bit for bit, eagerly and from
+    # a graph -- and the order among equal keys does matter: the same bodies listed in
+    # reverse give different bits.
+    n = 20000
+    b = ics.plummer(n, seed=3, a=0.02)
+    p = bh_params(size=10000.0, theta=0.5, dt=2e-3, eps=1e-3, leaf_size=1, wrap=0)
+    steps = 4
+
+    def fresh_each_step(bodies):
+        x = bodies
+        for _ in range(steps):
+            ctx.upload_bodies(x)
+            ctx.step(p, 1)
+            x = ctx.download_bodies()
+        return x
+
+    ctx.upload_bodies(b)
+    keys, _ = ctx.bh_keys(p)
+    assert np.unique(keys).size < n - 2000   # many bodies share a finest cell
+    ctx.step(p, steps)
+    on = ctx.download_bodies()
+    ctx.upload_bodies(b)
+    ctx.step_graph(p, steps)
+    graph = ctx.download_bodies()
+    ref = fresh_each_step(b)
+    assert np.array_equal(on, ref)
+    assert np.array_equal(graph, ref)
+    rev = fresh_each_step(b[::-1])[::-1]
+    assert not np.array_equal(rev[:, 8:11], ref[:, 8:11])
+
+
+def test_caller_order_calls_after_steps(ctx):
+    # partial downloads, partial uploads, staged readback and diagnostics address bodies by
+    # the caller's index whatever order the steps left on the device
+    n = 30000
+    b = ics.disk_bulge(n, seed=8)
+    p = bh_params(size=128.0, eps=0.01, leaf_size=16, wrap=0)
+    ctx.upload_bodies(b)
+    ctx.step(p, 3)
+    full = ctx.download_bodies()
+    ctx.upload_bodies(full)
+    d_ref = ctx.diagnostics(p)
+    ctx.upload_bodies(b)
+    ctx.step(p, 3)
+    assert np.array_equal(ctx.diagnostics(p), d_ref)
+    ctx.step(p, 0)
+    pm, vr, acc = ctx.download(offset=1000, n=500)
+    assert np.array_equal(pm[:, 0:3], full[1000:1500, 0:3]) and np.array_equal(pm[:, 3], full[1000:1500, 7])
+    assert np.array_equal(acc[:, 0:3], full[1000:1500, 8:11])
+    ctx.upload_bodies(b)
+    ctx.step(p, 3)
+    ctx.readback()
+    staged, was_staged = ctx.collect_bodies()
+    assert was_staged and np.array_equal(staged, full)
+    # a partial upload after steps changes exactly the bodies it names
+    ctx.upload_bodies(b)
+    ctx.step(p, 3)
+    new_pm = np.zeros((10, 4), np.float32)
+    new_pm[:, 0] = np.arange(10)
+    new_pm[:, 3] = 1.0
+    ctx.upload(pos_mass=new_pm, offset=20, n=10)
+    after = ctx.download_bodies()
+    want = full.copy()
+    want[20:30, 0:3] = new_pm[:, 0:3]
+    want[20:30, 7] = 1.0
+    assert np.array_equal(after, want)
