@@ -1,6 +1,6 @@
 """Benchmark: gravity on one MI355X.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 2] [--no-cpu] [--only direct|batch|bh]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 2] [--no-cpu] [--only LEG]
 
 Prints ONE JSON line.  The headline (value, roofline, cpu_baseline) is the direct sum at
 N = 262,144 (two Plummer spheres): a step is one fused force + kick-drift pass over all N
@@ -17,6 +17,9 @@ Secondary objects in the same line:
                a hipGraph, against the same systems stepped one launch per system;
   "barnes_hut" N = 2^20 exponential disk + Plummer bulge, theta 0.5, leaf 16, 10 steps,
                with the reference Barnes-Hut restatement timed on the CPU;
+  "bh_batched" the served workload's Barnes-Hut jobs in one Barnes-Hut batch (one sort,
+               build and walk per step for all of them), against the same systems one
+               context at a time;
   "served"     a Poisson stream of jobs through the continuous-batching server: served
                steps/s and p50/p99 latency.
 """
@@ -432,11 +435,11 @@ def run_bh(args) -> dict:
     nodes = int(cnt.value)
     ctx.close()
     # algorithmic bytes of one build per body: keys 16 in + 12 out; radix sort 8 (the
-    # digit counts of all passes in one read) + 8 passes x 24; gather 20 in + 16 out;
-    # moment scan 16 in + 32 out; levels 8 in + ~1.07 out; next-smaller 4 out (325 B so
-    # far); node count and emit ~5 (per-position counts and kept flags) -- 330 B -- plus
-    # the 32 B of every node written
-    build_bytes = 330.0 * n + 32.0 * nodes
+    # digit counts of all passes in one read) + 8 passes x 24; levels 8 in + ~1.07 out;
+    # gather (position, velocity, caller's index and the sort's value) 40 in + 36 out;
+    # moment scan 16 in + 32 out; next-smaller 4 out (365 B so far); node count and emit ~5
+    # (per-position counts and kept flags) -- 370 B -- plus the 32 B of every node written
+    build_bytes = 370.0 * n + 32.0 * nodes
     build_gbs = build_bytes / (build_ms * 1e-3) / 1e9
     walk_tflops = FLOPS_PER_INTERACTION * inter / (walk_ms * 1e-3) / 1e12
     out = {
@@ -450,7 +453,7 @@ def run_bh(args) -> dict:
             "build": {"bound": "hbm", "achieved": build_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": build_gbs / HBM_PEAK_GBS, "ms": build_ms,
                       "algorithmic_bytes": build_bytes, "nodes": nodes,
-                      "bytes_formula": "330 B per body + 32 B per node"},
+                      "bytes_formula": "370 B per body + 32 B per node"},
             "walk": {"bound": "valu", "achieved": walk_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": walk_tflops / FP32_PEAK_TFLOPS, "ms": walk_ms,
                      "flops": f"{FLOPS_PER_INTERACTION} per body-node interaction"},
@@ -580,6 +583,59 @@ def run_served(args) -> dict:
     }
 
 
+def run_bh_batch(args) -> dict:
+    """The served workload's Barnes-Hut jobs (seed 2026: Plummer or disk+bulge, N
+    log-uniform in [16384, 262144], theta 0.5, leaf 16, eps 0.01, roots found from the
+    bodies), drawn on the device into one nbhip_bh_batch: 10 steps replayed from a captured
+    graph (wall time to idle), the build/walk split of 10 eager steps (HIP events), and the
+    interactions of one counted pass; against the same systems one context at a time, each
+    replaying 10 steps from its own graph (HIP events per system, summed)."""
+    from nbody_amd import _lib, serve
+    jobs = [j for j in serve.make_jobs(160, seed=SEED, device_ics=True) if j.mode == _lib.MODE_NLOGN]
+    bodies = sum(j.n for j in jobs)
+    steps = 10
+    ctx = _lib.Context(0)
+    with _lib.BhBatch(ctx) as b:
+        ids = [b.add_generated(j.ics, j.n, j.params) for j in jobs]
+        b.step(1, _lib.BATCH_ACCELERATE | _lib.BATCH_COUNT)
+        inter = float(sum(int(b.visits(i).sum()) for i in ids))
+        b.step(steps, _lib.BATCH_GRAPH)   # capture, and a warm run
+        b.download(ids[0])
+        t0 = time.perf_counter()
+        b.step(steps, _lib.BATCH_GRAPH)
+        while not b.idle():
+            time.sleep(20e-6)
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        build_ms, walk_ms = b.profile(steps)
+        info = b.info()
+    seq_ms = 0.0
+    for j in jobs:
+        ctx.generate(j.ics, j.n)
+        ctx.step_graph(j.params, steps)
+        ctx.synchronize()
+        ctx.timer_start()
+        ctx.step_graph(j.params, steps)
+        t, _ = ctx.timer_stop()
+        seq_ms += t / steps
+    ctx.close()
+    tflops = inter * FLOPS_PER_INTERACTION / (walk_ms * 1e-3) / 1e12
+    return {
+        "workload": f"{len(jobs)} Barnes-Hut systems of the served workload, {bodies} bodies in all, "
+                    "theta 0.5, leaf 16, roots found from the bodies; one Barnes-Hut batch, steps "
+                    "replayed from a captured graph",
+        "systems": len(jobs), "bodies": bodies,
+        "ms_per_step": ms, "ns_per_body_step": ms * 1e6 / bodies,
+        "build_ms": build_ms, "walk_ms": walk_ms,
+        "interactions_per_step": inter,
+        "walk_roofline": {"bound": "valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": tflops / FP32_PEAK_TFLOPS,
+                          "flops": f"{FLOPS_PER_INTERACTION} per body-node interaction"},
+        "device_bytes": int(info.device_bytes),
+        "one_context_each_ms_per_step": seq_ms,
+        "speedup_vs_one_context_each": seq_ms / ms,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -589,7 +645,8 @@ def main() -> int:
     ap.add_argument("--bh-bodies", type=int, default=1 << 20,
                     help="bodies of the Barnes-Hut leg (the C3 workload is 2^20; smaller sizes are "
                          "the served jobs' range)")
-    ap.add_argument("--only", choices=["direct", "c1", "batch", "integrate", "bh", "served"], default=None,
+    ap.add_argument("--only", choices=["direct", "c1", "batch", "integrate", "bh", "bh_batch", "served"],
+                    default=None,
                     help="run one workload (the headline fields then describe it)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -609,6 +666,8 @@ def main() -> int:
         out["integrate"] = run_integrate(args)
     if args.only in (None, "bh"):
         out["barnes_hut"] = run_bh(args)
+    if args.only in (None, "bh_batch"):
+        out["bh_batched"] = run_bh_batch(args)
     if args.only in (None, "served"):
         out["served"] = run_served(args)
     print(json.dumps(out))
