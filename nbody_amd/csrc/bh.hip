@@ -1117,43 +1117,47 @@ __device__ void order_run(unsigned* perm, const unsigned* order, int a, int b)
     }
 }
 
-// One block per 4096 positions: e for its positions (16 per thread), then the minima of its
-// 256 chunks of 16, of its 16 chunks of 256, and of all 4096.  The thread that meets the
-// first position of a run of equal keys puts the run in the caller's order (order_run).
+// One block per 4096 positions: e for its positions, computed at positions 256 i + t so
+// that consecutive threads read consecutive keys, and passed through LDS so that each
+// thread then holds 16 consecutive ones; then the minima of the block's 256 chunks of 16,
+// of its 16 chunks of 256, and of all 4096.  The thread that meets the first position of a
+// run of equal keys puts the run in the caller's order (order_run).
 __global__ __launch_bounds__(256) void k_levels(const unsigned long long* K, int n, unsigned char* L0,
                                                 unsigned char* L1, unsigned char* L2, unsigned char* L3, int l3len,
                                                 unsigned* perm, const unsigned* order)
 {
+    __shared__ unsigned char e_sh[4096];
     __shared__ unsigned char m1[256];
     __shared__ unsigned char m2[16];
     const int t = threadIdx.x;
-    const int p0 = blockIdx.x * 4096 + t * 16;
-    unsigned w[4] = { 0u, 0u, 0u, 0u };
-    unsigned mn = 255u;
-    unsigned long long kp = p0 < n ? K[p0] : 0ull;
-    // the key before this thread's first position (any other value at position 0)
-    unsigned long long kb = (p0 > 0 && p0 < n) ? K[p0 - 1] : ~kp;
+    const int base = blockIdx.x * 4096;
+#pragma unroll 4
     for (int i = 0; i < 16; ++i) {
-        const int p = p0 + i;
+        const int p = base + 256 * i + t;
         unsigned e;
         if (p < n - 1) {
-            const unsigned long long kq = K[p + 1];
+            const unsigned long long kp = K[p], kq = K[p + 1];
             const unsigned long long x = kp ^ kq;
             e = unsigned(x == 0ull ? LEVELS : (__clzll((long long)x) - 1) / 3) + 1u;
-            if (x == 0ull && kb != kp) {
+            if (x == 0ull && (p == 0 || K[p - 1] != kp)) {
                 int b = p + 1;
                 while (b + 1 < n && K[b + 1] == kp) ++b;
                 order_run(perm, order, p, b);
             }
-            kb = kp;
-            kp = kq;
         } else {
             e = p == n - 1 ? 0u : 255u;
         }
-        w[i >> 2] |= e << (8 * (i & 3));
-        mn = min(mn, e);
+        e_sh[256 * i + t] = (unsigned char)e;
     }
-    *reinterpret_cast<uint4*>(L0 + p0) = make_uint4(w[0], w[1], w[2], w[3]);
+    __syncthreads();
+    const uint4 w = *reinterpret_cast<const uint4*>(e_sh + 16 * t);
+    unsigned mn = 255u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const unsigned v = j == 0 ? w.x : j == 1 ? w.y : j == 2 ? w.z : w.w;
+        mn = min(min(min(mn, v & 255u), (v >> 8) & 255u), min((v >> 16) & 255u, v >> 24));
+    }
+    *reinterpret_cast<uint4*>(L0 + base + 16 * t) = w;
     m1[t] = (unsigned char)mn;
     L1[blockIdx.x * 256 + t] = (unsigned char)mn;
     __syncthreads();
