@@ -3,8 +3,9 @@
     python -m nbody_amd.tune [--n 262144] [--workload two_plummer|plummer] [--eps 0.005]
 
 The shape chosen is kept by the context for that body count (nbhip_tune); this script
-only reports the sweep, as a table of (P bodies per lane, W waves per block, j-split, ms
-per force pass, TFLOP/s at 20 flop per interaction).
+only reports the sweep, as a table of (P bodies per lane, W waves per block, j-split --
+"pair-symmetric" for the tiles that evaluate each pair once --, ms per force pass,
+TFLOP/s at 20 flop per interaction).
 """
 from __future__ import annotations
 
@@ -31,8 +32,9 @@ def main() -> None:
     print("| P | W | j-split | ms | TFLOP/s |")
     print("|---|---|---|---|---|")
     for P, W, js, ms in sorted(rows, key=lambda r: r[3]):
-        print(f"| {P} | {W} | {js} | {ms:.3f} | {20 * inter / (ms * 1e-3) / 1e12:.1f} |")
-    print(f"chosen: P={best[0]} W={best[1]} j-split={best[2]} ({best[3]:.3f} ms)")
+        print(f"| {P} | {W} | {js if js else 'pair-symmetric'} | {ms:.3f} | {20 * inter / (ms * 1e-3) / 1e12:.1f} |")
+    kind = f"j-split={best[2]}" if best[2] else "pair-symmetric"
+    print(f"chosen: P={best[0]} W={best[1]} {kind} ({best[3]:.3f} ms)")
 
 
 if __name__ == "__main__":
