@@ -114,9 +114,13 @@ void check_system(const nbhip_bh_batch* b, size_t n, const nbhip_params* p)
 {
     if (!p || n == 0) fail(NBHIP_ERR_INVALID, "a batch system needs bodies and params");
     if (p->mode != NBHIP_MODE_NLOGN) fail(NBHIP_ERR_UNSUPPORTED, "a Barnes-Hut batch runs NLogN systems only");
+    // the parameter checks of a single context's Barnes-Hut step (nbhip_step)
     if (!(p->eps >= 0.f)) fail(NBHIP_ERR_INVALID, "eps must be >= 0");
     if (p->opening != NBHIP_OPEN_REFERENCE && p->opening != NBHIP_OPEN_TEXTBOOK)
         fail(NBHIP_ERR_INVALID, "unknown opening criterion %d", p->opening);
+    if (p->leaf_size < 1) fail(NBHIP_ERR_INVALID, "leaf_size must be >= 1");
+    if (!(p->size >= 0.f)) fail(NBHIP_ERR_INVALID, "NLogN needs a root size >= 0 (0: found from the bodies)");
+    if (!(p->theta >= 0.f)) fail(NBHIP_ERR_INVALID, "theta must be >= 0");
     if (b->systems.size() >= size_t(BH_BATCH_MAX_SYSTEMS))
         fail(NBHIP_ERR_UNSUPPORTED, "a Barnes-Hut batch holds at most %d systems", BH_BATCH_MAX_SYSTEMS);
     const size_t total = b->work->count + n;
@@ -166,7 +170,7 @@ void rebuild_tables(nbhip_bh_batch* b)
         d.size = sy.p.size > 0.f ? sy.p.size : 0.f;  // otherwise found from the bodies
         d.theta = sy.p.theta;
         d.opening = sy.p.opening;
-        d.leaf = std::max(1, sy.p.leaf_size);
+        d.leaf = sy.p.leaf_size;
         d.G = sy.p.G;
         d.eps2 = sy.p.eps * sy.p.eps;
         d.dt = sy.p.dt;
@@ -226,6 +230,28 @@ bool host_any_radius(const float* bodies, size_t n)
     for (size_t i = 0; i < n; ++i)
         if (bodies[i * 12 + 3] != 0.f) return true;
     return false;
+}
+
+template <typename T>
+size_t held(const DevBuf<T>& d)
+{
+    return d.cap * sizeof(T);
+}
+
+// Device memory the batch holds: its own tables and spare arrays, and every array of its
+// context (bodies, build buffers, sort state, nodes).
+size_t device_bytes(const nbhip_bh_batch* b)
+{
+    const nbhip_ctx* w = b->work;
+    size_t t = held(b->d_sys) + held(b->d_items) + held(b->d_waves) + held(b->roots) + held(b->ext) +
+               held(b->spare_vr) + held(b->spare_acc) + held(b->spare_order);
+    t += held(w->pm[0]) + held(w->pm[1]) + held(w->vr) + held(w->acc) + held(w->order) + held(w->aos);
+    t += held(w->bh_keys[0]) + held(w->bh_keys[1]) + held(w->bh_vals[0]) + held(w->bh_vals[1]);
+    t += held(w->bh_spm) + held(w->bh_svr) + held(w->bh_sorder) + held(w->bh_vcache) + held(w->bh_pref);
+    t += held(w->bh_base) + held(w->bh_kept) + held(w->bh_levels) + held(w->bh_nodes) + held(w->bh_top);
+    t += held(w->bh_rs_state) + held(w->bh_rs_status) + held(w->bh_uscan) + held(w->bh_dscan);
+    t += held(w->bh_visits) + held(w->bh_root) + held(w->bh_root_part);
+    return t;
 }
 
 const BhBatchSystem& find(const nbhip_bh_batch* b, int id)
@@ -485,16 +511,9 @@ int nbhip_bh_batch_query(nbhip_bh_batch* b, nbhip_bh_batch_info* out)
 {
     return nbhip::bh_batch_guarded(b, [&] {
         if (!out) nbhip::fail(NBHIP_ERR_INVALID, "null output");
-        const nbhip_ctx* w = b->work;
         out->systems = b->systems.size();
-        out->bodies = w->count;
-        size_t bytes = sizeof(float4) * (w->pm[0].cap + w->pm[1].cap + w->vr.cap + w->acc.cap + b->spare_vr.cap +
-                                         b->spare_acc.cap + w->bh_spm.cap);
-        bytes += sizeof(unsigned long long) * (w->bh_keys[0].cap + w->bh_keys[1].cap);
-        bytes += sizeof(unsigned) * (w->bh_vals[0].cap + w->bh_vals[1].cap + w->bh_base.cap + w->bh_visits.cap);
-        bytes += sizeof(nbhip::Dbl4) * w->bh_pref.cap + sizeof(nbhip::BhNode) * w->bh_nodes.cap;
-        bytes += sizeof(uint4) * w->bh_vcache.cap + w->bh_levels.cap + w->bh_kept.cap;
-        out->device_bytes = bytes;
+        out->bodies = b->work->count;
+        out->device_bytes = nbhip::device_bytes(b);
         out->levels = nbhip::BH_BATCH_LEVELS;
         out->max_systems = nbhip::BH_BATCH_MAX_SYSTEMS;
     });

@@ -239,8 +239,15 @@ def test_errors(ctx):
         batch.step(3)                      # an empty batch: nothing to do
         with pytest.raises(_lib.HipError):
             batch.add(b, _lib.params(mode=_lib.MODE_N2))
-        with pytest.raises(_lib.HipError):
-            batch.add(b, bh(eps=-1.0))
+        # the parameter checks of a single context's Barnes-Hut step
+        for bad in (bh(eps=-1.0), bh(leaf_size=0), bh(size=-1.0), bh(theta=-0.5), bh(theta=float("nan")),
+                    bh(opening=7)):
+            with pytest.raises(_lib.HipError):
+                batch.add(b, bad)
+            with pytest.raises(_lib.HipError):
+                ctx.upload_bodies(b)
+                ctx.step(bad, 1)
+        assert batch.info().systems == 0 and batch.info().bodies == 0
         i = batch.add(b, bh(size=64.0))
         with pytest.raises(_lib.HipError):
             batch.step(1, _lib.BATCH_COUNT)    # counting needs accelerations only
