@@ -429,17 +429,21 @@ def run_bh(args) -> dict:
     e1 = ctx.diagnostics(p)
     # the phases of a step, timed by HIP events on the context's stream
     build_ms, walk_ms = ctx.bh_profile(p, steps)
+    passes = ctx.bh_sort_passes()        # radix passes the profiled steps' sorts ran
     import ctypes
     cnt = ctypes.c_size_t()
     ctx._check(ctx.L.nbhip_bh_nodes(ctx.h, ctypes.byref(p), None, 0, ctypes.byref(cnt)))
     nodes = int(cnt.value)
     ctx.close()
     # algorithmic bytes of one build per body: keys 16 in + 12 out; radix sort 8 (the
-    # digit counts of all passes in one read) + 8 passes x 24; levels 8 in + ~1.07 out;
-    # gather (position, velocity, caller's index and the sort's value) 40 in + 36 out;
-    # moment scan 16 in + 32 out; next-smaller 4 out (365 B so far); node count and emit ~5
-    # (per-position counts and kept flags) -- 370 B -- plus the 32 B of every node written
-    build_bytes = 370.0 * n + 32.0 * nodes
+    # digit counts of all passes in one read) + 24 per pass run (8 for a full sort; the
+    # passes of skipped low digits read nothing), then the run fix-up's read of the keys (8);
+    # levels 8 in + ~1.07 out; gather (position, velocity, caller's index and the sort's
+    # value) 40 in + 36 out; moment scan 16 in + 32 out; next-smaller 4 out; node count and
+    # emit ~5 (per-position counts and kept flags) -- 186 B + 24 per pass -- plus the 32 B
+    # of every node written
+    per_body = 186.0 + 24.0 * passes
+    build_bytes = per_body * n + 32.0 * nodes
     build_gbs = build_bytes / (build_ms * 1e-3) / 1e9
     walk_tflops = FLOPS_PER_INTERACTION * inter / (walk_ms * 1e-3) / 1e12
     out = {
@@ -453,7 +457,8 @@ def run_bh(args) -> dict:
             "build": {"bound": "hbm", "achieved": build_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": build_gbs / HBM_PEAK_GBS, "ms": build_ms,
                       "algorithmic_bytes": build_bytes, "nodes": nodes,
-                      "bytes_formula": "370 B per body + 32 B per node"},
+                      "radix_passes": passes,
+                      "bytes_formula": f"{per_body:.0f} B per body (186 + 24 per radix pass run) + 32 B per node"},
             "walk": {"bound": "valu", "achieved": walk_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": walk_tflops / FP32_PEAK_TFLOPS, "ms": walk_ms,
                      "flops": f"{FLOPS_PER_INTERACTION} per body-node interaction"},

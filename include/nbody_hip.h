@@ -276,6 +276,13 @@ int nbhip_bh_profile(nbhip_ctx* ctx, const nbhip_params* p, int steps, double* b
  * current body count.  Synchronises. */
 int nbhip_bh_root(nbhip_ctx* ctx, float* size);
 
+/* No counterpart in the reference (its build sorts nothing: source/detail/tree.h:14-29
+ * inserts body by body).  The radix passes the last Barnes-Hut key sort of this context
+ * ran: 8 for a full sort, fewer when the keys' runs of equal prefixes were short enough at
+ * the sort before for the low-digit passes to be skipped (the sorted order is the same
+ * either way); 0 before any sort.  Synchronises. */
+int nbhip_bh_sort_passes(nbhip_ctx* ctx, int* passes);
+
 /* ---- diagnostics (FP64 reductions on the device) --------------------------------- */
 
 /* out[0] kinetic energy, out[1] potential energy (softened, with the radius cutoff),

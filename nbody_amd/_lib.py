@@ -138,6 +138,7 @@ SIGNATURES = {
     "nbhip_bh_tune": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, _vp, C.POINTER(C.c_int)]),
     "nbhip_bh_configure": (C.c_int, [_vp, C.c_int]),
     "nbhip_bh_root": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+    "nbhip_bh_sort_passes": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "nbhip_bh_profile": (C.c_int, [_vp, C.POINTER(Params), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "nbhip_diagnostics": (C.c_int, [_vp, C.POINTER(Params), _vp]),
     "nbhip_force_error_f64": (C.c_int, [_vp, C.POINTER(Params), _vp, _sz, _vp, _vp]),
@@ -393,6 +394,12 @@ class Context:
         b, w = C.c_double(), C.c_double()
         self._check(self.L.nbhip_bh_profile(self.h, C.byref(p), steps, C.byref(b), C.byref(w)))
         return b.value, w.value
+
+    def bh_sort_passes(self) -> int:
+        """Radix passes the last Barnes-Hut key sort ran (8: a full sort; 0: none yet)."""
+        v = C.c_int()
+        self._check(self.L.nbhip_bh_sort_passes(self.h, C.byref(v)))
+        return v.value
 
     def bh_root(self) -> float:
         """The root-cube edge of the last Barnes-Hut build (found from the bodies when the

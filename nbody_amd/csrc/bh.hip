@@ -366,15 +366,44 @@ void exclusive_scan_apply(nbhip_ctx* c, Src in, int n, T* out, const T* part, in
 // digits starts by looking back over the tiles before it (decoupled look-back: every tile
 // publishes its digit counts as soon as it has them, then its running totals), and scatters.
 //
+// Partial sorts.  A Barnes-Hut step's keys arrive in the last step's key order, and bodies
+// sharing a 64 - 8 s bit key prefix (s = 2..4: a cell 16-21 levels above the finest) are
+// few: at 2^20 disk + bulge bodies at most 6 share a 40-bit prefix.  When every run of
+// equal (64 - 8 s)-bit prefixes was at most RS_RUN_CAP long at the last sort, the passes of
+// the s low digits return at once, and k_rs_fix sorts each run of equal prefixes by (key,
+// value) afterwards.  The value is the input position, so the result is the full stable
+// sort's, bit for bit, whatever s is; s only decides how much work it takes.  k_rs_fix also
+// flags the prefixes with runs beyond the cap for the next sort (k_rs_bases chooses s from
+// them).  A run that grew past the cap since is still sorted, by one thread, more slowly.
+//
 // Sort state (bh_rs_state, unsigned words): the 8 x 256 digit counts, zero between sorts
 // (k_rs_bases clears them once read; zeroed once at allocation); the 8 x 256 digit starts;
-// 8 tile tickets, cleared by k_rs_bases; and a fault word.  bh_rs_status holds, per pass,
-// 256 look-back words per tile and then per super-tile, cleared by k_rs_count: bit 30 =
-// count published, bit 31 = running total (this and every earlier super-tile) published,
-// low 30 bits = the value.
+// 8 tile tickets, started by k_rs_bases; a fault word; the low passes this sort skips;
+// and flags of whether the last sort found runs of equal 32-, 40- and 48-bit prefixes
+// longer than the cap (all set after allocation and whenever the bodies are replaced: a
+// full sort first).  bh_rs_status
+// holds, per pass, 256 look-back words per tile and then per super-tile, cleared by
+// k_rs_count: bit 30 = count published, bit 31 = running total (this and every earlier
+// super-tile) published, low 30 bits = the value.
 constexpr int RS_PASSES = 8;
 constexpr int RS_COUNTS = 0, RS_BASES = RS_PASSES * 256, RS_TICKETS = 2 * RS_PASSES * 256,
-              RS_FAULT = RS_TICKETS + RS_PASSES, RS_WORDS = RS_FAULT + 8;
+              RS_FAULT = RS_TICKETS + RS_PASSES, RS_SKIP = RS_FAULT + 8;
+// Long-run flags: prefix j (keys >> (8 * rs_skip_of(j))) has a run longer than RS_RUN_CAP
+// when any of its RS_FLAG_SLOTS words is nonzero.  Block b of k_rs_fix sets slot b mod
+// RS_FLAG_SLOTS, each slot in its own 128-byte line, spreading the blocks' stores.
+constexpr int RS_PREFIXES = 3, RS_FLAG_SLOTS = 64, RS_FLAG_STRIDE = 32;
+constexpr int RS_FLAGS = RS_SKIP + RS_FLAG_STRIDE, RS_WORDS = RS_FLAGS + RS_PREFIXES * RS_FLAG_SLOTS * RS_FLAG_STRIDE;
+constexpr unsigned RS_RUN_CAP = 32;
+constexpr int RS_MAX_SKIP = 4;
+__host__ __device__ constexpr int rs_skip_of(int j) { return RS_MAX_SKIP - j; }
+// A pass's tile ticket starts at its sort's skip count (bits 24-30) and, for a skipped pass,
+// bit 31, so the ticket a block takes tells it both at once; tiles count in the low 24 bits.
+constexpr unsigned RS_TICKET_SKIPPED = 1u << 31, RS_TICKET_TILE = (1u << 24) - 1u;
+__host__ __device__ constexpr unsigned rs_ticket_base(int pass, int skip)
+{
+    return (unsigned(skip) << 24) | (pass < skip ? RS_TICKET_SKIPPED : 0u);
+}
+__host__ __device__ constexpr int rs_flag(int j, int slot) { return RS_FLAGS + (j * RS_FLAG_SLOTS + slot) * RS_FLAG_STRIDE; }
 constexpr unsigned LB_COUNT = 1u << 30, LB_TOTAL = 1u << 31, LB_VALUE = LB_COUNT - 1u;
 // Look-back waits end here, about a second in: the tile then takes zero for the unpublished
 // value and sets the fault word, which the next synchronisation reports; the sort is then
@@ -420,17 +449,50 @@ __global__ __launch_bounds__(RB) void k_rs_count(const unsigned long long* keys,
     for (size_t i = size_t(blockIdx.x) * RB + threadIdx.x; i < status_words; i += size_t(stride)) status[i] = 0u;
 }
 
-// Block p of 8: digit starts of pass p from its counts, which it clears; block 0 also clears
-// the tile tickets.
-__global__ __launch_bounds__(256) void k_rs_bases(unsigned* state)
+// Block p of 8: digit starts of pass p from its counts, which it clears; block 0 also
+// chooses the low passes this sort skips: the most whose prefix runs were all within
+// RS_RUN_CAP at the last sort (forced >= 0 overrides), clearing the flags for k_rs_fix to
+// set again, and starts the passes' tile tickets from it.  Wave j of block 0 reads prefix
+// j's slots.
+__global__ __launch_bounds__(256) void k_rs_bases(unsigned* state, int forced)
 {
+    static_assert(RS_FLAG_SLOTS == 64 && RS_PREFIXES <= 4, "one wave per prefix");
     __shared__ unsigned sh[256];
+    __shared__ unsigned any_long[4];
     const int t = threadIdx.x, p = blockIdx.x;
     const unsigned v = state[RS_COUNTS + p * 256 + t];
     state[RS_COUNTS + p * 256 + t] = 0u;
     unsigned total;
     state[RS_BASES + p * 256 + t] = block_exclusive_scan(v, sh, total);
-    if (p == 0 && t < RS_PASSES) state[RS_TICKETS + t] = 0u;
+    if (p != 0) return;
+    __shared__ int skip_sh;
+    const int j = t >> 6, slot = t & 63;
+    if (j < RS_PREFIXES) {
+        const bool set = state[rs_flag(j, slot)] != 0u;
+        state[rs_flag(j, slot)] = 0u;
+        const bool any = __ballot(set) != 0ull;
+        if (slot == 0) any_long[j] = any ? 1u : 0u;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int skip = 0;
+        if (forced >= 0) {
+            skip = forced;
+        } else {
+            for (int q = RS_PREFIXES - 1; q >= 0; --q)
+                if (any_long[q] == 0u) skip = rs_skip_of(q);
+        }
+        state[RS_SKIP] = unsigned(skip);
+        skip_sh = skip;
+    }
+    __syncthreads();
+    if (t < RS_PASSES) state[RS_TICKETS + t] = rs_ticket_base(t, skip_sh);
+}
+
+// Every prefix flagged: the next sort is a full one.
+__global__ void k_rs_invalidate(unsigned* state)
+{
+    if (threadIdx.x < RS_PREFIXES) state[rs_flag(threadIdx.x, 0)] = 1u;
 }
 
 __device__ __forceinline__ unsigned lb_load(unsigned* word)
@@ -451,19 +513,44 @@ __device__ __forceinline__ unsigned lb_wait(unsigned* word, unsigned* fault)
     }
 }
 
+// The sort's three key and value buffers: A holds the keys to sort and, after the last
+// pass, the sorted ones.  With R passes run (8 minus the skipped ones) and R even, the passes
+// go back and forth between A and B; with R odd, pass 0 goes from A to B, the passes between
+// go back and forth between B and T, and the last one from there to A.  The result is in A
+// whatever R is, and T is touched only when R is odd (at 2^23 bodies a third 100 MB buffer
+// in the traffic costs every pass about 15%).
+struct RsBufs {
+    unsigned long long* k[3];  // A, B, T
+    unsigned* v[3];
+};
+__device__ __forceinline__ int rs_in(int r, int R)
+{
+    if ((R & 1) == 0) return r & 1;  // A, B, A, ...
+    return r == 0 ? 0 : (r & 1) ? 1 : 2;
+}
+__device__ __forceinline__ int rs_out(int r, int R)
+{
+    if ((R & 1) == 0) return (r & 1) ^ 1;  // B, A, B, ..., A
+    return r == R - 1 ? 0 : (r & 1) ? 2 : 1;
+}
+
 // One pass.  Tiles are numbered by a ticket taken when the block starts, so every tile a
 // block waits for belongs to a block that is already running, and that block publishes its
-// counts before it waits for anything.  Wave w of the block owns items [64 IPT w, 64 IPT
+// counts before it waits for anything.  The ticket also says whether this pass is skipped
+// (it then returns before reading a key) and how many passes before it were.  Wave w of the block owns items [64 IPT w, 64 IPT
 // (w + 1)) of the tile, 64 per round in index order: lanes holding the same digit are found
 // with 8 ballots, and the wave's own LDS counters give each item its rank among the wave's
 // earlier items of that digit.  The waves' counts then give every wave its start within each
 // digit, and the tile's counts go to the two-level look-back below.
 template <int RB, int IPT, bool STAGE, unsigned LB_SUPER>
-__global__ __launch_bounds__(RB) void k_rs_pass(const unsigned long long* kin, const unsigned* vin,
-                                                unsigned long long* kout, unsigned* vout, int n, int shift,
-                                                const unsigned* bases, unsigned* ticket, unsigned* status,
-                                                unsigned* fault)
+__global__ __launch_bounds__(RB) void k_rs_pass(RsBufs bufs, int n, int pass, const unsigned* bases,
+                                                unsigned* ticket, unsigned* status, unsigned* fault,
+                                                const unsigned* skip_word)
 {
+    // a pass that can be skipped (the low four) reads the skip count first and, skipped,
+    // returns at once; the others learn it from their ticket without a wait of their own
+    if (pass < RS_MAX_SKIP && pass < int(*skip_word)) return;
+    const int shift = 8 * pass;
     static_assert(RB >= 256, "one thread per digit for the look-back");
     constexpr int WAVES = RB / 64, LB_WINDOW = 8;
     static_assert(LB_SUPER >= 2, "a super-tile of at least two tiles");
@@ -480,7 +567,15 @@ __global__ __launch_bounds__(RB) void k_rs_pass(const unsigned long long* kin, c
     if (t == 0) s_tile = atomicAdd(ticket, 1u);
     for (int i = t; i < WAVES * 256; i += RB) (&wh[0][0])[i] = 0u;
     __syncthreads();
-    const unsigned tile = s_tile;
+    const unsigned tk = s_tile;
+    if (tk & RS_TICKET_SKIPPED) return;
+    const unsigned tile = tk & RS_TICKET_TILE;
+    const int skip = int((tk >> 24) & 127u);
+    const int bi = rs_in(pass - skip, RS_PASSES - skip), bo = rs_out(pass - skip, RS_PASSES - skip);
+    const unsigned long long* kin = bi == 0 ? bufs.k[0] : bi == 1 ? bufs.k[1] : bufs.k[2];
+    const unsigned* vin = bi == 0 ? bufs.v[0] : bi == 1 ? bufs.v[1] : bufs.v[2];
+    unsigned long long* kout = bo == 0 ? bufs.k[0] : bo == 1 ? bufs.k[1] : bufs.k[2];
+    unsigned* vout = bo == 0 ? bufs.v[0] : bo == 1 ? bufs.v[1] : bufs.v[2];
     const int first = int(tile) * (RB * IPT) + wave * (64 * IPT) + lane;
     unsigned long long key[IPT];
     unsigned val[IPT], rank[IPT];
@@ -627,8 +722,7 @@ __global__ __launch_bounds__(RB) void k_rs_pass(const unsigned long long* kin, c
 struct SortShape {
     int rb, ipt;
     unsigned super;  // tiles per super-tile of the two-level look-back
-    void (*pass)(const unsigned long long*, const unsigned*, unsigned long long*, unsigned*, int, int,
-                 const unsigned*, unsigned*, unsigned*, unsigned*);
+    void (*pass)(RsBufs, int, int, const unsigned*, unsigned*, unsigned*, unsigned*, const unsigned*);
 };
 
 template <int RB, int IPT, bool STAGE, unsigned SUPER>
@@ -682,17 +776,141 @@ int rs_count_blocks(int n) { return std::max(1, std::min(256, (n + 4095) / 4096)
 
 // The sort's buffers for n keys; the state words are zeroed when first allocated (never
 // under graph capture: bh_reserve allocates before a capture starts).
+// The sort's buffers for n keys; the state words are set up when first allocated (never
+// under graph capture: bh_reserve allocates before a capture starts).
 void rs_reserve(nbhip_ctx* c, int n)
 {
     if (!c->bh_rs_state.ptr) {
         c->bh_rs_state.grow(RS_WORDS, 0, c->stream);
         NBHIP_CHECK(hipMemsetAsync(c->bh_rs_state.ptr, 0, RS_WORDS * sizeof(unsigned), c->stream));
+        hipLaunchKernelGGL(k_rs_invalidate, dim3(1), dim3(64), 0, c->stream, c->bh_rs_state.ptr);
+        NBHIP_CHECK(hipGetLastError());
     }
     c->bh_rs_status.grow(size_t(RS_PASSES) * 256 * rs_status_tiles(n), 0, c->stream);
 }
 
-// Sorts keys[src]/vals[src] over 63 key bits; returns the buffer index of the result.
-int radix_sort(nbhip_ctx* c, int n, int src)
+// NBHIP_RS_SKIP = 0..4 fixes the low passes every sort skips (the result is the same);
+// otherwise each sort chooses from the last one's runs.  Read when a sort is enqueued (a
+// captured graph keeps the value it was captured with).
+int rs_forced_skip()
+{
+    const char* e = std::getenv("NBHIP_RS_SKIP");
+    const int k = e && *e ? std::atoi(e) : -1;
+    return (k >= 0 && k <= RS_MAX_SKIP) ? k : -1;
+}
+
+// Sorts positions [a, b) of (K, V) by (key, value).  The values are the input positions, so
+// this is the order the full stable sort gives.  Already in order: nothing; up to 64
+// entries: insertion; beyond: heapsort (the values are distinct, so the order is unique).
+__device__ __forceinline__ bool kv_less(unsigned long long ka, unsigned va, unsigned long long kb, unsigned vb)
+{
+    return ka < kb || (ka == kb && va < vb);
+}
+
+__device__ void sort_key_run(unsigned long long* K, unsigned* V, int a, int b)
+{
+    bool sorted = true;
+    for (int i = a + 1; i < b && sorted; ++i) sorted = !kv_less(K[i], V[i], K[i - 1], V[i - 1]);
+    if (sorted) return;
+    const int len = b - a;
+    if (len <= 64) {
+        for (int i = a + 1; i < b; ++i) {
+            const unsigned long long ki = K[i];
+            const unsigned vi = V[i];
+            int j = i - 1;
+            while (j >= a && kv_less(ki, vi, K[j], V[j])) {
+                K[j + 1] = K[j];
+                V[j + 1] = V[j];
+                --j;
+            }
+            K[j + 1] = ki;
+            V[j + 1] = vi;
+        }
+        return;
+    }
+    const auto less = [&](int x, int y) { return kv_less(K[a + x], V[a + x], K[a + y], V[a + y]); };
+    const auto swap = [&](int x, int y) {
+        const unsigned long long tk = K[a + x];
+        K[a + x] = K[a + y];
+        K[a + y] = tk;
+        const unsigned tv = V[a + x];
+        V[a + x] = V[a + y];
+        V[a + y] = tv;
+    };
+    const auto sift = [&](int root, int end) {
+        for (;;) {
+            int child = 2 * root + 1;
+            if (child >= end) return;
+            if (child + 1 < end && less(child, child + 1)) ++child;
+            if (!less(root, child)) return;
+            swap(root, child);
+            root = child;
+        }
+    };
+    for (int r = len / 2 - 1; r >= 0; --r) sift(r, len);
+    for (int end = len - 1; end > 0; --end) {
+        swap(0, end);
+        sift(0, end);
+    }
+}
+
+// After the passes: the runs of equal (64 - 8 skip)-bit prefixes, which the skipped passes
+// left in input order, are sorted by their first position's thread (no other thread writes
+// inside them), and for each measured prefix a flag records whether any run of it is longer
+// than RS_RUN_CAP, for the next sort.  A prefix at least as coarse as this sort's has its
+// runs contiguous already (sorting inside a finer run does not move their bounds), so
+// position i is inside a long run of it exactly when key i + RS_RUN_CAP shares it.  A finer
+// prefix can only have a long run inside a long run of this sort's, whose thread checks it
+// after sorting.  Most threads thus read four keys, all at once, and walk nothing.
+__global__ __launch_bounds__(256) void k_rs_fix(unsigned long long* K, unsigned* V, int n, unsigned* state)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int skip = int(state[RS_SKIP]);
+    bool long_run[RS_PREFIXES] = { false, false, false };
+    if (i < n) {
+        const unsigned long long k = K[i];
+        const unsigned long long kp = i > 0 ? K[i - 1] : ~k;  // position 0 starts every run
+        const unsigned long long kn = i + 1 < n ? K[i + 1] : ~k;
+        const bool far = i + int(RS_RUN_CAP) < n;
+        const unsigned long long kf = far ? K[i + RS_RUN_CAP] : ~k;
+        for (int j = 0; j < RS_PREFIXES; ++j) {
+            const int sj = rs_skip_of(j);
+            if (sj >= skip) long_run[j] = far && (k >> (8 * sj)) == (kf >> (8 * sj));
+        }
+        const int sh = 8 * skip;
+        if (skip > 0 && (k >> sh) != (kp >> sh) && (kn >> sh) == (k >> sh)) {
+            // i starts a run of two or more the skipped passes left unsorted
+            const unsigned long long top = k >> sh;
+            int e = i + 2;
+            while (e < n && (K[e] >> sh) == top) ++e;
+            sort_key_run(K, V, i, e);
+            if (e - i > int(RS_RUN_CAP)) {
+                for (int j = 0; j < RS_PREFIXES; ++j) {
+                    const int sj = rs_skip_of(j);
+                    if (sj >= skip) continue;
+                    for (int q = i; q + int(RS_RUN_CAP) < e && !long_run[j]; ++q)
+                        long_run[j] = (K[q] >> (8 * sj)) == (K[q + RS_RUN_CAP] >> (8 * sj));
+                }
+            }
+        }
+    }
+    // one flag per block and prefix, into the block's slot, by a plain store: every writer
+    // stores the same value and the next kernel reads it, so no atomic is needed; long runs
+    // are common at the coarse prefixes, and atomics from every block on a few words would
+    // queue at memory
+    __shared__ unsigned flags[RS_PREFIXES];
+    if (threadIdx.x < RS_PREFIXES) flags[threadIdx.x] = 0u;
+    __syncthreads();
+    for (int j = 0; j < RS_PREFIXES; ++j)
+        if (__ballot(long_run[j]) != 0ull && (threadIdx.x & 63) == 0) flags[j] = 1u;
+    __syncthreads();
+    if (threadIdx.x < RS_PREFIXES && flags[threadIdx.x] != 0u)
+        state[rs_flag(int(threadIdx.x), int(blockIdx.x % RS_FLAG_SLOTS))] = 1u;
+}
+
+// Sorts bh_keys[0]/bh_vals[0] (63 key bits, stable) in place, bh_keys/vals[1] and [2] carrying
+// the passes; returns the buffer index of the result (0).
+int radix_sort(nbhip_ctx* c, int n)
 {
     // look-back words carry 30-bit positions; 2^30 bodies would need over 288 GB here anyway
     if (n >= int(LB_COUNT)) fail(NBHIP_ERR_INVALID, "Barnes-Hut sorts at most 2^30 - 1 bodies, not %d", n);
@@ -703,20 +921,54 @@ int radix_sort(nbhip_ctx* c, int n, int src)
     const size_t per_pass = size_t(256) * rs_status_tiles(n);
     const size_t words = RS_PASSES * per_pass;
     hipLaunchKernelGGL(k_rs_count<RS_COUNT_BLOCK>, dim3(rs_count_blocks(n)), dim3(RS_COUNT_BLOCK), 0, c->stream,
-                       c->bh_keys[src].ptr, n, st + RS_COUNTS, c->bh_rs_status.ptr, words);
+                       c->bh_keys[0].ptr, n, st + RS_COUNTS, c->bh_rs_status.ptr, words);
     NBHIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_rs_bases, dim3(RS_PASSES), dim3(256), 0, c->stream, st);
+    hipLaunchKernelGGL(k_rs_bases, dim3(RS_PASSES), dim3(256), 0, c->stream, st, rs_forced_skip());
     NBHIP_CHECK(hipGetLastError());
+    const RsBufs bufs{ { c->bh_keys[0].ptr, c->bh_keys[1].ptr, c->bh_keys[2].ptr },
+                       { c->bh_vals[0].ptr, c->bh_vals[1].ptr, c->bh_vals[2].ptr } };
     for (int pass = 0; pass < RS_PASSES; ++pass) {
-        const int dst = 1 - src;
-        hipLaunchKernelGGL(sh.pass, dim3(nb), dim3(sh.rb), 0, c->stream, c->bh_keys[src].ptr, c->bh_vals[src].ptr,
-                           c->bh_keys[dst].ptr, c->bh_vals[dst].ptr, n, 8 * pass, st + RS_BASES + 256 * pass,
-                           st + RS_TICKETS + pass, c->bh_rs_status.ptr + per_pass * pass, st + RS_FAULT);
+        hipLaunchKernelGGL(sh.pass, dim3(nb), dim3(sh.rb), 0, c->stream, bufs, n, pass, st + RS_BASES + 256 * pass,
+                           st + RS_TICKETS + pass, c->bh_rs_status.ptr + per_pass * pass, st + RS_FAULT,
+                           st + RS_SKIP);
         NBHIP_CHECK(hipGetLastError());
-        src = dst;
     }
-    return src;
+    hipLaunchKernelGGL(k_rs_fix, dim3(unsigned((n + 255) / 256)), dim3(256), 0, c->stream, c->bh_keys[0].ptr,
+                       c->bh_vals[0].ptr, n, st);
+    NBHIP_CHECK(hipGetLastError());
+    return 0;
 }
+
+}  // namespace
+
+// The bodies were replaced: the next sort is a full one (it measures the runs again).
+// Never under stream capture (a replayed graph would reset the statistics every time).
+void bh_sort_invalidate(nbhip_ctx* c)
+{
+    if (!c->bh_rs_state.ptr) return;  // allocation sets the state up
+    hipLaunchKernelGGL(k_rs_invalidate, dim3(1), dim3(64), 0, c->stream, c->bh_rs_state.ptr);
+    NBHIP_CHECK(hipGetLastError());
+}
+
+// The radix passes the last sort ran (0 before any).
+int bh_sort_passes(nbhip_ctx* c)
+{
+    if (!c->bh_rs_state.ptr) return 0;
+    unsigned skip = 0;
+    NBHIP_CHECK(hipMemcpyAsync(&skip, c->bh_rs_state.ptr + RS_SKIP, sizeof skip, hipMemcpyDeviceToHost, c->stream));
+    NBHIP_CHECK(hipStreamSynchronize(c->stream));
+    return RS_PASSES - int(skip);
+}
+
+// A different root size changes every key: the same, when size differs from the last one.
+void bh_note_root(nbhip_ctx* c, float size)
+{
+    if (size == c->rs_root) return;
+    bh_sort_invalidate(c);
+    c->rs_root = size;
+}
+
+namespace {
 
 // ---- 3. bodies in key order + FP64 moments -------------------------------------------------
 // Each block gathers one scan tile (coalesced: position tile + i * SC_BLOCK + thread) of
@@ -1998,10 +2250,10 @@ void check_nlogn_count(size_t n)
 void bh_grow(nbhip_ctx* c, int n, int extra)
 {
     hipStream_t s = c->stream;
-    c->bh_keys[0].grow(n, 0, s);
-    c->bh_keys[1].grow(n, 0, s);
-    c->bh_vals[0].grow(n, 0, s);
-    c->bh_vals[1].grow(n, 0, s);
+    for (int i = 0; i < 3; ++i) {
+        c->bh_keys[i].grow(n, 0, s);
+        c->bh_vals[i].grow(n, 0, s);
+    }
     c->bh_spm.grow(n, 0, s);
     c->bh_svr.grow(n, 0, s);
     c->bh_sorder.grow(n, 0, s);
@@ -2154,12 +2406,15 @@ void bh_build(nbhip_ctx* c, const nbhip_params& p)
     const int n = int(c->count);
     hipStream_t s = c->stream;
     bh_grow(c, n, 0);
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    NBHIP_CHECK(hipStreamIsCapturing(s, &capture));
+    if (capture == hipStreamCaptureStatusNone) bh_note_root(c, p.size);  // a capture noted it first
 
     const float* root = bh_root(c, p, n);
     hipLaunchKernelGGL(k_keys, dim3(grid(n)), dim3(256), 0, s, c->pm[c->cur].ptr, n, p.size, root,
                        c->bh_keys[0].ptr, c->bh_vals[0].ptr);
     NBHIP_CHECK(hipGetLastError());
-    const int sorted = radix_sort(c, n, 0);
+    const int sorted = radix_sort(c, n);
     c->bh_sorted = sorted;
     const unsigned long long* keys = c->bh_keys[sorted].ptr;
     unsigned* perm = c->bh_vals[sorted].ptr;
@@ -2212,7 +2467,7 @@ void bh_batch_build(nbhip_ctx* c, const BhBatchView& v)
     hipLaunchKernelGGL(k_bkeys, dim3(v.nitems), dim3(256), 0, s, pm, v.sys, v.items, v.roots, c->bh_keys[0].ptr,
                        c->bh_vals[0].ptr);
     NBHIP_CHECK(hipGetLastError());
-    const int sorted = radix_sort(c, n, 0);
+    const int sorted = radix_sort(c, n);
     c->bh_sorted = sorted;
     const unsigned long long* keys = c->bh_keys[sorted].ptr;
     unsigned* perm = c->bh_vals[sorted].ptr;
@@ -2372,7 +2627,7 @@ void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm
     check_nlogn_count(c->count);
     const int n = int(c->count);
     hipStream_t s = c->stream;
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < 3; ++i) {
         c->bh_keys[i].grow(n, 0, s);
         c->bh_vals[i].grow(n, 0, s);
     }
@@ -2385,7 +2640,7 @@ void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm
                                    hipMemcpyDeviceToHost, s));
         NBHIP_CHECK(hipStreamSynchronize(s));
     }
-    const int sorted = radix_sort(c, n, 0);
+    const int sorted = radix_sort(c, n);
     c->bh_built = false;  // the key buffers no longer belong to the last build
     if (perm) {
         NBHIP_CHECK(hipMemcpyAsync(perm, c->bh_vals[sorted].ptr, size_t(n) * sizeof(uint32_t),
@@ -2472,7 +2727,7 @@ void restore_order(nbhip_ctx* c)
 
 void bh_release(nbhip_ctx* c)
 {
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < 3; ++i) {
         c->bh_keys[i].release();
         c->bh_vals[i].release();
     }

@@ -147,6 +147,7 @@ void join(nbhip_bh_batch* b, BhBatchSystem sy, int* id)
 {
     sy.id = b->next_id++;
     b->work->count = sy.off + sy.n;
+    bh_sort_invalidate(b->work);  // new keys: the next sort is a full one
     if (id) *id = sy.id;
     b->systems.emplace(sy.id, sy);
     b->dirty = true;
@@ -246,7 +247,7 @@ size_t device_bytes(const nbhip_bh_batch* b)
     size_t t = held(b->d_sys) + held(b->d_items) + held(b->d_waves) + held(b->roots) + held(b->ext) +
                held(b->spare_vr) + held(b->spare_acc) + held(b->spare_order);
     t += held(w->pm[0]) + held(w->pm[1]) + held(w->vr) + held(w->acc) + held(w->order) + held(w->aos);
-    t += held(w->bh_keys[0]) + held(w->bh_keys[1]) + held(w->bh_vals[0]) + held(w->bh_vals[1]);
+    for (int i = 0; i < 3; ++i) t += held(w->bh_keys[i]) + held(w->bh_vals[i]);
     t += held(w->bh_spm) + held(w->bh_svr) + held(w->bh_sorder) + held(w->bh_vcache) + held(w->bh_pref);
     t += held(w->bh_base) + held(w->bh_kept) + held(w->bh_levels) + held(w->bh_nodes) + held(w->bh_top);
     t += held(w->bh_rs_state) + held(w->bh_rs_status) + held(w->bh_uscan) + held(w->bh_dscan);
@@ -364,6 +365,7 @@ int nbhip_bh_batch_remove(nbhip_bh_batch* b, int id)
         std::swap(w->order, b->spare_order);
         nbhip::g_buffer_epoch.fetch_add(1);
         w->count = total;
+        nbhip::bh_sort_invalidate(w);
         b->dirty = true;
     });
 }

@@ -206,8 +206,8 @@ struct nbhip_ctx {
 
     // Barnes-Hut: keys + permutation (double-buffered for the radix sort), bodies in key
     // order, FP64 prefix sums, per-position node counts / pre-order bases, the nodes
-    nbhip::DevBuf<unsigned long long> bh_keys[2];
-    nbhip::DevBuf<unsigned> bh_vals[2];
+    nbhip::DevBuf<unsigned long long> bh_keys[3];
+    nbhip::DevBuf<unsigned> bh_vals[3];
     nbhip::DevBuf<float4> bh_spm;
     nbhip::DevBuf<float4> bh_svr;          // velocities + radii in key order
     nbhip::DevBuf<unsigned> bh_sorder;     // the caller's index of each body in key order
@@ -229,6 +229,7 @@ struct nbhip_ctx {
     int bh_split = 0;           // waves per 64-body group in the walk (0: by body count)
     bool bh_built = false;      // a tree of the current count exists on the device
     float bh_size = 0.f;        // root edge of that tree
+    float rs_root = -1.f;       // the root size the sort's run statistics were measured with
     unsigned long long bh_builds = 0;  // builds enqueued so far (graph replays included)
 
     // the last build in the reference's node layout (nbhip_bh_tree), made on demand
@@ -296,6 +297,9 @@ void bh_walk(nbhip_ctx* c, const nbhip_params& p, bool fuse_integrate, bool coun
 void bh_profile_steps(nbhip_ctx* c, const nbhip_params& p, int reps, double* build_ms, double* walk_ms);
 void bh_release(nbhip_ctx* c);
 void bh_keys(nbhip_ctx* c, const nbhip_params& p, uint64_t* keys, uint32_t* perm);
+void bh_sort_invalidate(nbhip_ctx* c);
+void bh_note_root(nbhip_ctx* c, float size);
+int bh_sort_passes(nbhip_ctx* c);
 void bh_reserve(nbhip_ctx* c);
 // Puts the bodies back in the caller's order (after Barnes-Hut walks), stream-ordered.
 void restore_order(nbhip_ctx* c);

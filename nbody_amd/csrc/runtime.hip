@@ -216,6 +216,7 @@ int nbhip_reserve(nbhip_ctx* c, size_t n)
         c->order.grow(n, 0, c->stream);
         nbhip::identity_order(c->stream, c->order.ptr, n);
         if (n != c->count) c->bh_built = false;
+        nbhip::bh_sort_invalidate(c);  // new bodies: the next Barnes-Hut sort is a full one
         c->count = n;
         c->radius_known = false;
         c->tuned_n = 0;
@@ -237,9 +238,11 @@ int nbhip_upload(nbhip_ctx* c, const float* pos_mass, const float* vel_radius, c
         NBHIP_CHECK(hipSetDevice(c->device));
         nbhip::restore_order(c);
         const size_t bytes = n * sizeof(float4);
-        if (pos_mass)
+        if (pos_mass) {
             NBHIP_CHECK(hipMemcpyAsync(c->pm[c->cur].ptr + offset, pos_mass, bytes,
                                        hipMemcpyHostToDevice, c->stream));
+            nbhip::bh_sort_invalidate(c);
+        }
         if (vel_radius) {
             NBHIP_CHECK(hipMemcpyAsync(c->vr.ptr + offset, vel_radius, bytes,
                                        hipMemcpyHostToDevice, c->stream));
@@ -458,6 +461,7 @@ void step_graph(nbhip_ctx* c, const nbhip_params& p, int steps)
         nbhip::direct_prepare_context(c, cfg);
     } else {
         nbhip::bh_reserve(c);
+        nbhip::bh_note_root(c, p.size);  // before the capture, which must not record it
     }
     char key[256];
     std::snprintf(key, sizeof key, "%d/%d/%zu/%d/%d/%d/%d/%d/%d/%d/%llu/", p.mode, steps, c->count, c->cur,
@@ -729,6 +733,16 @@ int nbhip_bh_root(nbhip_ctx* c, float* size)
         if (!size) fail(NBHIP_ERR_INVALID, "null output");
         NBHIP_CHECK(hipSetDevice(c->device));
         *size = nbhip::bh_root_size(c);
+    });
+}
+
+int nbhip_bh_sort_passes(nbhip_ctx* c, int* passes)
+{
+    return guarded(c, [&] {
+        check_ctx(c);
+        if (!passes) fail(NBHIP_ERR_INVALID, "null output");
+        NBHIP_CHECK(hipSetDevice(c->device));
+        *passes = nbhip::bh_sort_passes(c);
     });
 }
 

@@ -233,6 +233,32 @@ def test_many_one_body_systems_and_limits(ctx):
             assert np.array_equal(out[0, 0:3], x)
 
 
+def test_partial_sorts_leave_the_same_bodies(ctx, monkeypatch):
+    # The batch's one sort over every system's keys may skip low-digit passes (fixing the
+    # runs they leave): every fixed skip and the adaptive choice give the full sort's steps,
+    # bit for bit, across a join and a removal (each of which makes the next sort a full one)
+    members = systems()
+    res = {}
+    for skip in ("0", "2", "4", ""):
+        if skip:
+            monkeypatch.setenv("NBHIP_RS_SKIP", skip)
+        else:
+            monkeypatch.delenv("NBHIP_RS_SKIP", raising=False)
+        with _lib.BhBatch(ctx) as batch:
+            ids = {name: batch.add(b, p) for name, b, p in members[:5]}
+            batch.step(3)
+            batch.step(3, _lib.BATCH_GRAPH)
+            name, b, p = members[5]
+            ids[name] = batch.add(b, p)
+            batch.step(2, _lib.BATCH_GRAPH)
+            batch.remove(ids.pop("disk_wrap"))
+            batch.step(2)
+            res[skip] = {name: batch.download(i) for name, i in ids.items()}
+    for k, v in res.items():
+        for name, out in v.items():
+            assert np.array_equal(out.view(np.uint32), res["0"][name].view(np.uint32)), (k, name)
+
+
 def test_errors(ctx):
     b = ics.plummer(100, seed=1)
     with _lib.BhBatch(ctx) as batch:

@@ -69,6 +69,77 @@ def test_keys_and_permutation_bit_exact(ctx, orc, n, size):
     assert np.array_equal(perm, orc.stable_sort(ok))
 
 
+@pytest.mark.parametrize("skip", [0, 1, 2, 3, 4])
+def test_partial_sorts_equal_the_full_sort(ctx, orc, monkeypatch, skip):
+    # A sort may skip up to four low-digit passes; k_rs_fix then sorts the runs of equal key
+    # prefixes those leave in input order, by (key, input position).  Whatever the number
+    # skipped, keys and permutation equal the oracle's stable sort bit for bit: short runs
+    # (disk + bulge: insertion), and runs far beyond the cap (a dense cluster under a wide
+    # root: the heapsort path), with ties.
+    monkeypatch.setenv("NBHIP_RS_SKIP", str(skip))
+    cluster = ics.plummer(30000, seed=6, a=0.02)
+    cluster[11, 0:3] = cluster[4, 0:3]
+    for name, b, size in (("disk", ics.disk_bulge(100000, seed=5), 128.0), ("cluster", cluster, 10000.0)):
+        ctx.upload_bodies(b)
+        keys, perm = ctx.bh_keys(bh_params(size=size))
+        ok = orc.morton_keys(b, size, 21)
+        assert np.array_equal(keys, ok), name
+        assert np.array_equal(perm, orc.stable_sort(ok)), name
+
+
+def test_partial_sort_steps_equal_full_sort_steps(monkeypatch):
+    # Steps with every sort a full one (NBHIP_RS_SKIP=0), with each fixed skip, and with the
+    # skip chosen from the last sort's runs (unset) leave the same bodies, bit for bit,
+    # eagerly and from graphs; so does a cluster whose runs exceed the cap, and a switch of
+    # root size (all keys change) between steps.
+    disk = ics.disk_bulge(60000, seed=9)
+    cluster = ics.plummer(20000, seed=10, a=0.02)
+    p = bh_params(size=128.0, theta=0.5, eps=0.01, dt=1e-3, leaf_size=16, wrap=0)
+    q = bh_params(size=10000.0, theta=0.5, eps=1e-3, dt=2e-3, leaf_size=1, wrap=0)
+    res = {}
+    for skip in ("0", "2", "3", "4", ""):
+        if skip:
+            monkeypatch.setenv("NBHIP_RS_SKIP", skip)
+        else:
+            monkeypatch.delenv("NBHIP_RS_SKIP", raising=False)
+        out = []
+        with _lib.Context(0) as c:
+            c.upload_bodies(disk)
+            c.step(p, 3)
+            c.step_graph(p, 4)
+            c.step_graph(p, 4)
+            out.append(c.download_bodies())
+            c.step(bh_params(size=256.0, theta=0.5, eps=0.01, dt=1e-3, leaf_size=16, wrap=0), 2)
+            out.append(c.download_bodies())
+            c.upload_bodies(cluster)
+            c.step(q, 3)
+            c.step_graph(q, 3)
+            out.append(c.download_bodies())
+        res[skip] = out
+    for k, v in res.items():
+        for a, ref in zip(v, res["0"]):
+            assert np.array_equal(a.view(np.uint32), ref.view(np.uint32)), k
+
+
+def test_adaptive_sort_skips_passes_on_key_ordered_steps(ctx, monkeypatch):
+    # New bodies get a full sort; once a step has left them in key order with short runs
+    # of equal key prefixes (disk + bulge under a 128 root: at most a few bodies share a
+    # 40-bit prefix), the sorts skip low-digit passes; new bodies again: a full sort.
+    monkeypatch.delenv("NBHIP_RS_SKIP", raising=False)
+    b = ics.disk_bulge(1 << 18, seed=12)
+    p = bh_params(size=128.0, theta=0.5, eps=0.01, dt=1e-3, leaf_size=16, wrap=0)
+    ctx.upload_bodies(b)
+    ctx.step(p, 1)
+    assert ctx.bh_sort_passes() == 8
+    ctx.step(p, 2)
+    assert ctx.bh_sort_passes() < 8
+    ctx.step_graph(p, 3)
+    assert ctx.bh_sort_passes() < 8
+    ctx.upload_bodies(b)
+    ctx.step(p, 1)
+    assert ctx.bh_sort_passes() == 8
+
+
 def check_tree(nodes, perm, b):
     """Structural invariants of the pre-order tree."""
     total = nodes.size
