@@ -24,6 +24,7 @@ def alone(job, bh_mode):
             bodies = ctx.download_bodies()
         if job.mode == _lib.MODE_N2:
             with _lib.Batch(ctx) as b:
+                b.configure(*serve.DIRECT_BATCH_SHAPE)   # the server's launch shape
                 sid = b.add(bodies, job.params)
                 b.step(job.steps)
                 return b.download(sid)
