@@ -308,7 +308,8 @@ typedef struct nbhip_bh_node {
     float    com[3];   /* centre of mass (accumulated in FP64)                        */
     float    mass;
     uint32_t next;     /* pre-order index of the first node after this subtree        */
-    float    open_sq;  /* applied as one mass when |com - x|^2 > open_sq              */
+    float    open_sq;  /* applied as one mass when |com - x|^2 > open_sq; -inf for a
+                          leaf and for a node of zero mass (never opened)              */
     uint32_t first;    /* leaf / bucket: first body, as a position in key order       */
     uint32_t count;    /* 0 = internal node; >= 1 = leaf holding `count` bodies       */
 } nbhip_bh_node;

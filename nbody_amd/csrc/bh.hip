@@ -1663,6 +1663,8 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
         for (int l = e.root_level; l < lvl; ++l) s = s * .5f;
         const float s2 = s * s;
         nd.thr = opening == NBHIP_OPEN_TEXTBOOK ? s2 / theta_sq : s2 * theta_sq;
+        // a node of zero mass passes every test: never opened, and its pull is zero
+        if (nd.cm.w == 0.f) nd.thr = -INFINITY;
         nd.next = next;
         nd.first = unsigned(a);
         nd.count = bucket ? unsigned(b - a + 1) : 0u;
@@ -1702,7 +1704,7 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs e)
         if (slot < e.capacity) {
             BhNode nd;
             nd.cm = e.spm[a];
-            nd.thr = 0.f;
+            nd.thr = -INFINITY;  // a leaf is always applied
             nd.next = e.base[a + 1];
             nd.first = unsigned(a);
             nd.count = 1;
@@ -1845,10 +1847,26 @@ __device__ __forceinline__ void store_stream(float4* p, const float4 v)
     __builtin_nontemporal_store(F4v{ v.x, v.y, v.z, v.w }, reinterpret_cast<F4v*>(p));
 }
 
+// Node i of the pre-order array.  Below 2^27 nodes the byte offset fits 32 bits, and the
+// scalar load takes it as a register offset: one shift per step instead of a 64-bit
+// multiply-add.
+template <bool WIDE>
+__device__ __forceinline__ BhNode load_node(const BhNode* __restrict__ nodes, unsigned i)
+{
+    if constexpr (WIDE) return nodes[i];
+    return *reinterpret_cast<const BhNode*>(reinterpret_cast<const char*>(nodes) + (i << 5));
+}
+
+// Whether a node array of `capacity` entries needs 64-bit node addressing in the walk.
+inline bool walk_wide(size_t capacity)
+{
+    return capacity > (size_t(1) << 27);
+}
+
 // One lane's reference traversal of the pre-order run [its start, end): idx is the lane's
 // first node (end for a lane with no body), and the wave visits the union of its lanes'
 // traversals in pre-order.  Interactions go to acc; COUNT also counts them in applied.
-template <bool CUT, bool COUNT>
+template <bool CUT, bool COUNT, bool WIDE>
 __device__ __forceinline__ void walk_run(const BhNode* __restrict__ nodes, const float4* __restrict__ spm, int n,
                                          float eps2, const float4 x, const float r2, unsigned idx,
                                          const unsigned end, Accum& acc, unsigned& applied)
@@ -1857,19 +1875,25 @@ __device__ __forceinline__ void walk_run(const BhNode* __restrict__ nodes, const
     // opened node's first child, next in pre-order) that is the new minimum and the
     // wave-wide reduction is skipped.  The node is wave-uniform (scalar registers); the
     // per-lane choices -- apply it, open it, apply a bucket's bodies -- are predicates on
-    // straight-line code rather than branches.  Partial sums fold into the compensated
-    // totals every 8 steps of the wave (at most 8 nodes or buckets per lane in between).
+    // straight-line code rather than branches.  The scalar unit, one instruction per cycle
+    // shared by the CU's four SIMDs, bounds this loop as much as the vector unit does, so
+    // the step keeps its scalar work small: the node's address is one shift, and one
+    // compare decides every node kind.  Partial sums fold into the compensated totals
+    // every 8 steps of the wave (at most 8 nodes or buckets per lane in between).
     unsigned cur = __ockl_wfred_min_u32(idx);
-    unsigned steps = 0;
+    unsigned clock = 0u;  // wraps to zero on every 8th step
     while (cur < end) {
-        const BhNode nd = nodes[cur];
+        const BhNode nd = load_node<WIDE>(nodes, cur);
+        const unsigned step = cur + 1;
+        // pre-order pointers only move forward
+        const unsigned skip = max(nd.next, step);
         const bool at = idx == cur;
-        // empty nodes are skipped (reference bhtree.cpp:108-113); a leaf is applied
-        // directly; otherwise the reference's inverted criterion decides
-        // (bhtree.cpp:125-128), on the distance the force then uses
-        unsigned mbits = __float_as_uint(nd.cm.w);
-        asm("" : "+s"(mbits));  // a scalar test (the node is wave-uniform), not a per-lane compare
-        const bool on = at && (mbits << 1) != 0u;
+        // a leaf is applied directly (reference bhtree.cpp:118-122); otherwise the
+        // reference's inverted criterion decides (bhtree.cpp:125-128), on the distance the
+        // force then uses.  Leaves and nodes of zero mass carry the threshold -inf, so they
+        // always pass; an empty node's pull is then exactly zero (its weight is zero and
+        // d2 + eps2 > 0 wherever no radius cutoff removes d2 = 0), which is the reference's
+        // skipping of empty nodes (bhtree.cpp:108-113).
         float dx, dy, dz, d2;
         {
 #pragma clang fp contract(off)
@@ -1878,41 +1902,44 @@ __device__ __forceinline__ void walk_run(const BhNode* __restrict__ nodes, const
             dz = nd.cm.z - x.z;
             d2 = ((dx * dx) + (dy * dy)) + (dz * dz);
         }
-        const bool agg = (nd.count == 1) | (d2 > nd.thr);
-        interact_d2<CUT>(on && agg, dx, dy, dz, d2, nd.cm.w, r2, eps2, acc.tx, acc.ty, acc.tz);
-        if (COUNT) applied += (on && agg) ? 1u : 0u;
-        const bool opened = on && !agg;
-        // open an internal node: its first child follows in pre-order
-        const unsigned nxt = (nd.count == 0 && opened) ? cur + 1 : nd.next;
-        // an opened bucket, body by body, four bodies' loads in flight at a time; the
-        // trip count is wave-uniform and zero unless some lane opened a bucket
-        const unsigned nb = (nd.count > 1 && __ballot(opened) != 0ull) ? min(nd.count, unsigned(n) - nd.first) : 0u;
-        unsigned j = 0;
-        for (; j + 4 <= nb; j += 4) {
-            const unsigned b = nd.first + j;
-            const float4 q0 = spm[b], q1 = spm[b + 1], q2 = spm[b + 2], q3 = spm[b + 3];
-            interact_if<CUT>(opened, x, r2, q0, eps2, acc.tx, acc.ty, acc.tz);
-            interact_if<CUT>(opened, x, r2, q1, eps2, acc.tx, acc.ty, acc.tz);
-            interact_if<CUT>(opened, x, r2, q2, eps2, acc.tx, acc.ty, acc.tz);
-            interact_if<CUT>(opened, x, r2, q3, eps2, acc.tx, acc.ty, acc.tz);
+        const bool pass = d2 > nd.thr;
+        interact_d2<CUT>(at && pass, dx, dy, dz, d2, nd.cm.w, r2, eps2, acc.tx, acc.ty, acc.tz);
+        if (COUNT) applied += (at && pass && nd.cm.w != 0.f) ? 1u : 0u;
+        if (nd.count > 1u) {
+            // an opened bucket, body by body, four bodies' loads in flight at a time; the
+            // trip count is wave-uniform and zero unless some lane opened the bucket
+            const bool opened = at && !pass;
+            const unsigned nb = __ballot(opened) != 0ull ? min(nd.count, unsigned(n) - nd.first) : 0u;
+            unsigned j = 0;
+            for (; j + 4 <= nb; j += 4) {
+                const unsigned b = nd.first + j;
+                const float4 q0 = spm[b], q1 = spm[b + 1], q2 = spm[b + 2], q3 = spm[b + 3];
+                interact_if<CUT>(opened, x, r2, q0, eps2, acc.tx, acc.ty, acc.tz);
+                interact_if<CUT>(opened, x, r2, q1, eps2, acc.tx, acc.ty, acc.tz);
+                interact_if<CUT>(opened, x, r2, q2, eps2, acc.tx, acc.ty, acc.tz);
+                interact_if<CUT>(opened, x, r2, q3, eps2, acc.tx, acc.ty, acc.tz);
+            }
+            for (; j < nb; ++j) interact_if<CUT>(opened, x, r2, spm[nd.first + j], eps2, acc.tx, acc.ty, acc.tz);
+            if (COUNT) applied += opened ? nd.count : 0u;
         }
-        for (; j < nb; ++j) interact_if<CUT>(opened, x, r2, spm[nd.first + j], eps2, acc.tx, acc.ty, acc.tz);
-        if (COUNT) applied += (opened && nd.count > 1) ? nd.count : 0u;
-        if (at) idx = max(nxt, cur + 1);  // pre-order pointers only move forward
+        // a lane that opens an internal node goes on to its first child, next in pre-order;
+        // every other lane on the node goes past its subtree (a leaf's or a bucket's
+        // subtree is itself)
+        idx = at ? (pass ? skip : step) : idx;
         // every pointer is now > cur: if a lane is on cur + 1, that is the minimum.  If not,
         // every lane that sat on cur went to the same place, past cur's subtree; when no
         // other lane waits before that point, it is the minimum too.
-        const unsigned step = cur + 1;
         if (__ballot(idx == step) != 0ull) {
             cur = step;
         } else {
-            const unsigned skip = max(nd.next, step);
             if (__ballot(idx < skip) == 0ull)
                 cur = skip;
             else
                 cur = max(wave_min(idx), step);  // the minimum is >= step; max keeps cur moving
         }
-        if ((++steps & 7u) == 0u) acc.fold();
+        clock += 0x20000000u;
+        asm("" : "+s"(clock));  // wave-uniform: a scalar counter
+        if (clock == 0u) acc.fold();
     }
 }
 
@@ -1961,7 +1988,7 @@ constexpr int walk_threads()
     return K == 8 ? 512 : 256;
 }
 
-template <bool CUT, bool STEP, bool COUNT, int K>
+template <bool CUT, bool STEP, bool COUNT, int K, bool WIDE>
 __global__ __launch_bounds__(walk_threads<K>()) void k_walk(WalkArgs a)
 {
     constexpr int GROUPS = walk_threads<K>() / 64 / K;  // 64-body groups per block
@@ -2010,7 +2037,7 @@ __global__ __launch_bounds__(walk_threads<K>()) void k_walk(WalkArgs a)
     }
     Accum acc;
     unsigned applied = 0;
-    walk_run<CUT, COUNT>(a.nodes, a.spm, a.n, a.eps2, x, r2, idx, end, acc, applied);
+    walk_run<CUT, COUNT, WIDE>(a.nodes, a.spm, a.n, a.eps2, x, r2, idx, end, acc, applied);
     if constexpr (K > 1) {
         // parts 1..K-1 hand their sums to part 0, which adds them in part order
         __shared__ float red[GROUPS][K - 1][7][64];
@@ -2067,7 +2094,7 @@ struct BatchWalkArgs {
     int n;                  // bodies in the batch
 };
 
-template <bool STEP, bool COUNT>
+template <bool STEP, bool COUNT, bool WIDE>
 __global__ __launch_bounds__(256) void k_bwalk(BatchWalkArgs a)
 {
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
@@ -2089,9 +2116,9 @@ __global__ __launch_bounds__(256) void k_bwalk(BatchWalkArgs a)
             const float r = a.svr[k].w;
             r2 = r * r;
         }
-        walk_run<true, COUNT>(a.nodes, a.spm, a.n, sy.eps2, x, r2, idx, end, acc, applied);
+        walk_run<true, COUNT, WIDE>(a.nodes, a.spm, a.n, sy.eps2, x, r2, idx, end, acc, applied);
     } else {
-        walk_run<false, COUNT>(a.nodes, a.spm, a.n, sy.eps2, x, 0.f, idx, end, acc, applied);
+        walk_run<false, COUNT, WIDE>(a.nodes, a.spm, a.n, sy.eps2, x, 0.f, idx, end, acc, applied);
     }
     if (!valid) return;
     const unsigned body = min(a.sorder[k], unsigned(sy.n - 1));
@@ -2515,22 +2542,31 @@ void bh_batch_walk(nbhip_ctx* c, const BhBatchView& v, bool fuse, bool count)
     a.nwaves = v.nwaves;
     a.n = n;
     using Walk = void (*)(BatchWalkArgs);
-    const Walk k = count ? (fuse ? (Walk)k_bwalk<true, true> : (Walk)k_bwalk<false, true>)
-                         : (fuse ? (Walk)k_bwalk<true, false> : (Walk)k_bwalk<false, false>);
+    const Walk k = walk_wide(c->bh_nodes.cap)
+                       ? (count ? (fuse ? (Walk)k_bwalk<true, true, true> : (Walk)k_bwalk<false, true, true>)
+                                : (fuse ? (Walk)k_bwalk<true, false, true> : (Walk)k_bwalk<false, false, true>))
+                       : (count ? (fuse ? (Walk)k_bwalk<true, true, false> : (Walk)k_bwalk<false, true, false>)
+                                : (fuse ? (Walk)k_bwalk<true, false, false> : (Walk)k_bwalk<false, false, false>));
     hipLaunchKernelGGL(k, dim3(unsigned((v.nwaves + 3) / 4)), dim3(256), 0, c->stream, a);
     NBHIP_CHECK(hipGetLastError());
     c->launches += 1;
 }
 
-template <int K>
+template <int K, bool WIDE>
 void (*walk_kernel(bool cut, bool fuse, bool count))(WalkArgs)
 {
     using Walk = void (*)(WalkArgs);
     if (count)
-        return cut ? (fuse ? (Walk)k_walk<true, true, true, K> : (Walk)k_walk<true, false, true, K>)
-                   : (fuse ? (Walk)k_walk<false, true, true, K> : (Walk)k_walk<false, false, true, K>);
-    return cut ? (fuse ? (Walk)k_walk<true, true, false, K> : (Walk)k_walk<true, false, false, K>)
-               : (fuse ? (Walk)k_walk<false, true, false, K> : (Walk)k_walk<false, false, false, K>);
+        return cut ? (fuse ? (Walk)k_walk<true, true, true, K, WIDE> : (Walk)k_walk<true, false, true, K, WIDE>)
+                   : (fuse ? (Walk)k_walk<false, true, true, K, WIDE> : (Walk)k_walk<false, false, true, K, WIDE>);
+    return cut ? (fuse ? (Walk)k_walk<true, true, false, K, WIDE> : (Walk)k_walk<true, false, false, K, WIDE>)
+               : (fuse ? (Walk)k_walk<false, true, false, K, WIDE> : (Walk)k_walk<false, false, false, K, WIDE>);
+}
+
+template <int K>
+void (*walk_kernel_of(bool cut, bool fuse, bool count, bool wide))(WalkArgs)
+{
+    return wide ? walk_kernel<K, true>(cut, fuse, count) : walk_kernel<K, false>(cut, fuse, count);
 }
 
 // Waves per 64-body group in the walk: the context's fixed choice, else by body count as
@@ -2571,10 +2607,11 @@ void bh_walk(nbhip_ctx* c, const nbhip_params& p, bool fuse, bool count)
     a.top = c->bh_top.ptr;
     const int split = walk_split(c, int(n));
     using Walk = void (*)(WalkArgs);
-    const Walk k = split == 8   ? walk_kernel<8>(cut, fuse, count)
-                   : split == 4 ? walk_kernel<4>(cut, fuse, count)
-                   : split == 2 ? walk_kernel<2>(cut, fuse, count)
-                                : walk_kernel<1>(cut, fuse, count);
+    const bool wide = walk_wide(c->bh_nodes.cap);
+    const Walk k = split == 8   ? walk_kernel_of<8>(cut, fuse, count, wide)
+                   : split == 4 ? walk_kernel_of<4>(cut, fuse, count, wide)
+                   : split == 2 ? walk_kernel_of<2>(cut, fuse, count, wide)
+                                : walk_kernel_of<1>(cut, fuse, count, wide);
     const int threads = split == 8 ? walk_threads<8>() : 256;
     const int bodies = threads / split;  // bodies per block
     hipLaunchKernelGGL(k, dim3(unsigned((n + size_t(bodies) - 1) / size_t(bodies))), dim3(threads), 0, c->stream, a);

@@ -35,7 +35,8 @@ def bucket_tree(keys, spm, size, theta, leaf, textbook=False, levels=21):
         return np.float32(s2 / th2) if textbook else np.float32(s2 * th2)
 
     def leaf_entry(a):
-        rows.append([a, 1, len(rows) + 1, np.float32(0.0), *spm[a, :3], spm[a, 3]])
+        # a leaf is always applied: its threshold is -inf
+        rows.append([a, 1, len(rows) + 1, np.float32(-np.inf), *spm[a, :3], spm[a, 3]])
 
     def cell(s, e):
         m = min(cpl(s, e - 1), levels)
@@ -44,7 +45,9 @@ def bucket_tree(keys, spm, size, theta, leaf, textbook=False, levels=21):
         M = w.sum()
         com = (spm[s:e, :3].astype(np.float64) * w[:, None]).sum(0) / M if M != 0 else np.zeros(3)
         bucket = e - s <= leaf or m == levels
-        rows.append([s, e - s if bucket else 0, 0, open_sq(m), *com, M])
+        # a cell of zero mass (as FP32) is never opened: its threshold is -inf
+        thr = open_sq(m) if np.float32(M) != 0 else np.float32(-np.inf)
+        rows.append([s, e - s if bucket else 0, 0, thr, *com, M])
         if not bucket:
             shift = 60 - 3 * m
             a = s
