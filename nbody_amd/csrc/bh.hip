@@ -1857,7 +1857,25 @@ __device__ __forceinline__ BhNode load_node(const BhNode* __restrict__ nodes, un
     return *reinterpret_cast<const BhNode*>(reinterpret_cast<const char*>(nodes) + (i << 5));
 }
 
-// Whether a node array of `capacity` entries needs 64-bit node addressing in the walk.
+// Bodies of an opened bucket, by the same 32-bit byte offsets (the node capacity is above
+// twice the body count, so 16 bytes a body stay below 4 GiB too), four in one load.
+struct Body4 {
+    float4 b[4];
+};
+template <bool WIDE>
+__device__ __forceinline__ float4 load_body(const float4* __restrict__ spm, unsigned i)
+{
+    if constexpr (WIDE) return spm[i];
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(spm) + (i << 4));
+}
+template <bool WIDE>
+__device__ __forceinline__ Body4 load_bodies4(const float4* __restrict__ spm, unsigned i)
+{
+    if constexpr (WIDE) return Body4{ { spm[i], spm[i + 1], spm[i + 2], spm[i + 3] } };
+    return *reinterpret_cast<const Body4*>(reinterpret_cast<const char*>(spm) + (i << 4));
+}
+
+// Whether a node array of `capacity` entries needs 64-bit addressing in the walk.
 inline bool walk_wide(size_t capacity)
 {
     return capacity > (size_t(1) << 27);
@@ -1912,20 +1930,22 @@ __device__ __forceinline__ void walk_run(const BhNode* __restrict__ nodes, const
             const unsigned nb = __ballot(opened) != 0ull ? min(nd.count, unsigned(n) - nd.first) : 0u;
             unsigned j = 0;
             for (; j + 4 <= nb; j += 4) {
-                const unsigned b = nd.first + j;
-                const float4 q0 = spm[b], q1 = spm[b + 1], q2 = spm[b + 2], q3 = spm[b + 3];
-                interact_if<CUT>(opened, x, r2, q0, eps2, acc.tx, acc.ty, acc.tz);
-                interact_if<CUT>(opened, x, r2, q1, eps2, acc.tx, acc.ty, acc.tz);
-                interact_if<CUT>(opened, x, r2, q2, eps2, acc.tx, acc.ty, acc.tz);
-                interact_if<CUT>(opened, x, r2, q3, eps2, acc.tx, acc.ty, acc.tz);
+                const Body4 q = load_bodies4<WIDE>(spm, nd.first + j);
+                interact_if<CUT>(opened, x, r2, q.b[0], eps2, acc.tx, acc.ty, acc.tz);
+                interact_if<CUT>(opened, x, r2, q.b[1], eps2, acc.tx, acc.ty, acc.tz);
+                interact_if<CUT>(opened, x, r2, q.b[2], eps2, acc.tx, acc.ty, acc.tz);
+                interact_if<CUT>(opened, x, r2, q.b[3], eps2, acc.tx, acc.ty, acc.tz);
             }
-            for (; j < nb; ++j) interact_if<CUT>(opened, x, r2, spm[nd.first + j], eps2, acc.tx, acc.ty, acc.tz);
+            for (; j < nb; ++j)
+                interact_if<CUT>(opened, x, r2, load_body<WIDE>(spm, nd.first + j), eps2, acc.tx, acc.ty, acc.tz);
             if (COUNT) applied += opened ? nd.count : 0u;
         }
         // a lane that opens an internal node goes on to its first child, next in pre-order;
         // every other lane on the node goes past its subtree (a leaf's or a bucket's
-        // subtree is itself)
-        idx = at ? (pass ? skip : step) : idx;
+        // subtree is itself).  Lanes not on the node already point past it, at step or
+        // beyond, so one maximum moves every lane on it to step.
+        idx = max(idx, step);
+        if (at && pass) idx = skip;
         // every pointer is now > cur: if a lane is on cur + 1, that is the minimum.  If not,
         // every lane that sat on cur went to the same place, past cur's subtree; when no
         // other lane waits before that point, it is the minimum too.
