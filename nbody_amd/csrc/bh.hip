@@ -2596,7 +2596,7 @@ int walk_split(const nbhip_ctx* c, int n)
 {
     if (c->bh_split > 0) return c->bh_split;
     if (n < (1 << 17)) return 8;
-    if (n < (1 << 20)) return 4;
+    if (n < (1 << 18)) return 4;
     return 2;
 }
 
