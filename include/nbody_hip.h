@@ -526,7 +526,9 @@ int nbhip_bh_batch_add_generated(nbhip_bh_batch* b, const nbhip_ics* spec, size_
 int nbhip_bh_batch_remove(nbhip_bh_batch* b, int id);
 
 /* Advance every system `steps` steps (flags: NBHIP_BATCH_GRAPH, NBHIP_BATCH_ACCELERATE,
- * NBHIP_BATCH_COUNT with ACCELERATE).  Enqueues only on the batch's stream. */
+ * NBHIP_BATCH_COUNT with ACCELERATE).  Enqueues only on the batch's stream.  With
+ * NBHIP_BATCH_GRAPH one step is captured (again after a membership change) and replayed
+ * `steps` times. */
 int nbhip_bh_batch_step(nbhip_bh_batch* b, int steps, unsigned flags);
 
 /* Non-blocking: *idle = 1 when all work enqueued on the batch has finished, else 0. */

@@ -5,10 +5,10 @@
 // The batch keeps its systems' bodies back to back in the arrays of a context of its own
 // (its stream, its build buffers): system s at positions [off_s, off_s + n_s), in the order
 // of their ids, which is also the order of their indices in the keys.  A system joins at
-// the end of the arrays; when one leaves, the ones after it move down (into the spare
-// arrays, which then become the current ones).  The device tables -- one entry per system,
+// the end of the arrays; when one leaves, the ones after it move down (through the spare
+// arrays).  The device tables -- one entry per system,
 // per 1024-position scan tile of a system, per 64-body walk group of a system -- are made
-// again whenever the membership changes, and a captured step sequence with them.  A walk
+// again whenever the membership changes, and a captured step with them.  A walk
 // leaves each system's bodies in the key order of its build; the batch's order array holds,
 // at each position, the body's index within its system, which a download puts back.
 #include <algorithm>
@@ -54,10 +54,9 @@ struct nbhip_bh_batch {
     nbhip::DevBuf<float4> spare_vr, spare_acc;
     nbhip::DevBuf<unsigned> spare_order;
 
-    // captured step sequence
+    // one captured step
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int g_steps = -1;
     unsigned g_flags = 0;
     long g_version = -1;
     unsigned long long g_epoch = 0;
@@ -68,7 +67,6 @@ struct nbhip_bh_batch {
         if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;
         graph = nullptr;
-        g_steps = -1;
     }
     ~nbhip_bh_batch()
     {
@@ -334,38 +332,36 @@ int nbhip_bh_batch_add_generated(nbhip_bh_batch* b, const nbhip_ics* spec, size_
 int nbhip_bh_batch_remove(nbhip_bh_batch* b, int id)
 {
     return nbhip::bh_batch_guarded(b, [&] {
-        (void)nbhip::find(b, id);
+        const nbhip::BhBatchSystem gone = nbhip::find(b, id);
         nbhip_ctx* w = b->work;
         hipStream_t s = w->stream;
         NBHIP_CHECK(hipStreamSynchronize(s));
-        b->systems.erase(id);
-        size_t total = 0;
-        for (const auto& kv : b->systems) total += kv.second.n;
-        // every remaining system into the spare arrays, back to back in id order
-        w->pm[1].grow(std::max<size_t>(1, total), 0, s);
-        b->spare_vr.grow(std::max<size_t>(1, total), 0, s);
-        b->spare_acc.grow(std::max<size_t>(1, total), 0, s);
-        b->spare_order.grow(std::max<size_t>(1, total), 0, s);
-        size_t off = 0;
-        for (auto& kv : b->systems) {
-            nbhip::BhBatchSystem& sy = kv.second;
-            const size_t bytes = sy.n * sizeof(float4);
-            NBHIP_CHECK(hipMemcpyAsync(w->pm[1].ptr + off, w->pm[0].ptr + sy.off, bytes, hipMemcpyDeviceToDevice, s));
-            NBHIP_CHECK(hipMemcpyAsync(b->spare_vr.ptr + off, w->vr.ptr + sy.off, bytes, hipMemcpyDeviceToDevice, s));
-            NBHIP_CHECK(hipMemcpyAsync(b->spare_acc.ptr + off, w->acc.ptr + sy.off, bytes, hipMemcpyDeviceToDevice, s));
-            NBHIP_CHECK(hipMemcpyAsync(b->spare_order.ptr + off, w->order.ptr + sy.off, sy.n * sizeof(unsigned),
-                                       hipMemcpyDeviceToDevice, s));
-            sy.off = off;
-            off += sy.n;
+        // the systems after it, one contiguous run of positions, move down by its size:
+        // through the spare arrays (the run may overlap its new place), four copies each way
+        const size_t from = gone.off + gone.n, len = w->count - from;
+        if (len > 0) {
+            w->pm[1].grow(len, 0, s);
+            b->spare_vr.grow(len, 0, s);
+            b->spare_acc.grow(len, 0, s);
+            b->spare_order.grow(len, 0, s);
+            const size_t f4 = len * sizeof(float4), u = len * sizeof(unsigned);
+            NBHIP_CHECK(hipMemcpyAsync(w->pm[1].ptr, w->pm[0].ptr + from, f4, hipMemcpyDeviceToDevice, s));
+            NBHIP_CHECK(hipMemcpyAsync(b->spare_vr.ptr, w->vr.ptr + from, f4, hipMemcpyDeviceToDevice, s));
+            NBHIP_CHECK(hipMemcpyAsync(b->spare_acc.ptr, w->acc.ptr + from, f4, hipMemcpyDeviceToDevice, s));
+            NBHIP_CHECK(hipMemcpyAsync(b->spare_order.ptr, w->order.ptr + from, u, hipMemcpyDeviceToDevice, s));
+            NBHIP_CHECK(hipMemcpyAsync(w->pm[0].ptr + gone.off, w->pm[1].ptr, f4, hipMemcpyDeviceToDevice, s));
+            NBHIP_CHECK(hipMemcpyAsync(w->vr.ptr + gone.off, b->spare_vr.ptr, f4, hipMemcpyDeviceToDevice, s));
+            NBHIP_CHECK(hipMemcpyAsync(w->acc.ptr + gone.off, b->spare_acc.ptr, f4, hipMemcpyDeviceToDevice, s));
+            NBHIP_CHECK(hipMemcpyAsync(w->order.ptr + gone.off, b->spare_order.ptr, u, hipMemcpyDeviceToDevice, s));
+            NBHIP_CHECK(hipStreamSynchronize(s));
         }
-        NBHIP_CHECK(hipStreamSynchronize(s));
-        std::swap(w->pm[0], w->pm[1]);
-        std::swap(w->vr, b->spare_vr);
-        std::swap(w->acc, b->spare_acc);
-        std::swap(w->order, b->spare_order);
-        nbhip::g_buffer_epoch.fetch_add(1);
-        w->count = total;
-        nbhip::bh_sort_invalidate(w);
+        b->systems.erase(id);
+        for (auto& kv : b->systems)
+            if (kv.second.off > gone.off) kv.second.off -= gone.n;
+        w->count -= gone.n;
+        // No sort statistics are reset: the remaining bodies keep their keys' order (a
+        // system's index in the keys only drops with its place), so no run of equal key
+        // prefixes grows, and the partial sort the last sort chose still fits.
         b->dirty = true;
     });
 }
@@ -388,14 +384,16 @@ int nbhip_bh_batch_step(nbhip_bh_batch* b, int steps, unsigned flags)
         nbhip::bh_batch_reserve(w, v);  // every allocation before a capture
         if (count) w->bh_visits.grow(w->count, 0, s);
         if (flags & NBHIP_BATCH_GRAPH) {
+            // One step is captured and replayed `steps` times: every step launches the same
+            // kernels on the same arrays, so a chunk of any length reuses the graph, and a
+            // membership change captures one step's ~20 launches, not a whole chunk's.
             const unsigned long long epoch = nbhip::g_buffer_epoch.load();
-            const bool reuse = b->exec && b->g_steps == steps && b->g_flags == flags && b->g_version == b->version &&
-                               b->g_epoch == epoch;
+            const bool reuse = b->exec && b->g_flags == flags && b->g_version == b->version && b->g_epoch == epoch;
             if (!reuse) {
                 b->drop_graph();
                 NBHIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 try {
-                    nbhip::enqueue_steps(b, v, steps, fuse, false);
+                    nbhip::enqueue_steps(b, v, 1, fuse, false);
                 } catch (...) {
                     hipGraph_t g = nullptr;
                     (void)hipStreamEndCapture(s, &g);
@@ -404,12 +402,11 @@ int nbhip_bh_batch_step(nbhip_bh_batch* b, int steps, unsigned flags)
                 }
                 NBHIP_CHECK(hipStreamEndCapture(s, &b->graph));
                 NBHIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-                b->g_steps = steps;
                 b->g_flags = flags;
                 b->g_version = b->version;
                 b->g_epoch = epoch;
             }
-            NBHIP_CHECK(hipGraphLaunch(b->exec, s));
+            for (int k = 0; k < steps; ++k) NBHIP_CHECK(hipGraphLaunch(b->exec, s));
         } else {
             nbhip::enqueue_steps(b, v, steps, fuse, count);
         }
