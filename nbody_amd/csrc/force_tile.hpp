@@ -20,6 +20,8 @@
 // fixed order through LDS, so results do not depend on timing or on other work in
 // flight.
 #pragma once
+
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 namespace nbhip {
@@ -195,18 +197,24 @@ __device__ __forceinline__ void force_tile(ForceShared<P, W>& sh, const float4* 
 // a wave meet the same visitor in the same step.  Each pair is evaluated once: one
 // |d|^2 + eps^2, one v_rsq_f32, and the force goes both ways -- onto the home (its
 // registers) and, with the opposite sign, onto the visitor (the wave's LDS row for that
-// visitor, a plain read-modify-write: one lane per slot per step, in step order).  The W
-// waves' rows for a slice are then added in wave order.  A diagonal tile (a == b) takes
-// the home side only, every pair within the block met from both ends.
+// visitor, a plain read-modify-write: one lane per slot per step, in step order).  The
+// slice's visitors are stored twice over (entries l and l + 64), so lane l's visitor at
+// step k is entry 64 + l - k and every address in the step loop is a fixed offset from the
+// lane's own; likewise each visitor has two row slots, v and v + 64, zeroed when the slice
+// starts, and lane l adds into slot 64 + l - k.  (An LDS float atomic add in place of the
+// read-modify-write measured ten times slower.)  The W waves' rows for a slice are then
+// added in wave order, each wave's two slots in turn.  A diagonal tile (a == b) takes the home side only, every pair within the
+// block met from both ends.
 // Every body thus receives one partial sum per other block (slot b for its home side in
 // tile (block, b), slot a for its visitor side in tile (a, block)), written exactly once
 // into partial[slot][3][n]; sym_slot_sum adds a body's M slots in slot order.  Each cutoff
 // applies to its receiving body (home: its radius; visitor: its radius).
 template <int W>
 struct SymShared {
-    float4 vis[2][WAVE];       // visitors of the slice in use and of the next one
-    float vr2[2][WAVE];        // their squared radii (cutoff variant)
-    float row[2][W][3][WAVE];  // each wave's visitor sums: the slice in use, and the one before
+    float4 vis[2][2 * WAVE];       // visitors of the slice in use and of the next one, twice over
+    float vr2[2][2 * WAVE];        // their squared radii (cutoff variant)
+    float row[2][W][3][2 * WAVE];  // each wave's visitor sums (two slots a visitor): the slice
+                                   // in use, and the one before
 };
 
 // Tile (ta, tb) of one system: bodies pm[0, n), blocks of 64*P*W.  Must be called by every
@@ -246,6 +254,14 @@ __device__ __forceinline__ void sym_tile(SymShared<W>& sh, const float4* __restr
             r = rad * rad;
         }
     };
+    const auto put_visitor = [&](int buf, const float4 q, const float r) {
+        sh.vis[buf][lane] = q;
+        sh.vis[buf][lane + WAVE] = q;
+        if (CUT) {
+            sh.vr2[buf][lane] = r;
+            sh.vr2[buf][lane + WAVE] = r;
+        }
+    };
     // wave `sl % W` adds the W rows of slice sl in wave order and writes its visitors' slot
     const auto fold = [&](int sl) {
         if (wave != sl % W) return;
@@ -254,9 +270,9 @@ __device__ __forceinline__ void sym_tile(SymShared<W>& sh, const float4* __restr
         const int buf = sl & 1;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            float sum = sh.row[buf][0][k][lane];
+            float sum = sh.row[buf][0][k][lane] + sh.row[buf][0][k][lane + WAVE];
 #pragma unroll
-            for (int w = 1; w < W; ++w) sum += sh.row[buf][w][k][lane];
+            for (int w = 1; w < W; ++w) sum += sh.row[buf][w][k][lane] + sh.row[buf][w][k][lane + WAVE];
             partial[(size_t(ta) * 3 + k) * size_t(n) + vj] = sum;
         }
     };
@@ -264,8 +280,7 @@ __device__ __forceinline__ void sym_tile(SymShared<W>& sh, const float4* __restr
         float4 q;
         float r;
         load_visitor(0, q, r);
-        sh.vis[0][lane] = q;
-        sh.vr2[0][lane] = r;
+        put_visitor(0, q, r);
     }
     for (int sl = 0; sl < slices; ++sl) {
         const int buf = sl & 1;
@@ -274,63 +289,83 @@ __device__ __forceinline__ void sym_tile(SymShared<W>& sh, const float4* __restr
         float4 qn = make_float4(0.f, 0.f, 0.f, 0.f);
         float rn = 0.f;
         if (wave == 0 && sl + 1 < slices) load_visitor(sl + 1, qn, rn);  // in flight during the slice
+        // this wave's rows of slice sl start at zero (slice sl - 2 used them; it was folded
+        // before this slice's barrier)
         float* rw = &sh.row[buf][wave][0][0];
-        // Step k: lane l meets visitor v = (l - k) mod 64.  Its row slot was written by lane
-        // l - 1 in step k - 1 (at step 0 each lane starts its own slot); the compiler sees no
-        // dependence across lanes, so a compiler-only fence keeps each step's row accesses
-        // after the previous step's, and LDS runs a wave's accesses in program order.  The
-        // next visitor and this step's row sums are read before the arithmetic.
-        int v = lane;
-        float4 q = sh.vis[buf][v];
-        float rv2 = CUT ? sh.vr2[buf][v] : 0.f;
-#pragma unroll 2
-        for (int k = 0; k < WAVE; ++k) {
-            const int vn = (lane - k - 1) & (WAVE - 1);
-            const float4 qn_step = sh.vis[buf][vn];
-            const float rn_step = CUT ? sh.vr2[buf][vn] : 0.f;
-            float ox = 0.f, oy = 0.f, oz = 0.f;
-            if (!diag && k > 0) {
-                ox = rw[0 * WAVE + v];
-                oy = rw[1 * WAVE + v];
-                oz = rw[2 * WAVE + v];
-            }
-            float vx = 0.f, vy = 0.f, vz = 0.f;
+        if (!diag) {
 #pragma unroll
-            for (int p = 0; p < P; ++p) {
-                const float dx = q.x - x[p].x;
-                const float dy = q.y - x[p].y;
-                const float dz = q.z - x[p].z;
-                float wh, wv;
-                if (CUT) {
-                    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
-                    const float i3 = inv * inv * inv;
-                    wh = d2 > r2[p] ? q.w * i3 : 0.f;
-                    wv = d2 > rv2 ? x[p].w * i3 : 0.f;
-                } else {
-                    const float e2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-                    const float inv = __builtin_amdgcn_rsqf(e2);
-                    const float i3 = inv * inv * inv;
-                    wh = q.w * i3;
-                    wv = x[p].w * i3;
-                }
-                T[p][0] = fmaf(wh, dx, T[p][0]);
-                T[p][1] = fmaf(wh, dy, T[p][1]);
-                T[p][2] = fmaf(wh, dz, T[p][2]);
-                vx = fmaf(-wv, dx, vx);
-                vy = fmaf(-wv, dy, vy);
-                vz = fmaf(-wv, dz, vz);
+            for (int k = 0; k < 3; ++k) {
+                rw[k * 2 * WAVE + lane] = 0.f;
+                rw[k * 2 * WAVE + lane + WAVE] = 0.f;
             }
-            if (!diag) {
-                rw[0 * WAVE + v] = ox + vx;
-                rw[1 * WAVE + v] = oy + vy;
-                rw[2 * WAVE + v] = oz + vz;
-            }
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            v = vn;
-            q = qn_step;
-            rv2 = rn_step;
         }
+        const float4* vis = &sh.vis[buf][WAVE + lane];
+        const float* vr2 = &sh.vr2[buf][WAVE + lane];
+        float* slot = rw + WAVE + lane;
+        // Step k: lane l meets visitor 64 + l - k (of the doubled slice) and adds into row
+        // slot 64 + l - k, which lane l - 1 wrote in step k - 1 (or the zeroing did); the
+        // compiler sees no dependence across lanes, so a compiler-only fence keeps each
+        // step's row accesses after the previous step's, and LDS runs a wave's accesses in
+        // program order.  The next visitor and this step's row sums are read before the
+        // arithmetic.
+        // the step loop, with the visitor rows (ROWS) or without them (a diagonal tile)
+        const auto run = [&](auto rows_tag) {
+            constexpr bool ROWS = decltype(rows_tag)::value;
+            float4 q = vis[0];
+            float rv2 = CUT ? vr2[0] : 0.f;
+#pragma unroll 2
+            for (int k = 0; k < WAVE; ++k) {
+                const float4 qn_step = vis[-k - 1];  // entry 64 + l - 64 = l >= 0 at the last step
+                const float rn_step = CUT ? vr2[-k - 1] : 0.f;
+                float ox, oy, oz;
+                if (ROWS) {
+                    ox = slot[0 * 2 * WAVE - k];
+                    oy = slot[1 * 2 * WAVE - k];
+                    oz = slot[2 * 2 * WAVE - k];
+                }
+                float vx = 0.f, vy = 0.f, vz = 0.f;
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    const float dx = q.x - x[p].x;
+                    const float dy = q.y - x[p].y;
+                    const float dz = q.z - x[p].z;
+                    float wh, wv;
+                    if (CUT) {
+                        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
+                        const float i3 = inv * inv * inv;
+                        wh = d2 > r2[p] ? q.w * i3 : 0.f;
+                        wv = d2 > rv2 ? x[p].w * i3 : 0.f;
+                    } else {
+                        const float e2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+                        const float inv = __builtin_amdgcn_rsqf(e2);
+                        const float i3 = inv * inv * inv;
+                        wh = q.w * i3;
+                        wv = x[p].w * i3;
+                    }
+                    T[p][0] = fmaf(wh, dx, T[p][0]);
+                    T[p][1] = fmaf(wh, dy, T[p][1]);
+                    T[p][2] = fmaf(wh, dz, T[p][2]);
+                    if (ROWS) {
+                        vx = fmaf(-wv, dx, vx);
+                        vy = fmaf(-wv, dy, vy);
+                        vz = fmaf(-wv, dz, vz);
+                    }
+                }
+                if (ROWS) {
+                    slot[0 * 2 * WAVE - k] = ox + vx;
+                    slot[1 * 2 * WAVE - k] = oy + vy;
+                    slot[2 * 2 * WAVE - k] = oz + vz;
+                }
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                q = qn_step;
+                rv2 = rn_step;
+            }
+        };
+        if (diag)
+            run(std::false_type{});
+        else
+            run(std::true_type{});
 #pragma unroll
         for (int p = 0; p < P; ++p)
 #pragma unroll
@@ -338,10 +373,8 @@ __device__ __forceinline__ void sym_tile(SymShared<W>& sh, const float4* __restr
                 two_sum(S[p][k], C[p][k], T[p][k]);
                 T[p][k] = 0.f;
             }
-        if (wave == 0 && sl + 1 < slices) {
-            sh.vis[buf ^ 1][lane] = qn;  // last read in slice sl - 1, before this slice's barrier
-            sh.vr2[buf ^ 1][lane] = rn;
-        }
+        if (wave == 0 && sl + 1 < slices)
+            put_visitor(buf ^ 1, qn, rn);  // last read in slice sl - 1, before this slice's barrier
     }
     if (!diag) {
         __syncthreads();
