@@ -122,6 +122,31 @@ def test_graph_replays_equal_eager_steps(ctx):
                 assert np.array_equal(a.download(x), g.download(y))
 
 
+def test_graph_chunks_of_any_length_equal_eager(ctx):
+    # one step is captured and replayed once per step of a chunk: chunks of 3, 1 and 4
+    # steps reuse it, a removal (the systems after the removed one move down) captures it
+    # again; every chunk equals eager stepping bit for bit
+    sys_ = systems()[:5]
+    with _lib.BhBatch(ctx) as a, _lib.BhBatch(ctx) as g:
+        ia = {name: a.add(b, p) for name, b, p in sys_}
+        ig = {name: g.add(b, p) for name, b, p in sys_}
+
+        def same():
+            for name in ia:
+                assert np.array_equal(a.download(ia[name]), g.download(ig[name])), name
+
+        for k in (3, 1, 4):
+            a.step(k)
+            g.step(k, _lib.BATCH_GRAPH)
+            same()
+        a.remove(ia.pop("disk_wrap"))
+        g.remove(ig.pop("disk_wrap"))
+        for k in (2, 5):
+            a.step(k)
+            g.step(k, _lib.BATCH_GRAPH)
+            same()
+
+
 def keys18_unique(ctx, p):
     keys, _ = ctx.bh_keys(p)
     k18 = keys >> np.uint64(9)
@@ -236,7 +261,8 @@ def test_many_one_body_systems_and_limits(ctx):
 def test_partial_sorts_leave_the_same_bodies(ctx, monkeypatch):
     # The batch's one sort over every system's keys may skip low-digit passes (fixing the
     # runs they leave): every fixed skip and the adaptive choice give the full sort's steps,
-    # bit for bit, across a join and a removal (each of which makes the next sort a full one)
+    # bit for bit, across a join (which makes the next sort a full one) and a removal (after
+    # which the last sort's choice stands: the remaining keys keep their order)
     members = systems()
     res = {}
     for skip in ("0", "2", "4", ""):
