@@ -27,7 +27,7 @@ import os
 import numpy as np
 import pytest
 
-from bh_definition import bucket_tree
+from bh_definition import bucket_tree, walk
 from nbody_amd import _lib, ics
 
 pytestmark = pytest.mark.gpu
@@ -529,6 +529,58 @@ def test_bucket_forces_match_bucket_oracle(ctx, orc, n, leaf, kind, opening):
           f"99.9% {np.quantile(e[eq], 0.999):.2e} max {e[eq].max():.2e}")
     assert total == visits.sum()
     assert eq.mean() >= (0.999 if kind == "clumps" else 0.9999)
+    assert np.quantile(e[eq], 0.999) <= 1e-5 and e[eq].max() <= 1e-4
+
+
+@pytest.mark.parametrize("leaf,eps", [(1, 0.01), (16, 0.01), (16, 0.0)])
+def test_massless_bodies(ctx, orc, leaf, eps):
+    # Bodies of zero mass, scattered and filling whole cells.  The library's tree keeps
+    # every body and skips every node of zero mass (bhtree.cpp:108-113): leaves and empty
+    # nodes carry the threshold -inf, so an empty node always passes and adds exactly zero
+    # (its weight is zero; d^2 + eps^2 > 0 wherever the radius cutoff does not remove
+    # d^2 = 0), and the interaction count skips it.  Against the tree's definition
+    # (tests/bh_definition.py): the same nodes bit for bit, >= 99.99% identical per-body
+    # visit counts, those bodies within 1e-5 (99.9%) / 1e-4 (all), every force finite.
+    # The reference's own insert treats a leaf of zero mass as empty (bhtree.cpp:26-31):
+    # the next body landing there takes its place, while a massless body landing on a
+    # populated leaf splits it, so its tree's shape depends on the insertion order.
+    # Against the oracle, which restates that tree, the forces keep the same bars on the
+    # bodies whose traversal is the same (>= 98% of them here; the others differ only in
+    # where cells split, and the massless bodies themselves pull nothing either way).
+    n, size, theta = 8000, 64.0, 0.5
+    b = ics.plummer(n, seed=leaf + 17)
+    b[::5, 7] = 0.0
+    b[b[:, 0] > 1.2, 7] = 0.0
+    assert 0 < np.count_nonzero(b[:, 7]) < n
+    ctx.upload_bodies(b)
+    p = bh_params(size=size, theta=theta, eps=eps, leaf_size=leaf)
+    nodes = ctx.bh_nodes(p)
+    keys, perm = ctx.bh_keys(p)
+    spm = b[perm][:, [0, 1, 2, 7]]
+    rows = bucket_tree(np.sort(keys), spm, size, theta, leaf)
+    thr = np.array([r[3] for r in rows], np.float32)
+    assert np.array_equal(nodes["open_sq"].view(np.uint32), thr.view(np.uint32))
+    empty = nodes["mass"] == 0
+    assert np.any(empty & (nodes["count"] == 0))  # whole internal cells of zero mass
+    assert np.all(nodes["open_sq"][empty] == -np.inf)
+    visits, total = ctx.bh_interactions(p)
+    out = ctx.download_bodies()
+    assert total == visits.sum()
+    assert np.all(np.isfinite(out[:, 8:11]))
+    acc_d, vis_d = walk(rows, spm, eps=eps)
+    e = rel_err(out[perm, 8:11], acc_d)
+    eq = visits[perm] == vis_d
+    print(f"massless leaf={leaf} eps={eps} vs definition: same traversal {eq.mean():.5f}, "
+          f"99.9% {np.quantile(e[eq], 0.999):.2e} max {e[eq].max():.2e}")
+    assert eq.mean() >= 0.9999
+    assert np.quantile(e[eq], 0.999) <= 1e-5 and e[eq].max() <= 1e-4
+    x = b.copy()
+    _, xv = orc.bh_accelerate_buckets(x, size, theta=theta, leaf_size=leaf, eps=eps)
+    e = rel_err(out[:, 8:11], x[:, 8:11])
+    eq = visits == xv
+    print(f"massless leaf={leaf} eps={eps} vs oracle: same traversal {eq.mean():.5f}, "
+          f"99.9% {np.quantile(e[eq], 0.999):.2e} max {e[eq].max():.2e}")
+    assert eq.mean() >= 0.98
     assert np.quantile(e[eq], 0.999) <= 1e-5 and e[eq].max() <= 1e-4
 
 
