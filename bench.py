@@ -43,7 +43,7 @@ N_C2 = 262144
 EPS_C2 = 0.005
 DT_C2 = 5e-4
 SEED = 2026
-CPU_THREADS_MAX = 16              # the GPU box's CPU share for one GPU
+CPU_THREADS_MAX = 16              # the host CPU share of one GPU on the benchmark hosts
 CPU_TARGET_S = 10.0
 HBM_PEAK_GBS = 8000.0              # MI355X published HBM3E bandwidth
 

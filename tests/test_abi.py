@@ -38,7 +38,7 @@ def test_default_params_are_the_reference_defaults():
 
 
 def test_no_device_is_reported_not_raised():
-    # In a container without a GPU the probe reports a reason; on a GPU box it succeeds.
+    # On a host without a GPU the probe reports a reason; with a gfx950 device it succeeds.
     from nbody_amd import _lib
     ok, reason = _lib.probe()
     assert ok or reason
